@@ -1,0 +1,180 @@
+"""Benchmark of the 2D Euler FR hot path on one MI355X.
+
+Workload: 2D Euler (gamma = 1.4) on the periodic square [0,10]^2, 256 x 256
+uniform quads, p = 3 (16 Gauss points per element, 4,194,304 DOF), isentropic vortex (strength 5,
+centred) on rho = 1, u = v = 1, p = 1, Rusanov flux, SSP-RK3 at CFL 0.05 on the wave speed
+(dt = 0.05 dx / max(|u|+c, |v|+c) = 2/2999), FP64.
+One step = one SSP-RK3 step = three fused residual+stage launches on the device-resident state;
+the default step count (2999) is the full run to t = 2.
+
+value = DOF * RK stages / s over the timed steps.  roofline: fused stage kernel, algorithmic bytes
+per launch = 21.33 B/DOF (stage 1 reads u and writes u1: 16 B/DOF; stages 2-3 read u_s and u^n and
+write: 24 B/DOF) x DOF, divided by the average launch duration measured with HIP events on the
+stream the kernels run on.  cpu_baseline: the CPU oracle (C/OpenMP restatement of the reference
+dudt!) timed on this host over a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "fluxreconstruction.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = "DOF·RK-stage/s 2D Euler p=3 (%HBM peak); learned vs modal sensor error & cost"
+HBM_PEAK_GBS = 8000.0
+NX = NY = 256
+DEG = 3
+GAMMA = 1.4
+CFL = 0.05
+
+
+def cpu_baseline(u0, dt, budget_s=12.0):
+    """Oracle SSP-RK3 steps on the host cores for about budget_s seconds."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    orc = O.Fr2d(NX, NY, DEG, 10.0 / NX, 10.0 / NY, GAMMA, "rusanov", "periodic")
+    u = np.array(u0, order="F", copy=True)
+    t0 = time.perf_counter()
+    orc.ssprk3(u, dt, 1)
+    one = time.perf_counter() - t0
+    n = max(1, int(budget_s / max(one, 1e-3)))
+    t0 = time.perf_counter()
+    orc.ssprk3(u, dt, n)
+    el = time.perf_counter() - t0
+    dof = NX * NY * (DEG + 1) ** 2 * 4
+    return {
+        "value": dof * 3 * n / el,
+        "unit": "DOF·RK-stage/s",
+        "cores": int(O.lib().or_num_threads()),
+        "kind": "port",
+        "sample": f"{n} SSP-RK3 steps ({3 * n} residual stages) of the same 256x256 p=3 vortex case, "
+                  f"C/OpenMP oracle, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the full run to t = 2)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import frhip
+    from frhip import ics
+
+    ps = frhip.FRPSpace2D(0.0, 10.0, NX, 0.0, 10.0, NY, DEG, 1, 1)
+    u0 = ics.isentropic_vortex(ps, GAMMA, strength=5.0, center=(5.0, 5.0))
+    dt, n_full = ics.cfl_steps(u0, GAMMA, ps.dx0, CFL, 2.0)
+    if args.steps is None:
+        args.steps = n_full
+    dof = NX * NY * (DEG + 1) ** 2 * 4
+
+    stream = torch.cuda.Stream()
+    solver = frhip.FREuler2D(ps, GAMMA, "rusanov", "periodic")
+    solver.use_torch_stream(stream)
+    solver.set_state(u0)
+    with torch.cuda.stream(stream):
+        solver.step(dt, args.warmup)
+        torch.cuda.synchronize()
+        solver.set_state(u0)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        solver.step(dt, args.steps)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1)
+    uf = solver.get_state()
+    solver.check_state()
+    launches = 3 * args.steps
+    per_launch_s = ms * 1e-3 / launches
+    alg_bytes = (16.0 + 24.0 + 24.0) / 3.0 * dof
+    achieved = alg_bytes / per_launch_s / 1e9
+    value = dof * launches / (ms * 1e-3)
+
+    # residual-only drop-in dudt! on device arrays (16 B/DOF)
+    ud = torch.from_numpy(uf.ravel(order="F").copy()).cuda()
+    dud = torch.empty_like(ud)
+    with torch.cuda.stream(stream):
+        for _ in range(5):
+            solver.dudt(dud, ud)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        nres = 200
+        e0.record(stream)
+        for _ in range(nres):
+            solver.dudt(dud, ud)
+        e1.record(stream)
+        torch.cuda.synchronize()
+    res_s = e0.elapsed_time(e1) * 1e-3 / nres
+
+    cpu = None if args.no_cpu else cpu_baseline(u0, dt)
+    mass0 = float(np.einsum("ijkl,kl->", u0[1:-1, 1:-1, :, :, 0], ps.wp))
+    mass1 = float(np.einsum("ijkl,kl->", uf[1:-1, 1:-1, :, :, 0], ps.wp))
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "DOF·RK-stage/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "2D Euler isentropic vortex, [0,10]^2 periodic, 256x256 quads, p=3, Rusanov, "
+                        "SSP-RK3 CFL 0.05 (dt = 0.05 dx / max(|u|+c,|v|+c)), fused residual+stage kernel, "
+                        "device-resident state",
+            "dof": dof,
+            "dt": dt,
+            "t_end": dt * args.steps,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "fr2d_kernel<4,RUSANOV,STAGE>",
+            "avg_launch_us": per_launch_s * 1e6,
+            "alg_bytes_per_launch": alg_bytes,
+        },
+        "cpu_baseline": cpu,
+        "residual_only": {
+            "kernel": "fr2d_kernel<4,RUSANOV,RESIDUAL>",
+            "avg_launch_us": res_s * 1e6,
+            "dof_per_s": dof / res_s,
+            "hbm_gbs_alg": 16.0 * dof / res_s / 1e9,
+        },
+        "wall_s_timed": wall,
+        "mass_drift_rel": abs(mass1 - mass0) / abs(mass0),
+    }
+    if cpu:
+        out["gpu_over_cpu"] = value / cpu["value"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

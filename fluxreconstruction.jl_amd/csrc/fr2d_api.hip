@@ -1,0 +1,361 @@
+// fr2d_api.hip -- C ABI of the 2D Euler solver: context, drop-in residual, device-resident
+// integrator, dissipation calls (declared in include/fr.h).
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "fr2d_internal.h"
+
+namespace fr {
+
+
+// layout of Fr2dCtx::d_dops: V (np^2) | iV (np^2) | wp/4 (np) | ll (nsp) | lr (nsp)
+DissOps fr2d_diss_ops(const Fr2dCtx &c) {
+  const int np = c.nsp * c.nsp;
+  DissOps o;
+  o.V = c.d_dops;
+  o.iV = c.d_dops + np * np;
+  o.wts = c.d_dops + 2 * np * np;
+  o.ll = o.wts + np;
+  o.lr = o.ll + c.nsp;
+  return o;
+}
+
+static hipStream_t stream_of(const Fr2dCtx &c) { return c.stream ? c.stream : c.own_stream; }
+
+static int ensure_state(Fr2dCtx &c) {
+  if (!c.d_u) {
+    FR_HIP(hipMalloc(&c.d_u, c.len * sizeof(double)));
+    FR_HIP(hipMemsetAsync(c.d_u, 0, c.len * sizeof(double), stream_of(c)));
+  }
+  return FR_OK;
+}
+
+static int ensure_stage_buffers(Fr2dCtx &c, bool acc) {
+  if (!c.d_s1) FR_HIP(hipMalloc(&c.d_s1, c.len * sizeof(double)));
+  if (!c.d_s2) FR_HIP(hipMalloc(&c.d_s2, c.len * sizeof(double)));
+  if (acc && !c.d_acc) FR_HIP(hipMalloc(&c.d_acc, c.len * sizeof(double)));
+  return FR_OK;
+}
+
+// Stage launches of one explicit step on the device state (see fr.h: fr2d_step).  Stage buffers
+// carry the same ghost layer as the state (copied before the step loop), and no stage writes a
+// ghost cell, so FR_BC_GHOST keeps the ghost layer fixed during a step as the reference does.
+static int enqueue_steps(Fr2dCtx &c, int scheme, double dt, int nsteps, hipStream_t st) {
+  double *U = c.d_u, *S1 = c.d_s1, *S2 = c.d_s2;
+  for (int n = 0; n < nsteps; ++n) {
+    int rc = FR_OK;
+    switch (scheme) {
+      case FR_RK_SSPRK3:
+        // Shu-Osher: u1 = u + dt L(u); u2 = 3/4 u + 1/4 u1 + dt/4 L(u1); u = 1/3 u + 2/3 u2 + 2dt/3 L(u2)
+        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, U, S1, S2, 0.75, 0.25, 0.25 * dt, nullptr, nullptr, 0.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, U, S2, U, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, nullptr, nullptr, 0.0, st)))
+          return rc;
+        break;
+      case FR_RK_RK4: {
+        double *A = c.d_acc;
+        if ((rc = fr2d_launch_stage(c, U, U, S2, 1.0, 0.0, 0.5 * dt, U, A, dt / 6.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, U, S2, S1, 1.0, 0.0, 0.5 * dt, A, A, dt / 3.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, U, S1, S2, 1.0, 0.0, dt, A, A, dt / 3.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, A, S2, U, 1.0, 0.0, dt / 6.0, nullptr, nullptr, 0.0, st))) return rc;
+        break;
+      }
+      case FR_RK_MIDPOINT:
+        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, U, S1, U, 1.0, 0.0, dt, nullptr, nullptr, 0.0, st))) return rc;
+        break;
+      case FR_RK_EULER:
+        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st))) return rc;
+        if ((rc = fr2d_launch_copy_ghost(c, U, S1, st))) return rc;  // keep the pair identical
+        FR_HIP(hipMemcpyAsync(U, S1, c.len * sizeof(double), hipMemcpyDeviceToDevice, st));
+        break;
+      default:
+        return fail(FR_E_INVALID, "fr2d_step: unknown scheme");
+    }
+  }
+  return FR_OK;
+}
+
+static int prepare_step(Fr2dCtx &c, int scheme, hipStream_t st) {
+  FR_REQUIRE(c.d_u, FR_E_INVALID, "fr2d_step: no state (call fr2d_set_state first)");
+  int rc = ensure_stage_buffers(c, scheme == FR_RK_RK4);
+  if (rc) return rc;
+  if ((rc = fr2d_launch_copy_ghost(c, c.d_u, c.d_s1, st))) return rc;
+  if ((rc = fr2d_launch_copy_ghost(c, c.d_u, c.d_s2, st))) return rc;
+  return FR_OK;
+}
+
+static int finish_step(Fr2dCtx &c, hipStream_t st) {
+  if (c.bc == FR_BC_PERIODIC) return fr2d_launch_fill_ghost(c, c.d_u, FR_GHOST_PERIODIC, st);
+  return FR_OK;
+}
+
+}  // namespace fr
+
+using namespace fr;
+
+struct fr2d_ctx : Fr2dCtx {};
+
+extern "C" {
+
+int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
+  FR_REQUIRE(out && d, FR_E_INVALID, "fr2d_create: null argument");
+  *out = nullptr;
+  FR_REQUIRE(d->nx >= 1 && d->ny >= 1, FR_E_INVALID, "fr2d_create: nx, ny must be >= 1");
+  FR_REQUIRE(d->deg >= 1 && d->deg <= 4, FR_E_UNSUPPORTED, "fr2d_create: degree must be in 1..4");
+  FR_REQUIRE(d->flux >= FR_FLUX_HLL && d->flux <= FR_FLUX_ROE, FR_E_INVALID, "fr2d_create: unknown flux");
+  FR_REQUIRE(d->bc == FR_BC_GHOST || d->bc == FR_BC_PERIODIC, FR_E_INVALID, "fr2d_create: unknown bc");
+  FR_REQUIRE(d->gamma > 1.0 && std::isfinite(d->gamma), FR_E_INVALID, "fr2d_create: gamma must be > 1");
+  FR_REQUIRE(d->iJ11 > 0.0 && d->iJ22 > 0.0, FR_E_INVALID, "fr2d_create: iJ11, iJ22 must be > 0");
+  FR_REQUIRE((double)(d->nx + 2) * (d->ny + 2) * (d->deg + 1) * (d->deg + 1) * 4 < 9.0e18, FR_E_INVALID,
+             "fr2d_create: grid too large");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(FR_E_HIP, "fr2d_create: no HIP device visible");
+  fr2d_ctx *c = new (std::nothrow) fr2d_ctx();
+  FR_REQUIRE(c, FR_E_NOMEM, "fr2d_create: out of host memory");
+  c->nx = d->nx;
+  c->ny = d->ny;
+  c->deg = d->deg;
+  c->nsp = d->deg + 1;
+  c->flux = d->flux;
+  c->bc = d->bc;
+  c->gamma = d->gamma;
+  c->iJ11 = d->iJ11;
+  c->iJ22 = d->iJ22;
+  c->len = (size_t)(d->nx + 2) * (d->ny + 2) * c->nsp * c->nsp * 4;
+  if (!build_ops(d->deg, FR_CORR_RADAU, c->ops)) {
+    delete c;
+    return fail(FR_E_INVALID, "fr2d_create: operator construction failed");
+  }
+  const int n = c->nsp, np = n * n;
+  // caller-provided operator data (FRPSpace2D fields) take precedence
+  if (d->ll) std::memcpy(c->ops.ll, d->ll, sizeof(double) * n);
+  if (d->lr) std::memcpy(c->ops.lr, d->lr, sizeof(double) * n);
+  if (d->dl) std::memcpy(c->ops.dl, d->dl, sizeof(double) * n * n);
+  if (d->dhl) std::memcpy(c->ops.dhl, d->dhl, sizeof(double) * n);
+  if (d->dhr) std::memcpy(c->ops.dhr, d->dhr, sizeof(double) * n);
+  if (d->V) std::memcpy(c->ops.V, d->V, sizeof(double) * np * np);
+  if (d->iV) std::memcpy(c->ops.iV, d->iV, sizeof(double) * np * np);
+  hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipMalloc(&c->d_count, 4 * sizeof(int));
+  const size_t nd = 2 * (size_t)np * np + np + 2 * n;
+  if (e == hipSuccess) e = hipMalloc(&c->d_dops, nd * sizeof(double));
+  if (e == hipSuccess) {
+    double host[2 * 625 + 25 + 10];
+    std::memcpy(host, c->ops.V, sizeof(double) * np * np);
+    std::memcpy(host + np * np, c->ops.iV, sizeof(double) * np * np);
+    for (int q = 0; q < np; ++q) host[2 * np * np + q] = c->ops.wp[q] / 4.0;
+    std::memcpy(host + 2 * np * np + np, c->ops.ll, sizeof(double) * n);
+    std::memcpy(host + 2 * np * np + np + n, c->ops.lr, sizeof(double) * n);
+    e = hipMemcpy(c->d_dops, host, nd * sizeof(double), hipMemcpyHostToDevice);
+  }
+  if (e != hipSuccess) {
+    int rc = hip_fail(e, "fr2d_create", __FILE__, __LINE__);
+    fr2d_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return FR_OK;
+}
+
+int fr2d_destroy(fr2d_ctx *c) {
+  if (!c) return FR_OK;
+  hipStream_t st = stream_of(*c);
+  if (st) (void)hipStreamSynchronize(st);
+  double *bufs[] = {c->d_u, c->d_s1, c->d_s2, c->d_acc, c->d_in, c->d_out, c->d_dops};
+  for (double *b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->d_count) (void)hipFree(c->d_count);
+  if (c->d_mlp) (void)hipFree(c->d_mlp);
+  if (c->d_feat) (void)hipFree(c->d_feat);
+  if (c->d_strength) (void)hipFree(c->d_strength);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return FR_OK;
+}
+
+int fr2d_set_stream(fr2d_ctx *c, void *s) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr2d_set_stream: null context");
+  c->stream = (hipStream_t)s;
+  return FR_OK;
+}
+
+void *fr2d_get_stream(fr2d_ctx *c) { return c ? (void *)stream_of(*c) : nullptr; }
+
+size_t fr2d_state_len(const fr2d_ctx *c) { return c ? c->len : 0; }
+
+int fr2d_euler_rhs_dev(fr2d_ctx *c, const double *d_u, double *d_du, double t) {
+  (void)t;  // autonomous system: the reference dudt! ignores t
+  FR_REQUIRE(c && d_u && d_du, FR_E_INVALID, "fr2d_euler_rhs_dev: null argument");
+  FR_REQUIRE(d_u != d_du, FR_E_INVALID, "fr2d_euler_rhs_dev: u and du must not alias");
+  return fr2d_launch_residual(*c, d_u, d_du, stream_of(*c));
+}
+
+int fr2d_euler_rhs(fr2d_ctx *c, const double *u, double *du, double t) {
+  (void)t;
+  FR_REQUIRE(c && u && du, FR_E_INVALID, "fr2d_euler_rhs: null argument");
+  hipStream_t st = stream_of(*c);
+  const size_t bytes = c->len * sizeof(double);
+  if (!c->d_in) FR_HIP(hipMalloc(&c->d_in, bytes));
+  if (!c->d_out) FR_HIP(hipMalloc(&c->d_out, bytes));
+  FR_HIP(hipMemcpyAsync(c->d_in, u, bytes, hipMemcpyHostToDevice, st));
+  int rc = fr2d_launch_residual(*c, c->d_in, c->d_out, st);
+  if (rc) return rc;
+  FR_HIP(hipMemcpyAsync(du, c->d_out, bytes, hipMemcpyDeviceToHost, st));
+  FR_HIP(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
+int fr2d_set_state(fr2d_ctx *c, const double *u) {
+  FR_REQUIRE(c && u, FR_E_INVALID, "fr2d_set_state: null argument");
+  int rc = ensure_state(*c);
+  if (rc) return rc;
+  hipStream_t st = stream_of(*c);
+  FR_HIP(hipMemcpyAsync(c->d_u, u, c->len * sizeof(double), hipMemcpyHostToDevice, st));
+  FR_HIP(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
+int fr2d_get_state(fr2d_ctx *c, double *u) {
+  FR_REQUIRE(c && u, FR_E_INVALID, "fr2d_get_state: null argument");
+  FR_REQUIRE(c->d_u, FR_E_INVALID, "fr2d_get_state: no state");
+  hipStream_t st = stream_of(*c);
+  FR_HIP(hipMemcpyAsync(u, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToHost, st));
+  FR_HIP(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
+int fr2d_set_state_dev(fr2d_ctx *c, const double *d_u) {
+  FR_REQUIRE(c && d_u, FR_E_INVALID, "fr2d_set_state_dev: null argument");
+  int rc = ensure_state(*c);
+  if (rc) return rc;
+  FR_HIP(hipMemcpyAsync(c->d_u, d_u, c->len * sizeof(double), hipMemcpyDeviceToDevice, stream_of(*c)));
+  return FR_OK;
+}
+
+int fr2d_get_state_dev(fr2d_ctx *c, double *d_u) {
+  FR_REQUIRE(c && d_u, FR_E_INVALID, "fr2d_get_state_dev: null argument");
+  FR_REQUIRE(c->d_u, FR_E_INVALID, "fr2d_get_state_dev: no state");
+  FR_HIP(hipMemcpyAsync(d_u, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToDevice, stream_of(*c)));
+  return FR_OK;
+}
+
+double *fr2d_state_dev(fr2d_ctx *c) {
+  if (!c) return nullptr;
+  if (ensure_state(*c)) return nullptr;
+  return c->d_u;
+}
+
+int fr2d_step(fr2d_ctx *c, int scheme, double dt, int nsteps) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr2d_step: null context");
+  FR_REQUIRE(nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_step: bad dt / nsteps");
+  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_step: unknown scheme");
+  hipStream_t st = stream_of(*c);
+  int rc = prepare_step(*c, scheme, st);
+  if (rc) return rc;
+  if ((rc = enqueue_steps(*c, scheme, dt, nsteps, st))) return rc;
+  return finish_step(*c, st);
+}
+
+int fr2d_step_graph(fr2d_ctx *c, int scheme, double dt, int nsteps, int nrep) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr2d_step_graph: null context");
+  FR_REQUIRE(nsteps >= 1 && nrep >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_step_graph: bad arguments");
+  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_step_graph: unknown scheme");
+  hipStream_t st = stream_of(*c);
+  int rc = prepare_step(*c, scheme, st);
+  if (rc) return rc;
+  // capture on a private stream so a caller stream in relaxed use is never put in capture mode
+  hipStream_t cap = c->own_stream;
+  if (cap != st) {
+    hipEvent_t ev;
+    FR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    FR_HIP(hipEventRecord(ev, st));
+    FR_HIP(hipStreamWaitEvent(cap, ev, 0));
+    FR_HIP(hipEventDestroy(ev));
+  }
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  FR_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+  rc = enqueue_steps(*c, scheme, dt, nsteps, cap);
+  hipError_t e = hipStreamEndCapture(cap, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  FR_HIP(e);
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  for (int r = 0; r < nrep && e == hipSuccess; ++r) e = hipGraphLaunch(exec, cap);
+  hipError_t e2 = hipStreamSynchronize(cap);
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  FR_HIP(e);
+  FR_HIP(e2);
+  return finish_step(*c, st);
+}
+
+int fr2d_fill_ghost(fr2d_ctx *c, int rule) {
+  FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_fill_ghost: no state");
+  FR_REQUIRE(rule == FR_GHOST_PERIODIC || rule == FR_GHOST_OUTFLOW, FR_E_INVALID, "fr2d_fill_ghost: unknown rule");
+  return fr2d_launch_fill_ghost(*c, c->d_u, rule, stream_of(*c));
+}
+
+int fr2d_sensor_filter(fr2d_ctx *c, int mode, double lambda, double S0, double kappa, int *n_flagged) {
+  FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_sensor_filter: no state");
+  FR_REQUIRE(mode >= FR_SENSOR_OFF && mode <= FR_SENSOR_LEARNED, FR_E_INVALID, "fr2d_sensor_filter: unknown mode");
+  FR_REQUIRE(lambda >= 0.0 && kappa > 0.0, FR_E_INVALID, "fr2d_sensor_filter: lambda >= 0 and kappa > 0 required");
+  hipStream_t st = stream_of(*c);
+  if (mode == FR_SENSOR_OFF) {
+    if (n_flagged) *n_flagged = 0;
+    return FR_OK;
+  }
+  if (mode == FR_SENSOR_LEARNED && !c->mlp_loaded) mode = FR_SENSOR_MODAL;  // plain sensor is the fallback
+  FR_HIP(hipMemsetAsync(c->d_count, 0, sizeof(int), st));
+  int rc = FR_OK;
+  if (mode == FR_SENSOR_LEARNED) {
+    if ((rc = fr2d_launch_mlp_sensor(*c, c->d_u, st))) return rc;
+  }
+  if ((rc = fr2d_launch_sensor_filter(*c, c->d_u, mode, lambda, S0, kappa, c->d_count, st))) return rc;
+  if (n_flagged) {
+    FR_HIP(hipMemcpyAsync(n_flagged, c->d_count, sizeof(int), hipMemcpyDeviceToHost, st));
+    FR_HIP(hipStreamSynchronize(st));
+  }
+  return FR_OK;
+}
+
+int fr2d_positivity(fr2d_ctx *c) {
+  FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_positivity: no state");
+  hipStream_t st = stream_of(*c);
+  FR_HIP(hipMemsetAsync(c->d_count, 0, sizeof(int), st));
+  int rc = fr2d_launch_positivity(*c, c->d_u, c->d_count, st);
+  if (rc) return rc;
+  int bad = 0;
+  FR_HIP(hipMemcpyAsync(&bad, c->d_count, sizeof(int), hipMemcpyDeviceToHost, st));
+  FR_HIP(hipStreamSynchronize(st));
+  if (bad) return fail(FR_E_LIMITER, "fr2d_positivity: " + std::to_string(bad) + " element(s) failed 0 < t1 <= 1");
+  return FR_OK;
+}
+
+int fr2d_check_state(fr2d_ctx *c) {
+  FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_check_state: no state");
+  hipStream_t st = stream_of(*c);
+  FR_HIP(hipMemsetAsync(c->d_count, 0, 2 * sizeof(int), st));
+  int rc = fr2d_launch_check(*c, c->d_u, c->d_count, st);
+  if (rc) return rc;
+  int cnt[2] = {0, 0};
+  FR_HIP(hipMemcpyAsync(cnt, c->d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  FR_HIP(hipStreamSynchronize(st));
+  if (cnt[0]) return fail(FR_E_NONFINITE, "fr2d_check_state: " + std::to_string(cnt[0]) + " non-finite value(s)");
+  if (cnt[1])
+    return fail(FR_E_NEGATIVE_STATE,
+                "fr2d_check_state: " + std::to_string(cnt[1]) + " point(s) with rho <= 0 or p <= 0");
+  return FR_OK;
+}
+
+}  // extern "C"

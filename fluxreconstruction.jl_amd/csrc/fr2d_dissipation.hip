@@ -1,0 +1,332 @@
+// fr2d_dissipation.hip -- per-step dissipation kernels of the 2D solver (gfx950):
+// modal shock sensor + L2 modal filter (example/shock-vortex.jl:308-321, src/dissipation.jl:13-23),
+// positivity limiter (src/dissipation.jl:125-215), state checks and ghost-layer copies.
+// One thread per element; the element's planes are read straight from the SoA state (lanes =
+// consecutive elements, so every access of a wavefront is contiguous).
+#include <algorithm>
+
+#include "fr2d_internal.h"
+
+namespace fr {
+
+__device__ __forceinline__ int shock_switch(double Se, double S0, double kappa) {
+  double sigma;
+  if (Se < S0 - kappa)
+    sigma = 1.0;
+  else if (Se < S0 + kappa)
+    sigma = 0.5 * (1.0 - sin(0.5 * M_PI * (Se - S0) / kappa));
+  else
+    sigma = 0.0;
+  return sigma < 0.99 ? 1 : 0;
+}
+
+// Sensor + filter over every cell, ghosts included (the reference loops over axes(itg.u, 1:2)).
+// V, iV: nodal<->modal matrices (np x np column-major, node q = k + NSP*l, mode a*NSP + b).
+// mode FR_SENSOR_MODAL: strength lambda on cells flagged by the Persson-Peraire switch;
+// FR_SENSOR_LEARNED: strength = lambda * s[e], s produced per cell by the MLP sensor kernels.
+template <int NSP>
+__global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int ncell, long long ldp,
+                                                                 const double *__restrict__ V,
+                                                                 const double *__restrict__ iV, int mode,
+                                                                 double lambda, double S0, double kappa,
+                                                                 const float *__restrict__ strength, int *count) {
+  constexpr int NP = NSP * NSP;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= ncell) return;
+  double nod[NP], mod[NP];
+#pragma unroll
+  for (int q = 0; q < NP; ++q) nod[q] = u[e + ldp * q];
+#pragma unroll
+  for (int a = 0; a < NP; ++a) {
+    double s = 0.0;
+#pragma unroll
+    for (int b = 0; b < NP; ++b) s += iV[a + NP * b] * nod[b];
+    mod[a] = s;
+  }
+  double lam = 0.0;
+  if (mode == FR_SENSOR_MODAL) {
+    double ss = 0.0;
+#pragma unroll
+    for (int a = 0; a < NP; ++a) ss += mod[a] * mod[a];
+    const double su = mod[NP - 1] * mod[NP - 1] / ss;
+    lam = shock_switch(log10(su), S0, kappa) ? lambda : 0.0;
+  } else {
+    lam = lambda * (double)strength[e];
+  }
+  if (!(lam > 0.0)) return;
+  if (count) atomicAdd(count, 1);
+  for (int m = 0; m < 4; ++m) {
+    if (m > 0) {
+#pragma unroll
+      for (int q = 0; q < NP; ++q) nod[q] = u[e + ldp * (q + NP * m)];
+#pragma unroll
+      for (int a = 0; a < NP; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NP; ++b) s += iV[a + NP * b] * nod[b];
+        mod[a] = s;
+      }
+    }
+    // KitBase filter_l2! on the flattened mode vector: mode n /= 1 + lambda (n+1)^2 n^2
+#pragma unroll
+    for (int a = 1; a < NP; ++a) mod[a] /= (1.0 + lam * (double)((a + 1) * (a + 1)) * (double)(a * a));
+#pragma unroll
+    for (int q = 0; q < NP; ++q) {
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < NP; ++b) s += V[q + NP * b] * mod[b];
+      u[e + ldp * (q + NP * m)] = s;
+    }
+  }
+}
+
+// Root of tj_equation (src/dissipation.jl:208-215) in closed form: rho, m and E are affine in t,
+// so rho(t) (p(t) - eps) = (g-1)(rho E - |m|^2/2) - eps rho is a quadratic A t^2 + B t + C; the
+// largest root in [0, 1] is returned (the one NewtonRaphson reaches from t = 1).
+__device__ __forceinline__ double tj_root(const double *ut, const double *um, double g, double eps) {
+  const double r0 = um[0], r1 = ut[0] - um[0];
+  const double x0 = um[1], x1 = ut[1] - um[1];
+  const double y0 = um[2], y1 = ut[2] - um[2];
+  const double e0 = um[3], e1 = ut[3] - um[3];
+  const double A = (g - 1.0) * (r1 * e1 - 0.5 * (x1 * x1 + y1 * y1));
+  const double B = (g - 1.0) * (r0 * e1 + r1 * e0 - (x0 * x1 + y0 * y1)) - eps * r1;
+  const double C = (g - 1.0) * (r0 * e0 - 0.5 * (x0 * x0 + y0 * y0)) - eps * r0;
+  double best = 0.0;
+  if (fabs(A) < 1e-300) {
+    if (B != 0.0) {
+      const double t = -C / B;
+      if (t >= 0.0 && t <= 1.0) best = t;
+    }
+    return best;
+  }
+  const double disc = B * B - 4.0 * A * C;
+  if (disc < 0.0) return 0.0;
+  const double sq = sqrt(disc);
+  const double qq = -0.5 * (B + (B >= 0.0 ? sq : -sq));
+  const double t1 = qq / A, t2 = (qq != 0.0) ? C / qq : t1;
+  if (t1 >= 0.0 && t1 <= 1.0 && t1 > best) best = t1;
+  if (t2 >= 0.0 && t2 <= 1.0 && t2 > best) best = t2;
+  return best;
+}
+
+// lambda = rho / (2 p) of KitBase conserve_prim
+__device__ __forceinline__ double prim_lambda(const double *w, double g) {
+  return 0.5 * w[0] / (g - 1.0) / (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+}
+
+// Positivity limiter of src/dissipation.jl:125-206 on every interior cell, weights wp/4.
+template <int NSP>
+__global__ __launch_bounds__(256) void fr2d_positivity_kernel(double *u, int nx, int ny, long long ldp,
+                                                              const double *__restrict__ wts,
+                                                              const double *__restrict__ ll,
+                                                              const double *__restrict__ lr, double g,
+                                                              int *bad) {
+  constexpr int NP = NSP * NSP;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= (long long)nx * ny) return;
+  const long long nxg = nx + 2;
+  const long long e = (t % nx + 1) + nxg * (t / nx + 1);
+  auto U = [&](int k, int l, int m) -> double & { return u[e + ldp * (k + NSP * (l + NSP * m))]; };
+  double um[4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    double s = 0.0;
+#pragma unroll
+    for (int l = 0; l < NSP; ++l)
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) s += U(k, l, m) * wts[k + NSP * l];
+    um[m] = s;
+  }
+  const double p_mean = 0.5 * um[0] * (1.0 / prim_lambda(um, g));
+  const double eps = fmin(1e-13, fmin(um[0], p_mean));
+  // boundary states before any correction: face f (1 bottom, 2 right, 3 top, 4 left), point j
+  double rmin = INFINITY;
+#pragma unroll
+  for (int j = 0; j < NSP; ++j)
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      double s = 0.0;
+#pragma unroll
+      for (int q = 0; q < NSP; ++q) {
+        const double w = (f == 0 || f == 3) ? ll[q] : lr[q];
+        s += ((f == 0 || f == 2) ? U(j, q, 0) : U(q, j, 0)) * w;
+      }
+      rmin = fmin(rmin, s);
+    }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) rmin = fmin(rmin, u[e + ldp * q]);
+  const double t1 = fmin((um[0] - eps) / (um[0] - rmin + 1e-8), 1.0);
+  if (!(t1 > 0.0 && t1 <= 1.0)) {
+    atomicAdd(bad, 1);
+    return;
+  }
+  // pressure corrector on the (pre-fix) boundary states, then on the density-fixed points
+  double tmin = INFINITY;
+  bool have = false;
+#pragma unroll 1
+  for (int j = 0; j < NSP; ++j)
+#pragma unroll 1
+    for (int f = 0; f < 4; ++f) {
+      double w[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < NSP; ++q) {
+          const double wq = (f == 0 || f == 3) ? ll[q] : lr[q];
+          s += ((f == 0 || f == 2) ? U(j, q, m) : U(q, j, m)) * wq;
+        }
+        w[m] = s;
+      }
+      if (prim_lambda(w, g) < eps) {
+        have = true;
+        tmin = fmin(tmin, tj_root(w, um, g, eps));
+      }
+    }
+#pragma unroll
+  for (int q = 0; q < NP; ++q) {
+    const double r = t1 * (u[e + ldp * q] - um[0]) + um[0];
+    u[e + ldp * q] = r;
+    double w[4] = {r, u[e + ldp * (q + NP)], u[e + ldp * (q + 2 * NP)], u[e + ldp * (q + 3 * NP)]};
+    if (prim_lambda(w, g) < eps) {
+      have = true;
+      tmin = fmin(tmin, tj_root(w, um, g, eps));
+    }
+  }
+  if (have) {
+    const double t2 = fmin(tmin, 1.0);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) u[e + ldp * (q + NP * m)] = t2 * (u[e + ldp * (q + NP * m)] - um[m]) + um[m];
+  }
+}
+
+// counts non-finite values (count[0]) and non-positive density / pressure points (count[1])
+template <int NSP>
+__global__ __launch_bounds__(256) void fr2d_check_kernel(const double *u, int nx, int ny, long long ldp, double g,
+                                                         int *count) {
+  constexpr int NP = NSP * NSP;
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= (long long)nx * ny) return;
+  const long long e = (t % nx + 1) + (long long)(nx + 2) * (t / nx + 1);
+  int nonfinite = 0, negative = 0;
+  for (int q = 0; q < NP; ++q) {
+    double w[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      w[m] = u[e + ldp * (q + NP * m)];
+      if (!isfinite(w[m])) ++nonfinite;
+    }
+    const double p = (g - 1.0) * (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
+    if (!(w[0] > 0.0) || !(p > 0.0)) ++negative;
+  }
+  if (nonfinite) atomicAdd(count, nonfinite);
+  if (negative) atomicAdd(count + 1, negative);
+}
+
+// copy the ghost ring of every plane from src to dst
+__global__ void fr2d_copy_ghost_kernel(const double *src, double *dst, int nx, int ny, long long ldp, int planes) {
+  const int nring = 2 * (nx + 2) + 2 * ny;
+  const long long total = (long long)nring * planes;
+  const long long nxg = nx + 2;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int g = (int)(t % nring);
+    const long long pl = t / nring;
+    int i, j;
+    if (g < nx + 2) {
+      i = g;
+      j = 0;
+    } else if (g < 2 * (nx + 2)) {
+      i = g - (nx + 2);
+      j = ny + 1;
+    } else if (g < 2 * (nx + 2) + ny) {
+      i = 0;
+      j = g - 2 * (nx + 2) + 1;
+    } else {
+      i = nx + 1;
+      j = g - 2 * (nx + 2) - ny + 1;
+    }
+    const long long o = (long long)i + nxg * j + ldp * pl;
+    dst[o] = src[o];
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+int fr2d_launch_copy_ghost(const Fr2dCtx &c, const double *src, double *dst, hipStream_t st) {
+  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
+  const int planes = c.nsp * c.nsp * 4;
+  const long long total = (long long)(2 * (c.nx + 2) + 2 * c.ny) * planes;
+  const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(fr2d_copy_ghost_kernel, dim3(blocks), dim3(256), 0, st, src, dst, c.nx, c.ny, ldp, planes);
+  FR_LAUNCHED("fr2d_copy_ghost_kernel");
+  return FR_OK;
+}
+
+
+template <int NSP>
+static int sensor_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int *cnt,
+                      hipStream_t st) {
+  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
+  const int ncell = (int)ldp;
+  DissOps o = fr2d_diss_ops(c);
+  hipLaunchKernelGGL(fr2d_sensor_filter_kernel<NSP>, dim3((ncell + 255) / 256), dim3(256), 0, st, d_u, ncell, ldp,
+                     o.V, o.iV, mode, lambda, S0, kappa, (const float *)c.d_strength, cnt);
+  FR_LAUNCHED("fr2d_sensor_filter_kernel");
+  return FR_OK;
+}
+
+int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
+                              int *d_count, hipStream_t st) {
+  switch (c.nsp) {
+    case 2: return sensor_nsp<2>(c, d_u, mode, lambda, S0, kappa, d_count, st);
+    case 3: return sensor_nsp<3>(c, d_u, mode, lambda, S0, kappa, d_count, st);
+    case 4: return sensor_nsp<4>(c, d_u, mode, lambda, S0, kappa, d_count, st);
+    case 5: return sensor_nsp<5>(c, d_u, mode, lambda, S0, kappa, d_count, st);
+    default: return fail(FR_E_UNSUPPORTED, "sensor: degree out of range");
+  }
+}
+
+template <int NSP>
+static int positivity_nsp(const Fr2dCtx &c, double *d_u, int *bad, hipStream_t st) {
+  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
+  const long long n = (long long)c.nx * c.ny;
+  DissOps o = fr2d_diss_ops(c);
+  hipLaunchKernelGGL(fr2d_positivity_kernel<NSP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_u, c.nx,
+                     c.ny, ldp, o.wts, o.ll, o.lr, c.gamma, bad);
+  FR_LAUNCHED("fr2d_positivity_kernel");
+  return FR_OK;
+}
+
+int fr2d_launch_positivity(const Fr2dCtx &c, double *d_u, int *d_count, hipStream_t st) {
+  switch (c.nsp) {
+    case 2: return positivity_nsp<2>(c, d_u, d_count, st);
+    case 3: return positivity_nsp<3>(c, d_u, d_count, st);
+    case 4: return positivity_nsp<4>(c, d_u, d_count, st);
+    case 5: return positivity_nsp<5>(c, d_u, d_count, st);
+    default: return fail(FR_E_UNSUPPORTED, "positivity: degree out of range");
+  }
+}
+
+template <int NSP>
+static int check_nsp(const Fr2dCtx &c, const double *d_u, int *cnt, hipStream_t st) {
+  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
+  const long long n = (long long)c.nx * c.ny;
+  hipLaunchKernelGGL(fr2d_check_kernel<NSP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_u, c.nx, c.ny,
+                     ldp, c.gamma, cnt);
+  FR_LAUNCHED("fr2d_check_kernel");
+  return FR_OK;
+}
+
+int fr2d_launch_check(const Fr2dCtx &c, const double *d_u, int *d_count, hipStream_t st) {
+  switch (c.nsp) {
+    case 2: return check_nsp<2>(c, d_u, d_count, st);
+    case 3: return check_nsp<3>(c, d_u, d_count, st);
+    case 4: return check_nsp<4>(c, d_u, d_count, st);
+    case 5: return check_nsp<5>(c, d_u, d_count, st);
+    default: return fail(FR_E_UNSUPPORTED, "check: degree out of range");
+  }
+}
+
+}  // namespace fr

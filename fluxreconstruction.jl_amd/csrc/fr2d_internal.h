@@ -1,0 +1,49 @@
+// fr2d_internal.h -- context of the 2D Euler solver and its kernel launchers.
+#pragma once
+
+#include "fr_common.h"
+
+namespace fr {
+
+struct Fr2dCtx {
+  int nx = 0, ny = 0, deg = 0, nsp = 0, flux = 0, bc = 0;
+  double gamma = 1.4, iJ11 = 1.0, iJ22 = 1.0;
+  HostOps ops;
+  size_t len = 0;  // doubles per state array
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  // device state and stage buffers of the integrator path
+  double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_acc = nullptr;
+  // staging buffers of the host-pointer residual
+  double *d_in = nullptr, *d_out = nullptr;
+  // per-cell flags / counters of the dissipation kernels
+  int *d_count = nullptr;
+  // learned sensor: BF16 MLP parameters, per-cell features and strengths in device memory
+  void *d_mlp = nullptr;
+  void *d_feat = nullptr;
+  void *d_strength = nullptr;
+  bool mlp_loaded = false;
+  // device copies of V, iV, wp/4, ll, lr used by the dissipation kernels
+  double *d_dops = nullptr;
+};
+
+int fr2d_launch_residual(const Fr2dCtx &c, const double *d_u, double *d_du, hipStream_t st);
+int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, double *d_out, double a0,
+                      double a1, double b, const double *acc_in, double *acc_out, double b2, hipStream_t st);
+int fr2d_launch_fill_ghost(const Fr2dCtx &c, double *d_u, int rule, hipStream_t st);
+int fr2d_launch_copy_ghost(const Fr2dCtx &c, const double *src, double *dst, hipStream_t st);
+int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
+                              int *d_count, hipStream_t st);
+int fr2d_launch_positivity(const Fr2dCtx &c, double *d_u, int *d_count, hipStream_t st);
+int fr2d_launch_check(const Fr2dCtx &c, const double *d_u, int *d_count, hipStream_t st);
+// learned sensor: per-cell strengths from the MLP into c.d_strength (fr_mlp.hip)
+int fr2d_launch_mlp_sensor(const Fr2dCtx &c, const double *d_u, hipStream_t st);
+
+// device copies of the dissipation operators, carved out of Fr2dCtx::d_dops
+struct DissOps {
+  const double *V, *iV, *wts, *ll, *lr;
+};
+DissOps fr2d_diss_ops(const Fr2dCtx &c);
+
+}  // namespace fr

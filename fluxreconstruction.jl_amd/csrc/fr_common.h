@@ -1,0 +1,165 @@
+// fr_common.h -- shared host/device helpers of libfrhip (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+
+#include "../../include/fr.h"
+
+namespace fr {
+
+// ------------------------------------------------------------------ error plumbing
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define FR_HIP(call)                                                   \
+  do {                                                                 \
+    hipError_t fr_e_ = (call);                                         \
+    if (fr_e_ != hipSuccess) return fr::hip_fail(fr_e_, #call, __FILE__, __LINE__); \
+  } while (0)
+
+#define FR_REQUIRE(cond, code, msg)          \
+  do {                                       \
+    if (!(cond)) return fr::fail(code, msg); \
+  } while (0)
+
+// A kernel launch is checked with hipGetLastError right after it.
+#define FR_LAUNCHED(name) FR_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------ host operator builders
+// (fr_host_ops.cpp; the same routines back the fr_* operator entry points of fr.h)
+void gauss_legendre(int n, double *x, double *w);
+double dlegendre(int n, double x);
+void dcorrection(int kind, int deg, const double *x, int n, double *dgl, double *dgr);
+void lagrange_point(const double *sp, int nsp, double x, double *l);
+void dlagrange(const double *sp, int nsp, double *lpdm);
+double jacobi_p(double x, double alpha, double beta, int N);
+double djacobi_p(double x, double alpha, double beta, int N);
+bool invert(int n, const double *A, double *Ainv);
+
+// Operators of one degree: Gauss points, ll/lr, dl(m,k) column-major, correction derivatives,
+// quad Vandermonde (nsp^2 x nsp^2) and its inverse, tensor weights wp(k,l).
+struct HostOps {
+  int deg = 0, nsp = 0;
+  double r[8], w[8], ll[8], lr[8], dl[64], dhl[8], dhr[8];
+  double V[625], iV[625], wp[64];
+  double V1[64], iV1[64];
+};
+bool build_ops(int deg, int correction, HostOps &o);
+
+// XCD-aware tile order: consecutive tiles land on the same XCD (workgroups are dispatched to the
+// 8 XCDs round-robin), so neighbouring tiles share that XCD's L2 for their halo reads.
+__device__ __forceinline__ int xcd_tile(int bid, int nb) {
+  const int per = nb >> 3, rem = nb & 7;
+  const int x = bid & 7, loc = bid >> 3;
+  return (x < rem) ? x * (per + 1) + loc : rem * (per + 1) + (x - rem) * per + loc;
+}
+
+// ------------------------------------------------------------------ Euler physics (device)
+// Conserved 2D state w = [rho, rho u, rho v, E]; p = (g-1)(E - |m|^2 / (2 rho)).
+
+// x-direction physical flux and the primitive quantities the Riemann solvers need.
+struct XState {
+  double F[4];
+  double u, p, rinv;
+};
+
+__device__ __forceinline__ void xstate(const double *w, double gm1, XState &s) {
+  s.rinv = 1.0 / w[0];
+  s.u = w[1] * s.rinv;
+  const double v = w[2] * s.rinv;
+  s.p = gm1 * (w[3] - 0.5 * (w[1] * s.u + w[2] * v));
+  s.F[0] = w[1];
+  s.F[1] = w[1] * s.u + s.p;
+  s.F[2] = w[2] * s.u;
+  s.F[3] = (w[3] + s.p) * s.u;
+}
+
+// Interface flux in the x-direction for the normal-frame states wL | wR (4 variables).
+// FLUX: FR_FLUX_HLL (Davis wave-speed bounds, KitBase flux_hll!), FR_FLUX_RUSANOV, FR_FLUX_ROE
+// (Roe-averaged eigen-decomposition, no entropy fix).
+template <int FLUX>
+__device__ __forceinline__ void riemann_x4(const double *wL, const double *wR, double g, double gm1,
+                                           double *fw) {
+  XState L, R;
+  xstate(wL, gm1, L);
+  xstate(wR, gm1, R);
+  if constexpr (FLUX == FR_FLUX_HLL) {
+    const double cL = sqrt(g * L.p * L.rinv), cR = sqrt(g * R.p * R.rinv);
+    const double lmin = fmin(L.u - cL, R.u - cR);
+    const double lmax = fmax(L.u + cL, R.u + cR);
+    const double factor = 1.0 / (lmax - lmin);
+    const bool left = lmin >= 0.0, right = lmax <= 0.0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      const double h = factor * (lmax * L.F[m] - lmin * R.F[m] + lmax * lmin * (wR[m] - wL[m]));
+      fw[m] = left ? L.F[m] : (right ? R.F[m] : h);
+    }
+  } else if constexpr (FLUX == FR_FLUX_RUSANOV) {
+    const double cL = sqrt(g * L.p * L.rinv), cR = sqrt(g * R.p * R.rinv);
+    const double smax = fmax(fabs(L.u) + cL, fabs(R.u) + cR);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) fw[m] = 0.5 * (L.F[m] + R.F[m]) - 0.5 * smax * (wR[m] - wL[m]);
+  } else {
+    const double sL = sqrt(wL[0]), sR = sqrt(wR[0]);
+    const double vL = wL[2] * L.rinv, vR = wR[2] * R.rinv;
+    const double HL = (wL[3] + L.p) * L.rinv, HR = (wR[3] + R.p) * R.rinv;
+    const double ws = 1.0 / (sL + sR);
+    const double ut = (sL * L.u + sR * R.u) * ws;
+    const double vt = (sL * vL + sR * vR) * ws;
+    const double Ht = (sL * HL + sR * HR) * ws;
+    const double q2 = ut * ut + vt * vt;
+    const double a2 = gm1 * (Ht - 0.5 * q2);
+    const double at = sqrt(a2);
+    const double dr = wR[0] - wL[0], dp = R.p - L.p, du = R.u - L.u, dv = vR - vL;
+    const double rt = sL * sR;
+    const double ia2 = 1.0 / a2;
+    const double al1 = (dp - rt * at * du) * 0.5 * ia2;
+    const double al5 = (dp + rt * at * du) * 0.5 * ia2;
+    const double al2 = dr - dp * ia2;
+    const double al3 = rt * dv;
+    const double l1 = fabs(ut - at), l2 = fabs(ut), l5 = fabs(ut + at);
+    const double d0 = l1 * al1 + l2 * al2 + l5 * al5;
+    const double d1 = l1 * al1 * (ut - at) + l2 * al2 * ut + l5 * al5 * (ut + at);
+    const double d2 = l1 * al1 * vt + l2 * (al2 * vt + al3) + l5 * al5 * vt;
+    const double d3 = l1 * al1 * (Ht - ut * at) + l2 * (al2 * 0.5 * q2 + al3 * vt) + l5 * al5 * (Ht + ut * at);
+    fw[0] = 0.5 * (L.F[0] + R.F[0]) - 0.5 * d0;
+    fw[1] = 0.5 * (L.F[1] + R.F[1]) - 0.5 * d1;
+    fw[2] = 0.5 * (L.F[2] + R.F[2]) - 0.5 * d2;
+    fw[3] = 0.5 * (L.F[3] + R.F[3]) - 0.5 * d3;
+  }
+}
+
+// y-direction interface flux: rotate into the face-normal frame n = (0, 1), solve, rotate back
+// (KitBase local_frame / global_frame with cos = 0, sin = 1; example/shock-vortex.jl:84-87).
+template <int FLUX>
+__device__ __forceinline__ void riemann_y4(const double *wB, const double *wT, double g, double gm1,
+                                           double *fw) {
+  const double a[4] = {wB[0], wB[2], -wB[1], wB[3]};
+  const double b[4] = {wT[0], wT[2], -wT[1], wT[3]};
+  double f[4];
+  riemann_x4<FLUX>(a, b, g, gm1, f);
+  fw[0] = f[0];
+  fw[1] = -f[2];
+  fw[2] = f[1];
+  fw[3] = f[3];
+}
+
+// 1D Euler (3 variables) interface flux.
+template <int FLUX>
+__device__ __forceinline__ void riemann_x3(const double *wL, const double *wR, double g, double gm1,
+                                           double *fw) {
+  const double a[4] = {wL[0], wL[1], 0.0, wL[2]};
+  const double b[4] = {wR[0], wR[1], 0.0, wR[2]};
+  double f[4];
+  riemann_x4<FLUX>(a, b, g, gm1, f);
+  fw[0] = f[0];
+  fw[1] = f[1];
+  fw[2] = f[3];
+}
+
+}  // namespace fr

@@ -1,0 +1,36 @@
+"""frhip -- host side of the MI355X flux-reconstruction solver core.
+
+Mirrors the public API of FluxReconstruction.jl (src/FluxReconstruction.jl:19-51) over the C ABI
+of libfrhip.so (include/fr.h).  All numerical work runs in the shared library: operator data on
+its native host routines, residuals / stages / dissipation in its gfx950 kernels.
+"""
+from ._lib import FRError, lib, exported_symbols  # noqa: F401
+from .space import (  # noqa: F401
+    Line,
+    Quad,
+    PSpace1D,
+    PSpace2D,
+    FRPSpace1D,
+    FRPSpace2D,
+    legendre_point,
+    gausslegendre,
+    lagrange_point,
+    dlagrange,
+    standard_lagrange,
+    dradau,
+    dsd,
+    dhuynh,
+    JacobiP,
+    dJacobiP,
+    vandermonde_matrix,
+    dvandermonde_matrix,
+    rs_jacobi,
+    shock_detector,
+    L1_error,
+    L2_error,
+    Linf_error,
+)
+from .euler2d import FREuler2D  # noqa: F401
+from . import ics  # noqa: F401
+
+__version__ = "0.1.0"

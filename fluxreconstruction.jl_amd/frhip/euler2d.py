@@ -1,0 +1,158 @@
+"""2D Euler FR solver on uniform quads: the drop-in right-hand side and the device-resident
+integrator of libfrhip (include/fr.h, "2D Euler on uniform quads").
+
+The reference's user-level right-hand side is ``dudt!(du, u, p, t)`` of
+example/shock-vortex.jl:26-118 with the scratch tuple ``p`` of :138-153; here the scratch lives in
+the library context and ``FREuler2D.dudt(du, u, p, t)`` keeps the same argument meaning.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+from .space import FRPSpace2D
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+class FREuler2D:
+    """Context of one 2D Euler problem.
+
+    ps      FRPSpace2D with one ghost layer (ngx = ngy = 1)
+    gamma   ratio of specific heats
+    flux    "hll" (reference default), "rusanov" or "roe"
+    bc      "ghost" (ghost layer is data, refreshed by the caller as the reference scripts do)
+            or "periodic" (neighbours wrap inside the kernels)
+    """
+
+    def __init__(self, ps: FRPSpace2D, gamma: float = 1.4, flux: str = "hll", bc: str = "ghost"):
+        if getattr(ps, "ngx", 1) != 1 or getattr(ps, "ngy", 1) != 1:
+            raise ValueError("FREuler2D needs an FRPSpace2D with one ghost layer (ngx = ngy = 1)")
+        self.ps = ps
+        self.gamma = float(gamma)
+        self.flux = flux
+        self.bc = bc
+        n = ps.deg + 1
+        self.nsp = n
+        self.shape = (ps.nx + 2, ps.ny + 2, n, n, 4)
+        self._ops = [np.ascontiguousarray(a, dtype=np.float64) for a in (
+            ps.ll, ps.lr, np.asarray(ps.dl).ravel(order="F"), ps.dhl, ps.dhr,
+            np.asarray(ps.V).ravel(order="F"), np.asarray(ps.iV).ravel(order="F"))]
+        d = _lib.Fr2dDesc()
+        d.nx, d.ny, d.deg = ps.nx, ps.ny, ps.deg
+        d.gamma = self.gamma
+        d.flux = _lib.FLUX[flux]
+        d.bc = _lib.BC[bc]
+        d.iJ11 = float(ps.iJ_elem[0, 0])
+        d.iJ22 = float(ps.iJ_elem[1, 1])
+        d.ll, d.lr, d.dl, d.dhl, d.dhr, d.V, d.iV = [dptr(a) for a in self._ops]
+        h = C.c_void_p()
+        check(_lib.lib().fr2d_create(C.byref(h), C.byref(d)))
+        self.h = h
+        self.len = _lib.lib().fr2d_state_len(h)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            _lib.lib().fr2d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- streams
+    def use_torch_stream(self, stream=None):
+        """Launch on a torch CUDA/HIP stream (default: torch's current stream)."""
+        import torch
+
+        s = stream if stream is not None else torch.cuda.current_stream()
+        check(_lib.lib().fr2d_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def stream_handle(self) -> int:
+        return _lib.lib().fr2d_get_stream(self.h) or 0
+
+    # ---------------------------------------------------------------- right-hand side
+    def _check_host(self, a):
+        if a.shape != self.shape or a.dtype != np.float64 or not a.flags["F_CONTIGUOUS"]:
+            raise ValueError(f"expected a Fortran-ordered float64 array of shape {self.shape}")
+
+    def _check_dev(self, a):
+        if tuple(a.shape) != (self.len,) and tuple(a.shape) != self.shape[::-1]:
+            if a.numel() != self.len:
+                raise ValueError("device array has the wrong size")
+        if str(a.dtype) != "torch.float64" or not a.is_cuda or not a.is_contiguous():
+            raise ValueError("device arrays must be contiguous float64 torch tensors on the GPU")
+
+    def dudt(self, du, u, p=None, t: float = 0.0):
+        """In-place du = L(u) with du = 0 on the ghost layer (example/shock-vortex.jl:26-118).
+        numpy (host) arrays in the reference layout, or flat/reversed-shape torch GPU tensors."""
+        if _is_torch(u):
+            self._check_dev(u)
+            self._check_dev(du)
+            check(_lib.lib().fr2d_euler_rhs_dev(self.h, C.c_void_p(u.data_ptr()), C.c_void_p(du.data_ptr()), float(t)))
+        else:
+            self._check_host(u)
+            self._check_host(du)
+            check(_lib.lib().fr2d_euler_rhs(self.h, dptr(u), dptr(du), float(t)))
+        return None
+
+    __call__ = dudt
+
+    def rhs(self, u):
+        du = np.zeros(self.shape, order="F")
+        self.dudt(du, np.asfortranarray(u, dtype=np.float64))
+        return du
+
+    # ---------------------------------------------------------------- integrator
+    def set_state(self, u):
+        if _is_torch(u):
+            self._check_dev(u)
+            check(_lib.lib().fr2d_set_state_dev(self.h, C.c_void_p(u.data_ptr())))
+        else:
+            u = np.asfortranarray(u, dtype=np.float64)
+            self._check_host(u)
+            check(_lib.lib().fr2d_set_state(self.h, dptr(u)))
+
+    def get_state(self):
+        out = np.zeros(self.shape, order="F")
+        check(_lib.lib().fr2d_get_state(self.h, dptr(out)))
+        return out
+
+    def state_ptr(self) -> int:
+        return _lib.lib().fr2d_state_dev(self.h) or 0
+
+    def step(self, dt: float, nsteps: int = 1, scheme: str = "ssprk3"):
+        check(_lib.lib().fr2d_step(self.h, _lib.RK[scheme], float(dt), int(nsteps)))
+
+    def step_graph(self, dt: float, nsteps: int, nrep: int = 1, scheme: str = "ssprk3"):
+        check(_lib.lib().fr2d_step_graph(self.h, _lib.RK[scheme], float(dt), int(nsteps), int(nrep)))
+
+    def fill_ghost(self, rule: str = "periodic"):
+        check(_lib.lib().fr2d_fill_ghost(self.h, _lib.GHOST[rule]))
+
+    def sensor_filter(self, mode: str = "modal", lam: float = 5e-4, S0: float | None = None, kappa: float = 9.0):
+        """Modal sensor + L2 filter of example/shock-vortex.jl:308-321; returns #filtered cells."""
+        if S0 is None:
+            S0 = -3.0 * np.log10(self.ps.deg)
+        n = C.c_int(0)
+        check(_lib.lib().fr2d_sensor_filter(self.h, _lib.SENSOR[mode], float(lam), float(S0), float(kappa), C.byref(n)))
+        return n.value
+
+    def positivity(self):
+        """Positivity limiter of src/dissipation.jl:125-206 on every interior cell."""
+        check(_lib.lib().fr2d_positivity(self.h))
+
+    def check_state(self):
+        check(_lib.lib().fr2d_check_state(self.h))
+
+    def synchronize(self):
+        import torch
+
+        torch.cuda.synchronize()

@@ -1,0 +1,291 @@
+"""Host mirror of the reference's operator API and FR spaces, computed by libfrhip's native host
+routines (include/fr.h, "operator data").
+
+Names follow src/FluxReconstruction.jl:19-51 (Julia symbols with unicode prefixes are spelled
+out: ``∂lagrange`` -> ``dlagrange``, ``∂radau`` -> ``dradau`` ...).  Arrays are numpy, indices
+0-based, multi-dimensional arrays keep the reference's column-major (Fortran) layout.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+
+
+class Line:  # element-shape tags (src/data.jl:4-11)
+    pass
+
+
+class Quad:
+    pass
+
+
+def _f(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+
+
+# ------------------------------------------------------------------ polynomials (src/Polynomial)
+def legendre_point(p: int) -> np.ndarray:
+    """Gauss-Legendre points of degree p (src/Polynomial/poly_legendre.jl:6)."""
+    r = np.zeros(p + 1)
+    w = np.zeros(p + 1)
+    check(_lib.lib().fr_legendre_point(p, dptr(r), dptr(w)))
+    return r
+
+
+def gausslegendre(n: int):
+    r = np.zeros(n)
+    w = np.zeros(n)
+    check(_lib.lib().fr_legendre_point(n - 1, dptr(r), dptr(w)))
+    return r, w
+
+
+def lagrange_point(sp, x):
+    """Lagrange basis of solution points sp at x (src/Polynomial/poly_lagrange.jl:6-31)."""
+    sp = _f(sp)
+    if np.ndim(x) == 0:
+        out = np.zeros(sp.size)
+        check(_lib.lib().fr_lagrange_point(dptr(sp), sp.size, float(x), dptr(out)))
+        return out
+    return np.stack([lagrange_point(sp, xi) for xi in np.asarray(x)], axis=0)
+
+
+def dlagrange(sp) -> np.ndarray:
+    """lpdm[m, k] = l_k'(sp_m) (src/Polynomial/poly_lagrange.jl:38-59)."""
+    sp = _f(sp)
+    out = np.zeros(sp.size * sp.size)
+    check(_lib.lib().fr_dlagrange(dptr(sp), sp.size, dptr(out)))
+    return out.reshape((sp.size, sp.size), order="F")
+
+
+def standard_lagrange(x):
+    """(ll, lr, lpdm) (src/Polynomial/poly_lagrange.jl:99-105)."""
+    return lagrange_point(x, -1.0), lagrange_point(x, 1.0), dlagrange(x)
+
+
+def _dcorr(kind: str, p: int, x):
+    x = _f(np.atleast_1d(x))
+    dgl = np.zeros_like(x)
+    dgr = np.zeros_like(x)
+    check(_lib.lib().fr_dcorrection(_lib.CORR[kind], p, dptr(x), x.size, dptr(dgl), dptr(dgr)))
+    return dgl, dgr
+
+
+def dradau(p: int, x):
+    """Radau correction derivatives (src/Polynomial/poly_legendre.jl:29-37)."""
+    return _dcorr("radau", p, x)
+
+
+def dsd(p: int, x):
+    """Spectral-difference correction derivatives (src/Polynomial/poly_legendre.jl:44-54)."""
+    return _dcorr("sd", p, x)
+
+
+def dhuynh(p: int, x):
+    """Huynh g2 correction derivatives (src/Polynomial/poly_legendre.jl:61-71)."""
+    return _dcorr("huynh", p, x)
+
+
+def JacobiP(x, alpha, beta, N):
+    """Orthonormal Jacobi polynomial (src/Polynomial/poly_jacobi.jl:7-83)."""
+    xs = _f(np.atleast_1d(x))
+    out = np.zeros_like(xs)
+    check(_lib.lib().fr_jacobi_p(dptr(xs), xs.size, float(alpha), float(beta), int(N), dptr(out)))
+    return out if np.ndim(x) else float(out[0])
+
+
+def dJacobiP(x, alpha, beta, N):
+    """src/Polynomial/poly_jacobi.jl:88-107."""
+    xs = _f(np.atleast_1d(x))
+    out = np.zeros_like(xs)
+    check(_lib.lib().fr_djacobi_p(dptr(xs), xs.size, float(alpha), float(beta), int(N), dptr(out)))
+    return out if np.ndim(x) else float(out[0])
+
+
+def vandermonde_matrix(*args):
+    """vandermonde_matrix(N, r), (Line, N, r) or (Quad, N, r, s) (src/Transform/transform.jl:18-68)."""
+    if args[0] is Line:
+        args = args[1:]
+    if args[0] is Quad:
+        _, N, r, s = args
+        r = _f(r)
+        s = _f(s)
+        V = np.zeros(r.size * (N + 1) ** 2)
+        check(_lib.lib().fr_vandermonde_quad(N, dptr(r), dptr(s), r.size, dptr(V)))
+        return V.reshape((r.size, (N + 1) ** 2), order="F")
+    N, r = args
+    r = _f(r)
+    V = np.zeros(r.size * (N + 1))
+    check(_lib.lib().fr_vandermonde_1d(N, dptr(r), r.size, dptr(V)))
+    return V.reshape((r.size, N + 1), order="F")
+
+
+def dvandermonde_matrix(*args):
+    """r-derivative of the 1D Vandermonde matrix (src/Transform/transform.jl:76-89)."""
+    if args[0] is Line:
+        args = args[1:]
+    N, r = args
+    r = _f(r)
+    V = np.zeros(r.size * (N + 1))
+    check(_lib.lib().fr_dvandermonde_1d(N, dptr(r), r.size, dptr(V)))
+    return V.reshape((r.size, N + 1), order="F")
+
+
+def inv(A) -> np.ndarray:
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    out = np.zeros(n * n)
+    check(_lib.lib().fr_inv(n, dptr(_f(A)), dptr(out)))
+    return out.reshape((n, n), order="F")
+
+
+def rs_jacobi(r: float, s: float, vertices) -> np.ndarray:
+    """Bilinear quadrilateral Jacobian [xr xs; yr ys] (src/Geometry/geo_jacobi.jl:77-88)."""
+    J = np.zeros(4)
+    check(_lib.lib().fr_rs_jacobi_quad(float(r), float(s), dptr(_f(vertices)), dptr(J)))
+    return J.reshape((2, 2), order="F")
+
+
+def shock_detector(Se: float, deg: int, S0: float | None = None, kappa: float = 4.0) -> bool:
+    """Persson-Peraire switch (src/dissipation.jl:13-23)."""
+    if S0 is None:
+        S0 = -3.0 * np.log10(deg)
+    return bool(_lib.lib().fr_shock_detector(float(Se), int(deg), float(S0), float(kappa)))
+
+
+# ------------------------------------------------------------------ metrics (src/tools.jl:43-53)
+def L1_error(u, ue, dx):
+    return float(np.sum(np.abs(np.asarray(u) - np.asarray(ue)) * dx))
+
+
+def L2_error(u, ue, dx):
+    return float(np.sqrt(np.sum((np.abs(np.asarray(u) - np.asarray(ue)) * dx) ** 2)))
+
+
+def Linf_error(u, ue, dx):
+    return float(np.max(np.abs(np.asarray(u) - np.asarray(ue)) * dx))
+
+
+# ------------------------------------------------------------------ meshes (KitBase PSpace1D/2D)
+class PSpace1D:
+    """Uniform 1D mesh with ng ghost cells; x[i] for i = 1-ng .. nx+ng stored at i-1+ng."""
+
+    def __init__(self, x0, x1, nx, ng=0):
+        self.x0, self.x1, self.nx, self.ng = float(x0), float(x1), int(nx), int(ng)
+        d = (self.x1 - self.x0) / nx
+        idx = np.arange(1 - ng, nx + ng + 1)
+        self.x = self.x0 + (idx - 0.5) * d
+        self.dx = np.full(idx.size, d)
+
+
+class PSpace2D:
+    """Uniform 2D mesh with (ngx, ngy) ghost layers, KitBase conventions: centres x[i, j],
+    y[i, j] and vertices[i, j, v, d] (v = 1 bottom-left, 2 bottom-right, 3 top-right,
+    4 top-left; src/Geometry/geo_jacobi.jl:50-57)."""
+
+    def __init__(self, x0, x1, nx, y0, y1, ny, ngx=0, ngy=0):
+        self.x0, self.x1, self.nx = float(x0), float(x1), int(nx)
+        self.y0, self.y1, self.ny = float(y0), float(y1), int(ny)
+        self.ngx, self.ngy = int(ngx), int(ngy)
+        self.dx0 = (self.x1 - self.x0) / nx
+        self.dy0 = (self.y1 - self.y0) / ny
+        ii = np.arange(1 - ngx, nx + ngx + 1)
+        jj = np.arange(1 - ngy, ny + ngy + 1)
+        X, Y = np.meshgrid(self.x0 + (ii - 0.5) * self.dx0, self.y0 + (jj - 0.5) * self.dy0, indexing="ij")
+        self.x = np.asfortranarray(X)
+        self.y = np.asfortranarray(Y)
+        self.dx = np.full_like(self.x, self.dx0)
+        self.dy = np.full_like(self.y, self.dy0)
+        v = np.zeros(X.shape + (4, 2), order="F")
+        hx, hy = self.dx0 / 2, self.dy0 / 2
+        v[..., 0, 0], v[..., 0, 1] = X - hx, Y - hy
+        v[..., 1, 0], v[..., 1, 1] = X + hx, Y - hy
+        v[..., 2, 0], v[..., 2, 1] = X + hx, Y + hy
+        v[..., 3, 0], v[..., 3, 1] = X - hx, Y + hy
+        self.vertices = v
+
+
+class _Forward:
+    """Forward KitBase mesh fields through the FR space (src/tools.jl:5-37)."""
+
+    _base_fields = ("x0", "x1", "nx", "y0", "y1", "ny", "ng", "ngx", "ngy", "x", "y", "dx", "dy", "dx0", "dy0",
+                    "vertices")
+
+    def __getattr__(self, name):
+        if name in self._base_fields and "base" in self.__dict__:
+            return getattr(self.__dict__["base"], name)
+        raise AttributeError(name)
+
+
+class FRPSpace1D(_Forward):
+    """1D FR space (src/struct.jl:13-88): J, xpl, xpg, wp, dl, ll, lr, dll, dlr, dhl, dhr, V, iV."""
+
+    def __init__(self, x0, x1, nx, deg, ng=0, correction="radau"):
+        self.base = PSpace1D(x0, x1, nx, ng)
+        self.deg = int(deg)
+        self.np = deg + 1
+        self.correction = correction
+        self.J = self.base.dx / 2.0
+        r, w = gausslegendre(deg + 1)
+        self.xpl = r
+        self.wp = w
+        xi = np.append(self.base.x - 0.5 * self.base.dx, self.base.x[-1] + 0.5 * self.base.dx[-1])
+        self.xpg = np.asfortranarray(((1.0 - r[None, :]) / 2.0) * xi[:-1, None] + ((1.0 + r[None, :]) / 2.0) * xi[1:, None])
+        self.ll, self.lr, self.dl = standard_lagrange(r)
+        self.V = vandermonde_matrix(deg, r)
+        self.iV = inv(self.V)
+        dVf = dvandermonde_matrix(deg, np.array([-1.0, 1.0]))
+        self.dll = np.linalg.solve(self.V.T, dVf[0, :])
+        self.dlr = np.linalg.solve(self.V.T, dVf[1, :])
+        self.dhl, self.dhr = _dcorr(correction, deg, r)
+
+
+class FRPSpace2D(_Forward):
+    """2D FR space on uniform quads (src/struct.jl:99-245).
+
+    Fields: deg, np, xpl, xpg(i, j, k, l, d), wp(k, l), dl, ll, lr, dll, dlr, dhl, dhr, V, iV
+    (tensor Vandermonde, modes i-outer / j-inner), J / iJ per element (uniform rectangles: the
+    2x2 matrices diag(dx/2, dy/2) and its inverse, exposed as (nx+2g, ny+2g, 2, 2) views)."""
+
+    def __init__(self, x0, x1, nx, y0, y1, ny, deg, ngx=0, ngy=0):
+        base = x0 if isinstance(x0, PSpace2D) else PSpace2D(x0, x1, nx, y0, y1, ny, ngx, ngy)
+        if isinstance(x0, PSpace2D):
+            deg = x1
+        self.base = base
+        self.deg = int(deg)
+        self.np = (deg + 1) ** 2
+        n = deg + 1
+        r, w = gausslegendre(n)
+        self.xpl = r
+        self.wp = np.asfortranarray(np.outer(w, w))
+        Jc = rs_jacobi(0.0, 0.0, base.vertices[0, 0])
+        self.J_elem = Jc
+        self.iJ_elem = inv(Jc)
+        shape = base.x.shape
+        self.J = np.broadcast_to(Jc, shape + (2, 2))
+        self.iJ = np.broadcast_to(self.iJ_elem, shape + (2, 2))
+        v = base.vertices
+        rk = r[:, None]
+        rl = r[None, :]
+        xpg = np.zeros(shape + (n, n, 2), order="F")
+        for d in range(2):
+            xpg[..., d] = ((rk - 1.0) * (rl - 1.0) / 4)[None, None] * v[:, :, 0, d, None, None] \
+                + ((rk + 1.0) * (1.0 - rl) / 4)[None, None] * v[:, :, 1, d, None, None] \
+                + ((rk + 1.0) * (rl + 1.0) / 4)[None, None] * v[:, :, 2, d, None, None] \
+                + ((1.0 - rk) * (rl + 1.0) / 4)[None, None] * v[:, :, 3, d, None, None]
+        self.xpg = xpg
+        self.ll, self.lr, self.dl = standard_lagrange(r)
+        V1 = vandermonde_matrix(deg, r)
+        dVf = dvandermonde_matrix(deg, np.array([-1.0, 1.0]))
+        self.dll = np.linalg.solve(V1.T, dVf[0, :])
+        self.dlr = np.linalg.solve(V1.T, dVf[1, :])
+        self.dhl, self.dhr = dradau(deg, r)
+        r2d = np.repeat(r[:, None], n, axis=1)
+        s2d = np.repeat(r[None, :], n, axis=0)
+        self.V = vandermonde_matrix(Quad, deg, r2d.ravel(order="F"), s2d.ravel(order="F"))
+        self.iV = inv(self.V)
+
+    @property
+    def nsp(self) -> int:
+        return self.deg + 1

@@ -1,0 +1,190 @@
+/*
+ * fr.h -- C ABI of libfrhip.so, the MI355X (gfx950) flux-reconstruction solver core.
+ *
+ * Plain pointers and sizes only.  Every array uses the reference's Julia column-major layout, so
+ * a Julia caller passes its arrays (or their OffsetArray parents) unchanged:
+ *   2D state u(i, j, k, l, m), axes (0:nx+1, 0:ny+1, 1:nsp, 1:nsp, 1:4)
+ *       -> u[i + (nx+2)*(j + (ny+2)*(k + nsp*(l + nsp*m)))]      (example/shock-vortex.jl:160)
+ *   1D Euler state u(i, p, k), axes (1:ncell, 1:nsp, 1:3)        (src/Equation/eq_euler.jl:1)
+ *   1D advection state u(i, p), axes (1:ncell, 1:nsp)            (src/Equation/eq_advection.jl:1)
+ *   matrices such as dl(m, k) = l_k'(r_m)  -> dl[m + nsp*k]      (src/Polynomial/poly_lagrange.jl:109)
+ *
+ * Every call returns FR_OK (0) or an FR_E_* code; fr_last_error() gives the message of the
+ * calling thread's last failure.  Functions with a _dev suffix take device pointers and run
+ * asynchronously on the context's stream; all others take host pointers and return once done.
+ * Calls on one context are serialised by the caller; distinct contexts are independent.
+ */
+#ifndef FR_H
+#define FR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status / enums */
+enum {
+  FR_OK = 0,
+  FR_E_INVALID = 1,        /* bad argument (null pointer, size, enum value) */
+  FR_E_HIP = 2,            /* HIP runtime error */
+  FR_E_NONFINITE = 3,      /* NaN/Inf found by a checked call */
+  FR_E_NEGATIVE_STATE = 4, /* density or pressure <= 0 found by a checked call */
+  FR_E_UNSUPPORTED = 5,    /* configuration not implemented (e.g. degree out of range) */
+  FR_E_NOMEM = 6,
+  FR_E_LIMITER = 7         /* positivity limiter could not bring an element in range */
+};
+enum { FR_FLUX_HLL = 0, FR_FLUX_RUSANOV = 1, FR_FLUX_ROE = 2 };
+/* FR_BC_GHOST: the ghost layer of u is data (reference semantics: the host refreshes ghosts,
+ * example/shock-vortex.jl:324-326).  FR_BC_PERIODIC: neighbours wrap inside the kernels. */
+enum { FR_BC_GHOST = 0, FR_BC_PERIODIC = 1 };
+enum { FR_CORR_RADAU = 0, FR_CORR_SD = 1, FR_CORR_HUYNH = 2 };
+enum { FR_RK_SSPRK3 = 0, FR_RK_RK4 = 1, FR_RK_MIDPOINT = 2, FR_RK_EULER = 3 };
+/* ghost refresh rules for fr2d_fill_ghost (host loops of the reference scripts) */
+enum {
+  FR_GHOST_PERIODIC = 0, /* copy the opposite interior row/column (euler2d_wave.jl:127-131, no sign flips) */
+  FR_GHOST_OUTFLOW = 1   /* copy the adjacent interior row/column on every side (shock-vortex.jl:324-326
+                            plus the left side) */
+};
+/* sensor modes for fr2d_sensor_filter */
+enum { FR_SENSOR_OFF = 0, FR_SENSOR_MODAL = 1, FR_SENSOR_LEARNED = 2 };
+
+const char *fr_last_error(void);
+const char *fr_version(void);
+/* number of visible HIP devices (0 on a machine without a GPU; never fails) */
+int fr_device_count(void);
+
+/* ------------------------------------------------------------------ operator data (host, native)
+ * These replace the reference's setup routines; outputs are caller-owned host arrays. */
+/* Gauss-Legendre points/weights, ascending (src/Polynomial/poly_legendre.jl:6; struct.jl:47,178) */
+int fr_legendre_point(int deg, double *r, double *w);
+/* l_k(x) for solution points sp (src/Polynomial/poly_lagrange.jl:6-21) */
+int fr_lagrange_point(const double *sp, int nsp, double x, double *l);
+/* lpdm(m,k) = l_k'(sp_m) (src/Polynomial/poly_lagrange.jl:38-59) */
+int fr_dlagrange(const double *sp, int nsp, double *lpdm);
+/* correction-function derivatives dgl, dgr at x (src/Polynomial/poly_legendre.jl:29-71) */
+int fr_dcorrection(int kind, int deg, const double *x, int n, double *dgl, double *dgr);
+/* orthonormal Jacobi P_N^(a,b) at n points (src/Polynomial/poly_jacobi.jl:7-83,88-107) */
+int fr_jacobi_p(const double *x, int n, double alpha, double beta, int N, double *out);
+int fr_djacobi_p(const double *x, int n, double alpha, double beta, int N, double *out);
+/* 1D Vandermonde n x (N+1) and its r-derivative (src/Transform/transform.jl:18-31,76-89) */
+int fr_vandermonde_1d(int N, const double *r, int n, double *V);
+int fr_dvandermonde_1d(int N, const double *r, int n, double *Vr);
+/* quad tensor Vandermonde n x (N+1)^2, modes i-outer / j-inner (src/Transform/transform.jl:55-68) */
+int fr_vandermonde_quad(int N, const double *r, const double *s, int n, double *V);
+/* dense inverse (column-major) */
+int fr_inv(int n, const double *A, double *Ainv);
+/* bilinear quad Jacobian [xr xs; yr ys] at (r,s), vertices 4x2 (src/Geometry/geo_jacobi.jl:77-88) */
+int fr_rs_jacobi_quad(double r, double s, const double *vert, double *J);
+/* Persson-Peraire switch; returns 1 (shock) / 0 (src/dissipation.jl:13-23) */
+int fr_shock_detector(double Se, int deg, double S0, double kappa);
+
+/* ------------------------------------------------------------------ 2D Euler on uniform quads */
+typedef struct fr2d_ctx fr2d_ctx;
+
+typedef struct {
+  int nx, ny;       /* interior cells; the state carries one ghost layer (ngx = ngy = 1) */
+  int deg;          /* polynomial degree p, 1 <= p <= 4 */
+  double gamma;     /* ratio of specific heats */
+  int flux;         /* FR_FLUX_* */
+  int bc;           /* FR_BC_* */
+  double iJ11;      /* iJ[i,j][k,l][1,1] = 2/dx of FRPSpace2D (src/struct.jl:137-142) */
+  double iJ22;      /* iJ[i,j][k,l][2,2] = 2/dy */
+  /* Operator data as FRPSpace2D stores it (ps.ll, ps.lr, ps.dl, ps.dhl, ps.dhr, ps.V, ps.iV).
+   * Any of them may be NULL, in which case the library builds the Gauss/Radau operators itself
+   * (src/struct.jl:130-227). */
+  const double *ll, *lr, *dl, *dhl, *dhr, *V, *iV;
+} fr2d_desc;
+
+/* Replaces FRPSpace2D + the scratch tuple p of example/shock-vortex.jl:120-153 (the context owns
+ * all scratch and a device copy of the state for the stepping entry points). */
+int fr2d_create(fr2d_ctx **out, const fr2d_desc *desc);
+int fr2d_destroy(fr2d_ctx *ctx);
+/* run on a caller-provided hipStream_t (NULL = the context's own stream) */
+int fr2d_set_stream(fr2d_ctx *ctx, void *hip_stream);
+void *fr2d_get_stream(fr2d_ctx *ctx);
+/* number of doubles in one state array: (nx+2)(ny+2) nsp^2 4 */
+size_t fr2d_state_len(const fr2d_ctx *ctx);
+
+/* The drop-in dudt!(du, u, p, t) of example/shock-vortex.jl:26-118 (also euler2d_wave.jl:35-107):
+ * du = L(u) on interior cells, du = 0 on the ghost layer (shock-vortex.jl:27). */
+int fr2d_euler_rhs(fr2d_ctx *ctx, const double *u, double *du, double t);          /* host ptrs */
+int fr2d_euler_rhs_dev(fr2d_ctx *ctx, const double *d_u, double *d_du, double t);  /* device ptrs */
+
+/* Device-resident state for the integrator path (replaces ODEProblem/init/step!,
+ * shock-vortex.jl:293-305). */
+int fr2d_set_state(fr2d_ctx *ctx, const double *u);  /* host -> device */
+int fr2d_get_state(fr2d_ctx *ctx, double *u);        /* device -> host (synchronises) */
+int fr2d_set_state_dev(fr2d_ctx *ctx, const double *d_u);
+int fr2d_get_state_dev(fr2d_ctx *ctx, double *d_u);
+double *fr2d_state_dev(fr2d_ctx *ctx);               /* device pointer of the state (or NULL) */
+/* nsteps explicit steps of size dt with scheme FR_RK_*; each stage is one fused residual+update
+ * launch.  In FR_BC_GHOST mode the ghost layer is held fixed (L = 0 there). */
+int fr2d_step(fr2d_ctx *ctx, int scheme, double dt, int nsteps);
+/* capture the launches of nsteps steps into a hipGraph and replay it nrep times */
+int fr2d_step_graph(fr2d_ctx *ctx, int scheme, double dt, int nsteps, int nrep);
+/* refresh the ghost layer of the device state (FR_GHOST_*) */
+int fr2d_fill_ghost(fr2d_ctx *ctx, int rule);
+
+/* Per-step dissipation of example/shock-vortex.jl:296-321 on the device state:
+ * modal sensor (u_hat = iV * rho, su = u_hat[end]^2 / sum u_hat^2, shock_detector(log10 su, p,
+ * S0, kappa)) and KitBase's L2 modal filter with strength lambda on shocked cells.
+ * mode FR_SENSOR_LEARNED uses the MLP loaded by fr2d_set_sensor_mlp (strength = its output).
+ * If n_flagged is non-NULL it receives the number of filtered cells (synchronises). */
+int fr2d_sensor_filter(fr2d_ctx *ctx, int mode, double lambda, double S0, double kappa,
+                       int *n_flagged);
+/* Positivity limiter of src/dissipation.jl:125-206 on every interior cell of the device state,
+ * weights wp/4 (shock-vortex.jl:298-303).  Returns FR_E_LIMITER if an element fails the
+ * reference's t1 assertion (that element is left unchanged). */
+int fr2d_positivity(fr2d_ctx *ctx);
+/* Scan the device state for NaN/Inf (FR_E_NONFINITE) or rho <= 0 / p <= 0 (FR_E_NEGATIVE_STATE). */
+int fr2d_check_state(fr2d_ctx *ctx);
+
+/* ------------------------------------------------------------------ 1D (FREulerProblem / FRAdvectionProblem) */
+typedef struct fr1d_ctx fr1d_ctx;
+enum { FR_EQ_EULER = 0, FR_EQ_ADVECTION = 1 };
+enum { FR_BC1_DIRICHLET = 0, FR_BC1_PERIOD = 1 };
+typedef struct {
+  int equation;      /* FR_EQ_* */
+  int ncell, deg;    /* 1 <= deg <= 4 */
+  double dx;         /* uniform cell size; J = dx/2 (src/struct.jl:42) */
+  double gamma;      /* Euler */
+  double a;          /* advection speed */
+  int flux;          /* Euler Riemann flux, FR_FLUX_* */
+  int bc;            /* FR_BC1_* (src/Equation/eq_euler.jl:68-98, eq_advection.jl:251-273) */
+  int correction;    /* FR_CORR_* (src/struct.jl:63) */
+} fr1d_desc;
+int fr1d_create(fr1d_ctx **out, const fr1d_desc *desc);
+int fr1d_destroy(fr1d_ctx *ctx);
+size_t fr1d_state_len(const fr1d_ctx *ctx);
+/* frode_euler!(du, u, p, t) (src/Equation/eq_euler.jl:29-73) or
+ * frode_advection!(du, u, p, t) (src/Equation/eq_advection.jl:153-175) */
+int fr1d_rhs(fr1d_ctx *ctx, const double *u, double *du, double t);
+int fr1d_rhs_dev(fr1d_ctx *ctx, const double *d_u, double *d_du, double t);
+int fr1d_set_state(fr1d_ctx *ctx, const double *u);
+int fr1d_get_state(fr1d_ctx *ctx, double *u);
+int fr1d_step(fr1d_ctx *ctx, int scheme, double dt, int nsteps);
+
+/* ------------------------------------------------------------------ batched 1D Euler ensembles */
+typedef struct fr_ens_ctx fr_ens_ctx;
+/* nprob independent 1D Euler problems (layout of each member as fr1d); member b has ncell[b]
+ * cells of degree deg[b] (1..4) and cell size dx[b]. */
+int fr_ens_create(fr_ens_ctx **out, int nprob, const int *ncell, const int *deg, const double *dx,
+                  double gamma, int flux, int bc);
+int fr_ens_destroy(fr_ens_ctx *ctx);
+size_t fr_ens_state_len(const fr_ens_ctx *ctx);          /* total doubles over all members */
+size_t fr_ens_offset(const fr_ens_ctx *ctx, int member); /* first double of a member */
+int fr_ens_set_state(fr_ens_ctx *ctx, const double *u);  /* concatenated member states */
+int fr_ens_get_state(fr_ens_ctx *ctx, double *u);
+/* mode 0: one batched launch per stage; 1: one launch per member per stage (looped baseline) */
+int fr_ens_step(fr_ens_ctx *ctx, int mode, double dt, int nsteps);
+/* capture nsteps batched steps in a hipGraph and replay nrep times */
+int fr_ens_step_graph(fr_ens_ctx *ctx, double dt, int nsteps, int nrep);
+/* time candidate work mappings on the device and keep the fastest; returns its index */
+int fr_ens_autotune(fr_ens_ctx *ctx, double dt, int *chosen);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,346 @@
+"""ctypes view of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: the checker for tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py.  The product package never imports this module.
+Arrays follow the reference's Julia column-major layout (see fr_oracle.h).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+FLUX = {"hll": 0, "rusanov": 1, "roe": 2}
+BC = {"ghost": 0, "periodic": 1}
+CORR = {"radau": 0, "sd": 1, "huynh": 2}
+
+_dp = C.POINTER(C.c_double)
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return os.path.join(_HERE, "liboracle.so")
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(path):
+            build()
+        L = C.CDLL(path)
+        L.or_legendre_p.restype = C.c_double
+        L.or_dlegendre.restype = C.c_double
+        L.or_jacobi_p.restype = C.c_double
+        L.or_djacobi_p.restype = C.c_double
+        L.or_jacobi_p.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
+        L.or_djacobi_p.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int]
+        L.or_legendre_p.argtypes = [C.c_int, C.c_double]
+        L.or_dlegendre.argtypes = [C.c_int, C.c_double]
+        L.or_fr2d_len.restype = C.c_size_t
+        L.or_shock_detector.argtypes = [C.c_double, C.c_int, C.c_double, C.c_double]
+        L.or_shock_detector.restype = C.c_int
+        for name in ("or_L1_error", "or_L2_error", "or_Linf_error"):
+            getattr(L, name).restype = C.c_double
+            getattr(L, name).argtypes = [_dp, _dp, C.c_size_t, C.c_double]
+        L.or_num_threads.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and (a.flags["C_CONTIGUOUS"] or a.flags["F_CONTIGUOUS"])
+    return a.ctypes.data_as(_dp)
+
+
+def _f64(a) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+
+
+# ----------------------------------------------------------------- operators
+def gauss_legendre(n: int):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().or_gauss_legendre(C.c_int(n), _p(x), _p(w))
+    return x, w
+
+
+def dlegendre(n: int, x: float) -> float:
+    return lib().or_dlegendre(n, float(x))
+
+
+def dcorrection(kind: str, p: int, x):
+    x = _f64(x)
+    dgl = np.zeros_like(x)
+    dgr = np.zeros_like(x)
+    lib().or_dcorrection(C.c_int(CORR[kind]), C.c_int(p), _p(x), C.c_int(x.size), _p(dgl), _p(dgr))
+    return dgl, dgr
+
+
+def lagrange_point(sp, x: float):
+    sp = _f64(sp)
+    out = np.zeros_like(sp)
+    lib().or_lagrange_point(_p(sp), C.c_int(sp.size), C.c_double(x), _p(out))
+    return out
+
+
+def dlagrange(sp):
+    sp = _f64(sp)
+    n = sp.size
+    out = np.zeros(n * n)
+    lib().or_dlagrange(_p(sp), C.c_int(n), _p(out))
+    return out.reshape((n, n), order="F")
+
+
+def jacobi_p(x: float, a: float, b: float, N: int) -> float:
+    return lib().or_jacobi_p(float(x), float(a), float(b), int(N))
+
+
+def vandermonde_1d(N: int, r):
+    r = _f64(r)
+    V = np.zeros(r.size * (N + 1))
+    lib().or_vandermonde_1d(C.c_int(N), _p(r), C.c_int(r.size), _p(V))
+    return V.reshape((r.size, N + 1), order="F")
+
+
+def dvandermonde_1d(N: int, r):
+    r = _f64(r)
+    V = np.zeros(r.size * (N + 1))
+    lib().or_dvandermonde_1d(C.c_int(N), _p(r), C.c_int(r.size), _p(V))
+    return V.reshape((r.size, N + 1), order="F")
+
+
+def vandermonde_quad(N: int, r, s):
+    r = _f64(r)
+    s = _f64(s)
+    V = np.zeros(r.size * (N + 1) ** 2)
+    lib().or_vandermonde_quad(C.c_int(N), _p(r), _p(s), C.c_int(r.size), _p(V))
+    return V.reshape((r.size, (N + 1) ** 2), order="F")
+
+
+def inv(A):
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[0]
+    a = _f64(A)
+    out = np.zeros(n * n)
+    rc = lib().or_inv(C.c_int(n), _p(a), _p(out))
+    assert rc == 0
+    return out.reshape((n, n), order="F")
+
+
+def rs_jacobi_quad(r: float, s: float, vert):
+    v = _f64(np.asarray(vert, dtype=np.float64))
+    J = np.zeros(4)
+    lib().or_rs_jacobi_quad(C.c_double(r), C.c_double(s), _p(v), _p(J))
+    return J.reshape((2, 2), order="F")
+
+
+class Ops:
+    """Operator set of FRPSpace1D/2D computed by the oracle (src/struct.jl:40-85,130-227)."""
+
+    def __init__(self, deg: int, correction: str = "radau"):
+        n = deg + 1
+        self.deg = deg
+        self.nsp = n
+        self.r, self.w = gauss_legendre(n)
+        self.ll = lagrange_point(self.r, -1.0)
+        self.lr = lagrange_point(self.r, 1.0)
+        self.dl = dlagrange(self.r)
+        self.dhl, self.dhr = dcorrection(correction, deg, self.r)
+        self.V1 = vandermonde_1d(deg, self.r)
+        self.iV1 = inv(self.V1)
+        r2 = np.repeat(self.r[:, None], n, axis=1)  # r2d[:, j] = r
+        s2 = np.repeat(self.r[None, :], n, axis=0)  # s2d[i, :] = r
+        self.V = vandermonde_quad(deg, r2.ravel(order="F"), s2.ravel(order="F"))
+        self.iV = inv(self.V)
+        self.wp = np.outer(self.w, self.w)
+
+
+# ----------------------------------------------------------------- physics
+def riemann2(kind: str, wL, wR, g: float):
+    a = _f64(wL)
+    b = _f64(wR)
+    out = np.zeros(4)
+    lib().or_riemann2(C.c_int(FLUX[kind]), _p(a), _p(b), C.c_double(g), _p(out))
+    return out
+
+
+def riemann1(kind: str, wL, wR, g: float):
+    a = _f64(wL)
+    b = _f64(wR)
+    out = np.zeros(3)
+    lib().or_riemann1(C.c_int(FLUX[kind]), _p(a), _p(b), C.c_double(g), _p(out))
+    return out
+
+
+def euler_flux2(w, g: float):
+    a = _f64(w)
+    F = np.zeros(4)
+    G = np.zeros(4)
+    lib().or_euler_flux2(_p(a), C.c_double(g), _p(F), _p(G))
+    return F, G
+
+
+def prim_conserve2(prim, g: float):
+    a = _f64(prim)
+    out = np.zeros(4)
+    lib().or_prim_conserve2(_p(a), C.c_double(g), _p(out))
+    return out
+
+
+def conserve_prim2(w, g: float):
+    a = _f64(w)
+    out = np.zeros(4)
+    lib().or_conserve_prim2(_p(a), C.c_double(g), _p(out))
+    return out
+
+
+# ----------------------------------------------------------------- 2D residual
+class OrFr2d(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int),
+        ("ny", C.c_int),
+        ("nsp", C.c_int),
+        ("gamma", C.c_double),
+        ("flux", C.c_int),
+        ("bc", C.c_int),
+        ("iJ", C.c_double * 4),
+        ("ll", _dp),
+        ("lr", _dp),
+        ("dl", _dp),
+        ("dhl", _dp),
+        ("dhr", _dp),
+    ]
+
+
+class Fr2d:
+    """Oracle 2D Euler residual on a uniform rectangular grid (ghost layer of one cell)."""
+
+    def __init__(self, nx, ny, deg, dx, dy, gamma=1.4, flux="hll", bc="ghost", ops: Ops | None = None):
+        self.ops = ops or Ops(deg)
+        self.nx, self.ny, self.nsp = nx, ny, deg + 1
+        self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
+                      (self.ops.ll, self.ops.lr, self.ops.dl.ravel(order="F"), self.ops.dhl, self.ops.dhr)]
+        d = OrFr2d()
+        d.nx, d.ny, d.nsp = nx, ny, deg + 1
+        d.gamma = gamma
+        d.flux = FLUX[flux]
+        d.bc = BC[bc]
+        d.iJ[0] = 1.0 / (dx / 2.0)
+        d.iJ[1] = 0.0
+        d.iJ[2] = 0.0
+        d.iJ[3] = 1.0 / (dy / 2.0)
+        d.ll, d.lr, d.dl, d.dhl, d.dhr = [_p(a) for a in self._keep]
+        self.d = d
+        self.shape = (nx + 2, ny + 2, deg + 1, deg + 1, 4)
+
+    def _u(self, u):
+        assert u.shape == self.shape and u.dtype == np.float64 and u.flags["F_CONTIGUOUS"]
+        return u
+
+    def rhs(self, u):
+        u = self._u(u)
+        du = np.zeros(self.shape, order="F")
+        lib().or_fr2d_rhs(C.byref(self.d), _p(u), _p(du))
+        return du
+
+    def fill_periodic(self, u):
+        lib().or_fr2d_fill_periodic(C.byref(self.d), _p(self._u(u)))
+
+    def ssprk3(self, u, dt, nsteps):
+        lib().or_fr2d_ssprk3(C.byref(self.d), _p(self._u(u)), C.c_double(dt), C.c_int(nsteps))
+
+    def rk4(self, u, dt, nsteps):
+        lib().or_fr2d_rk4(C.byref(self.d), _p(self._u(u)), C.c_double(dt), C.c_int(nsteps))
+
+    def midpoint(self, u, dt, nsteps):
+        lib().or_fr2d_midpoint(C.byref(self.d), _p(self._u(u)), C.c_double(dt), C.c_int(nsteps))
+
+    def stage(self, u0, us, a0, a1, b):
+        out = np.zeros(self.shape, order="F")
+        lib().or_fr2d_stage(C.byref(self.d), _p(self._u(u0)), _p(self._u(us)), _p(out),
+                            C.c_double(a0), C.c_double(a1), C.c_double(b))
+        return out
+
+    def sensor_filter(self, u, lam=5e-4, S0=None, kappa=9.0):
+        deg = self.nsp - 1
+        if S0 is None:
+            S0 = -3.0 * np.log10(deg)
+        V = np.ascontiguousarray(self.ops.V.ravel(order="F"))
+        iV = np.ascontiguousarray(self.ops.iV.ravel(order="F"))
+        flags = np.zeros((self.nx + 2) * (self.ny + 2), dtype=np.int32)
+        n = lib().or_fr2d_sensor_filter(C.byref(self.d), _p(V), _p(iV), _p(self._u(u)), C.c_double(lam),
+                                         C.c_double(S0), C.c_double(kappa),
+                                         flags.ctypes.data_as(C.POINTER(C.c_int)))
+        return n, flags.reshape((self.nx + 2, self.ny + 2), order="F")
+
+    def positivity(self, u):
+        wp4 = np.ascontiguousarray((self.ops.wp / 4.0).ravel(order="F"))
+        return lib().or_fr2d_positivity(C.byref(self.d), _p(wp4), _p(self._u(u)))
+
+
+def positive_limiter2d(ue, g, weights, ll, lr):
+    """One element ue(k, l, m); returns (rc, limited copy)."""
+    a = np.array(ue, dtype=np.float64, order="F")
+    n = a.shape[0]
+    w = _f64(weights)
+    rc = lib().or_positive_limiter2d(_p(a), C.c_int(n), C.c_double(g), _p(w), _p(_f64(ll)), _p(_f64(lr)))
+    return rc, a
+
+
+def shock_detector(Se, deg, S0=None, kappa=4.0) -> bool:
+    if S0 is None:
+        S0 = -3.0 * np.log10(deg)
+    return bool(lib().or_shock_detector(float(Se), int(deg), float(S0), float(kappa)))
+
+
+def modal_filter_l2(u, lam):
+    a = _f64(u).copy()
+    lib().or_modal_filter_l2(_p(a), C.c_int(a.size), C.c_double(lam))
+    return a
+
+
+# ----------------------------------------------------------------- 1D residuals
+def fr1d_euler_rhs(u, J, ops: Ops, g, flux="hll", bc="dirichlet"):
+    u = np.asfortranarray(u, dtype=np.float64)
+    ncell, nsp, _ = u.shape
+    du = np.zeros_like(u, order="F")
+    Jv = _f64(J)
+    dl = _f64(ops.dl)
+    lib().or_fr1d_euler_rhs(C.c_int(ncell), C.c_int(nsp), _p(Jv), _p(_f64(ops.ll)), _p(_f64(ops.lr)), _p(dl),
+                            _p(_f64(ops.dhl)), _p(_f64(ops.dhr)), C.c_double(g), C.c_int(FLUX[flux]),
+                            C.c_int(0 if bc == "dirichlet" else 1), _p(u), _p(du))
+    return du
+
+
+def fr1d_advection_rhs(u, J, ops: Ops, a=1.0, bc="period"):
+    u = np.asfortranarray(u, dtype=np.float64)
+    ncell, nsp = u.shape
+    du = np.zeros_like(u, order="F")
+    lib().or_fr1d_advection_rhs(C.c_int(ncell), C.c_int(nsp), _p(_f64(J)), _p(_f64(ops.ll)), _p(_f64(ops.lr)),
+                                _p(_f64(ops.dl)), _p(_f64(ops.dhl)), _p(_f64(ops.dhr)), C.c_double(a),
+                                C.c_int(0 if bc == "dirichlet" else 1), _p(u), _p(du))
+    return du
+
+
+def L1_error(u, ue, dx):
+    a = _f64(u)
+    b = _f64(ue)
+    return lib().or_L1_error(_p(a), _p(b), a.size, dx)
+
+
+def L2_error(u, ue, dx):
+    a = _f64(u)
+    b = _f64(ue)
+    return lib().or_L2_error(_p(a), _p(b), a.size, dx)
+
+
+def Linf_error(u, ue, dx):
+    a = _f64(u)
+    b = _f64(ue)
+    return lib().or_Linf_error(_p(a), _p(b), a.size, dx)

@@ -1,0 +1,173 @@
+"""GPU parity of the 2D Euler residual, stages and dissipation kernels (libfrhip, gfx950)
+against the CPU oracle on the same seeded inputs.
+
+Tolerances: residual ||du_gpu - du_cpu||_inf / ||du_cpu||_inf <= 1e-12; solution after the
+stepping runs ||u_gpu - u_cpu||_2 / ||u_cpu||_2 <= 1e-10 (FP64 throughout).
+"""
+import numpy as np
+import pytest
+
+import frhip
+from frhip import ics
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RES_TOL = 1e-12
+STATE_TOL = 1e-10
+
+
+def rel_inf(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def setup(nx, ny, deg, flux, bc, L=10.0, seed=11, amp=0.01):
+    ps = frhip.FRPSpace2D(0.0, L, nx, 0.0, L, ny, deg, 1, 1)
+    u = ics.isentropic_vortex(ps, 1.4, strength=5.0)
+    if amp:
+        u = ics.perturbed(u, seed=seed, amplitude=amp)
+    solver = frhip.FREuler2D(ps, 1.4, flux, bc)
+    orc = O.Fr2d(nx, ny, deg, L / nx, L / ny, 1.4, flux, bc)
+    return ps, u, solver, orc
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("flux", ["hll", "rusanov", "roe"])
+@pytest.mark.parametrize("bc", ["ghost", "periodic"])
+def test_residual_parity(deg, flux, bc):
+    # 19 x 7 leaves partial tiles in both directions
+    ps, u, s, orc = setup(19, 7, deg, flux, bc)
+    du = s.rhs(u)
+    ref = orc.rhs(u)
+    assert rel_inf(du, ref) <= RES_TOL
+    assert np.all(du[0] == 0) and np.all(du[-1] == 0) and np.all(du[:, 0] == 0) and np.all(du[:, -1] == 0)
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (16, 4), (17, 5), (32, 12), (48, 3)])
+def test_residual_parity_shapes(nx, ny):
+    ps, u, s, orc = setup(nx, ny, 3, "hll", "ghost", seed=nx * 100 + ny)
+    assert rel_inf(s.rhs(u), orc.rhs(u)) <= RES_TOL
+
+
+def test_residual_garbage_du_overwritten():
+    ps, u, s, orc = setup(20, 9, 3, "rusanov", "periodic")
+    du = np.full(s.shape, np.nan, order="F")
+    s.dudt(du, u, None, 0.0)
+    assert np.all(np.isfinite(du))
+    assert rel_inf(du, orc.rhs(u)) <= RES_TOL
+
+
+def test_residual_device_entry_point():
+    import torch
+
+    ps, u, s, orc = setup(40, 24, 3, "rusanov", "periodic")
+    s.use_torch_stream()
+    ud = torch.from_numpy(u.ravel(order="F").copy()).cuda()
+    dud = torch.full_like(ud, 7.0)
+    s.dudt(dud, ud, None, 0.0)
+    torch.cuda.synchronize()
+    du = dud.cpu().numpy().reshape(s.shape, order="F")
+    assert rel_inf(du, orc.rhs(u)) <= RES_TOL
+
+
+def test_c2_full_size_residual():
+    """256 x 256, p = 3, Rusanov, periodic: the benchmark configuration."""
+    ps, u, s, orc = setup(256, 256, 3, "rusanov", "periodic", amp=0.0)
+    du = s.rhs(u)
+    ref = orc.rhs(u)
+    assert rel_inf(du, ref) <= RES_TOL
+    # discrete conservation on the periodic grid: sum_k,l wp du = 0 for every variable
+    w = ps.wp
+    tot = np.einsum("ijklm,kl->m", du[1:-1, 1:-1], w)
+    scale = np.einsum("ijklm,kl->m", np.abs(du[1:-1, 1:-1]), w)
+    assert np.all(np.abs(tot) <= 1e-12 * scale)
+
+
+@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler"])
+@pytest.mark.parametrize("bc", ["ghost", "periodic"])
+def test_stepping_parity(scheme, bc):
+    ps, u, s, orc = setup(24, 20, 3, "rusanov", bc, amp=0.0)
+    dt, _ = ics.cfl_steps(u, 1.4, ps.dx0, 0.05, 1.0)
+    s.set_state(u)
+    s.step(dt, 10, scheme)
+    ug = s.get_state()
+    uc = u.copy(order="F")
+    if scheme == "ssprk3":
+        orc.ssprk3(uc, dt, 10)
+    elif scheme == "rk4":
+        orc.rk4(uc, dt, 10)
+    elif scheme == "midpoint":
+        orc.midpoint(uc, dt, 10)
+    else:
+        for _ in range(10):
+            uc = orc.stage(uc, uc, 0.0, 1.0, dt)
+        if bc == "periodic":
+            orc.fill_periodic(uc)
+    assert rel_l2(ug, uc) <= STATE_TOL
+
+
+def test_graph_replay_matches_eager():
+    ps, u, s, orc = setup(32, 16, 3, "rusanov", "periodic", amp=0.0)
+    dt, _ = ics.cfl_steps(u, 1.4, ps.dx0, 0.05, 1.0)
+    s.set_state(u)
+    s.step(dt, 6)
+    eager = s.get_state()
+    s2 = frhip.FREuler2D(ps, 1.4, "rusanov", "periodic")
+    s2.set_state(u)
+    for _ in range(3):  # three replays of a 2-step graph, with host synchronisations between
+        s2.step_graph(dt, 2, 1)
+    graph = s2.get_state()
+    assert np.array_equal(eager, graph)
+
+
+def test_sensor_filter_parity():
+    ps = frhip.FRPSpace2D(0.0, 2.0, 40, 0.0, 1.0, 20, 3, 1, 1)
+    u = ics.shock_vortex(ps, 1.4)
+    u = ics.perturbed(u, seed=5, amplitude=0.02)
+    s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
+    orc = O.Fr2d(40, 20, 3, 2.0 / 40, 1.0 / 20, 1.4, "hll", "ghost")
+    s.set_state(u)
+    n_gpu = s.sensor_filter("modal", 5e-4, kappa=9.0)
+    uc = u.copy(order="F")
+    n_cpu, _ = orc.sensor_filter(uc, 5e-4, kappa=9.0)
+    assert n_gpu == n_cpu and n_cpu > 0
+    assert rel_inf(s.get_state(), uc) <= 1e-13
+
+
+def test_positivity_parity():
+    ps = frhip.FRPSpace2D(0.0, 1.0, 12, 0.0, 1.0, 10, 3, 1, 1)
+    u = ics.isentropic_vortex(ps, 1.4, strength=5.0, center=(0.5, 0.5))
+    rng = np.random.default_rng(3)
+    # push a few solution points below zero density / pressure
+    for _ in range(6):
+        i, j, k, l = rng.integers(1, 11), rng.integers(1, 9), rng.integers(0, 4), rng.integers(0, 4)
+        u[i, j, k, l, 0] *= -0.2
+        u[i, j, k, l, 3] *= 0.05
+    s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
+    orc = O.Fr2d(12, 10, 3, 1 / 12, 1 / 10, 1.4, "hll", "ghost")
+    s.set_state(u)
+    s.positivity()
+    uc = u.copy(order="F")
+    bad = orc.positivity(uc)
+    assert bad == 0
+    assert rel_inf(s.get_state(), uc) <= 1e-12
+    s.check_state()
+
+
+def test_check_state_flags_negative_and_nan():
+    ps, u, s, orc = setup(16, 8, 2, "hll", "ghost", amp=0.0)
+    bad = u.copy(order="F")
+    bad[3, 3, 0, 0, 0] = -1.0
+    s.set_state(bad)
+    with pytest.raises(frhip.FRError) as e:
+        s.check_state()
+    assert e.value.code == 4
+    bad[3, 3, 0, 0, 0] = np.nan
+    s.set_state(bad)
+    with pytest.raises(frhip.FRError) as e:
+        s.check_state()
+    assert e.value.code == 3
