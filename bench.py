@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the full run to t = 2)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 10 plain device copies of 64 MiB (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only; multi-GPU runs are not part of it "
@@ -125,6 +127,13 @@ def main():
         e1.record(stream)
         torch.cuda.synchronize()
     res_s = e0.elapsed_time(e1) * 1e-3 / nres
+
+    if args.copy_calibration:
+        src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(10):
+            dst.copy_(src)
+        torch.cuda.synchronize()
 
     cpu = None if args.no_cpu else cpu_baseline(u0, dt)
     mass0 = float(np.einsum("ijkl,kl->", u0[1:-1, 1:-1, :, :, 0], ps.wp))

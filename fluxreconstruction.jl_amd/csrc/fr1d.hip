@@ -1,0 +1,586 @@
+// fr1d.hip -- 1D FR residuals (src/Equation/eq_euler.jl:29-98, eq_advection.jl:153-273,
+// eq_scalar.jl:1-12) and batched ensembles of independent 1D Euler problems, for gfx950.
+//
+// One kernel body handles a chunk of consecutive elements of one problem: lane t of a workgroup
+// of T threads owns element first-1+t, so lanes 0 and T-1 are halo elements that only provide
+// their face traces; the T-2 interior lanes produce du.  Traces and interface fluxes go through
+// LDS.  A single problem is a work list of its chunks; an ensemble is the concatenated work list
+// of all members (any degree 1..4 per member) processed by a persistent grid.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "fr_common.h"
+
+namespace fr {
+
+constexpr int NSP_MAX = 5;
+constexpr int OPS_STRIDE = 4 * NSP_MAX + NSP_MAX * NSP_MAX;  // ll, lr, dhl, dhr, dl
+
+// device operator table: entry nsp (2..5) at OPS_STRIDE*(nsp-2): ll[5] lr[5] dhl[5] dhr[5] dl[25]
+struct Prob1D {
+  long long off;  // first double of the member in the concatenated state
+  int ncell, nsp;
+  double iJ;      // 2/dx
+};
+
+struct Work1D {
+  int prob, first;  // member, first interior element of the chunk
+};
+
+struct Stage1D {
+  const double *u;
+  double *out;
+  const double *u0;
+  const double *acc_in;
+  double *acc_out;
+  double a0, a1, b, b2;
+  int residual;
+};
+
+struct Phys1D {
+  int eq;       // FR_EQ_EULER / FR_EQ_ADVECTION
+  int bc;       // FR_BC1_*
+  double g, gm1, a;
+};
+
+template <int NSP, int EQ, int FLUX>
+__device__ __forceinline__ void chunk1d(const Prob1D &pb, int first, const double *__restrict__ ops,
+                                        const Phys1D &ph, const Stage1D &sa, double *sh) {
+  constexpr int NV = (EQ == FR_EQ_EULER) ? 3 : 1;
+  const int T = blockDim.x;
+  const int tid = threadIdx.x;
+  const int n = pb.ncell;
+  const double *ll = ops, *lr = ops + NSP_MAX, *dhl = ops + 2 * NSP_MAX, *dhr = ops + 3 * NSP_MAX,
+               *dl = ops + 4 * NSP_MAX;
+  // shared layout: uR[NV][T] | fR[NV][T] | Fs[NV][T]
+  double *sUR = sh, *sFR = sh + NV * T, *sFS = sh + 2 * NV * T;
+  int e = first - 1 + tid;
+  bool valid = true;
+  if (e < 0 || e >= n) {
+    // only the two neighbours of the domain wrap (periodic); lanes further out are unused
+    if (ph.bc == FR_BC1_PERIOD && (e == -1 || e == n))
+      e = (e < 0) ? e + n : e - n;
+    else
+      valid = false;
+  }
+  const bool interior = tid >= 1 && tid <= T - 2 && (first - 1 + tid) < n;
+  const double *__restrict__ u = sa.u + pb.off;
+  double w[NSP][NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+#pragma unroll
+    for (int p = 0; p < NSP; ++p) w[p][k] = valid ? u[e + (long long)n * (p + NSP * k)] : (k == 0 ? 1.0 : (k == 2 ? 2.5 : 0.0));
+  // discontinuous flux f = F(u) / J, traces of u and f
+  double f[NSP][NV], uL[NV], uR[NV], fL[NV], fR[NV];
+#pragma unroll
+  for (int p = 0; p < NSP; ++p) {
+    if constexpr (EQ == FR_EQ_EULER) {
+      const double r = 1.0 / w[p][0];
+      const double vel = w[p][1] * r;
+      const double pr = ph.gm1 * (w[p][2] - 0.5 * w[p][1] * vel);
+      f[p][0] = w[p][1] * pb.iJ;
+      f[p][1] = (w[p][1] * vel + pr) * pb.iJ;
+      f[p][2] = ((w[p][2] + pr) * vel) * pb.iJ;
+    } else {
+      f[p][0] = ph.a * w[p][0] * pb.iJ;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double a = 0, b = 0, c = 0, d = 0;
+#pragma unroll
+    for (int q = 0; q < NSP; ++q) {
+      a += w[q][k] * ll[q];
+      b += w[q][k] * lr[q];
+      c += f[q][k] * ll[q];
+      d += f[q][k] * lr[q];
+    }
+    uL[k] = a;
+    uR[k] = b;
+    fL[k] = c;
+    fR[k] = d;
+    sUR[k * T + tid] = b;
+    sFR[k * T + tid] = d;
+  }
+  __syncthreads();
+  // interface flux on the left face of the own element (face between e-1 and e)
+  if (tid >= 1 && valid) {
+    const bool nb_ok = (ph.bc == FR_BC1_PERIOD) || (first - 2 + tid) >= 0;
+    if (nb_ok) {
+      if constexpr (EQ == FR_EQ_EULER) {
+        double wl[3], fw[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) wl[k] = sUR[k * T + tid - 1];
+        riemann_x3<FLUX>(wl, uL, ph.g, ph.gm1, fw);
+        // the reference divides the interface flux by J in the correction
+#pragma unroll
+        for (int k = 0; k < 3; ++k) sFS[k * T + tid] = fw[k] * pb.iJ;
+      } else {
+        // regularised local Lax-Friedrichs of eq_advection.jl:128-137 (+1e-6 on the periodic face
+        // :163); f is already divided by J
+        const double fl = sFR[tid - 1], ul = sUR[tid - 1];
+        const int ge = first - 1 + tid;  // global element right of the face
+        const double eps = (ge == 0 || ge == n) ? 1e-6 : 1e-8;
+        const double au = (fL[0] - fl) / (uL[0] - ul + eps);
+        sFS[tid] = 0.5 * (fL[0] + fl) - 0.5 * fabs(au) * (uL[0] - ul);
+      }
+    }
+  }
+  __syncthreads();
+  if (interior) {
+    const int ge = first - 1 + tid;
+    const bool zero = (ph.bc == FR_BC1_DIRICHLET) && (ge == 0 || ge == n - 1);
+    double *__restrict__ out = sa.out + pb.off;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double cl = sFS[k * T + tid] - fL[k];
+      const double cr = sFS[k * T + tid + 1] - fR[k];
+#pragma unroll
+      for (int p = 0; p < NSP; ++p) {
+        double d = 0.0;
+#pragma unroll
+        for (int q = 0; q < NSP; ++q) d += f[q][k] * dl[p + NSP * q];
+        double du = -(d + cl * dhl[p] + cr * dhr[p]);
+        if (zero) du = 0.0;
+        const long long idx = ge + (long long)n * (p + NSP * k);
+        if (sa.residual) {
+          out[idx] = du;
+        } else {
+          double v = sa.a1 * w[p][k];
+          if (sa.a0 != 0.0) v = sa.a0 * sa.u0[pb.off + idx] + v;
+          out[idx] = v + sa.b * du;
+          if (sa.acc_out) sa.acc_out[pb.off + idx] = sa.acc_in[pb.off + idx] + sa.b2 * du;
+        }
+      }
+    }
+  }
+  __syncthreads();  // the shared buffers are reused by the next chunk of a persistent loop
+}
+
+template <int EQ, int FLUX>
+__global__ __launch_bounds__(256) void fr1d_kernel(const Prob1D *__restrict__ probs, const Work1D *__restrict__ work,
+                                                   int nwork, const double *__restrict__ ops, Phys1D ph, Stage1D sa) {
+  constexpr int NV = (EQ == FR_EQ_EULER) ? 3 : 1;
+  __shared__ double sh[3 * NV * 256];
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const Work1D wk = work[wi];
+    const Prob1D pb = probs[wk.prob];
+    const double *o = ops + OPS_STRIDE * (pb.nsp - 2);
+    switch (pb.nsp) {
+      case 2: chunk1d<2, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+      case 3: chunk1d<3, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+      case 4: chunk1d<4, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+      default: chunk1d<5, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ batched problem set
+struct Batch1D {
+  int eq = FR_EQ_EULER, flux = FR_FLUX_HLL, bc = FR_BC1_DIRICHLET, correction = FR_CORR_RADAU;
+  double gamma = 1.4, a = 1.0;
+  std::vector<Prob1D> probs;
+  size_t len = 0;
+  hipStream_t stream = nullptr;
+  Prob1D *d_probs = nullptr;
+  double *d_ops = nullptr;
+  double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_acc = nullptr;
+  double *d_in = nullptr, *d_out = nullptr;
+  // work lists per thread count (64, 128, 256): concatenated list and per-member sub-lists
+  int threads = 256, grid = 0;  // grid 0 = one workgroup per work item
+  Work1D *d_work[3] = {nullptr, nullptr, nullptr};
+  int nwork[3] = {0, 0, 0};
+  std::vector<int> member_first[3], member_count[3];
+
+  int build(int flux_, int eq_) {
+    flux = flux_;
+    eq = eq_;
+    len = 0;
+    for (auto &p : probs) {
+      p.off = (long long)len;
+      len += (size_t)p.ncell * p.nsp * (eq == FR_EQ_EULER ? 3 : 1);
+    }
+    std::vector<double> tab(OPS_STRIDE * 4, 0.0);
+    for (int nsp = 2; nsp <= 5; ++nsp) {
+      HostOps o;
+      if (!build_ops(nsp - 1, correction, o)) return fail(FR_E_INVALID, "fr1d: operator construction failed");
+      double *t = tab.data() + OPS_STRIDE * (nsp - 2);
+      for (int q = 0; q < nsp; ++q) {
+        t[q] = o.ll[q];
+        t[NSP_MAX + q] = o.lr[q];
+        t[2 * NSP_MAX + q] = o.dhl[q];
+        t[3 * NSP_MAX + q] = o.dhr[q];
+      }
+      for (int q = 0; q < nsp * nsp; ++q) t[4 * NSP_MAX + q] = o.dl[q];
+    }
+    FR_HIP(hipMalloc(&d_ops, tab.size() * sizeof(double)));
+    FR_HIP(hipMemcpy(d_ops, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+    FR_HIP(hipMalloc(&d_probs, probs.size() * sizeof(Prob1D)));
+    FR_HIP(hipMemcpy(d_probs, probs.data(), probs.size() * sizeof(Prob1D), hipMemcpyHostToDevice));
+    const int Ts[3] = {64, 128, 256};
+    for (int v = 0; v < 3; ++v) {
+      const int per = Ts[v] - 2;
+      std::vector<Work1D> wl;
+      member_first[v].assign(probs.size(), 0);
+      member_count[v].assign(probs.size(), 0);
+      for (size_t b = 0; b < probs.size(); ++b) {
+        member_first[v][b] = (int)wl.size();
+        for (int f = 0; f < probs[b].ncell; f += per) wl.push_back(Work1D{(int)b, f});
+        member_count[v][b] = (int)wl.size() - member_first[v][b];
+      }
+      nwork[v] = (int)wl.size();
+      FR_HIP(hipMalloc(&d_work[v], wl.size() * sizeof(Work1D)));
+      FR_HIP(hipMemcpy(d_work[v], wl.data(), wl.size() * sizeof(Work1D), hipMemcpyHostToDevice));
+    }
+    FR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    FR_HIP(hipMalloc(&d_u, len * sizeof(double)));
+    FR_HIP(hipMemset(d_u, 0, len * sizeof(double)));
+    return FR_OK;
+  }
+
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    double *bufs[] = {d_ops, d_u, d_s1, d_s2, d_acc, d_in, d_out};
+    for (double *b : bufs)
+      if (b) (void)hipFree(b);
+    if (d_probs) (void)hipFree(d_probs);
+    for (auto *w : d_work)
+      if (w) (void)hipFree(w);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  static int tindex(int T) { return T == 64 ? 0 : (T == 128 ? 1 : 2); }
+
+  // one launch over work items [w0, w0 + nw) of the list for T threads
+  int launch(const Stage1D &sa, int T, int grid_, int w0, int nw, hipStream_t st) const {
+    const int v = tindex(T);
+    Phys1D ph{eq, bc, gamma, gamma - 1.0, a};
+    const Work1D *wl = d_work[v] + w0;
+    const int g = grid_ > 0 ? std::min(grid_, nw) : nw;
+    if (nw <= 0) return FR_OK;
+    if (eq == FR_EQ_ADVECTION) {
+      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_ADVECTION, FR_FLUX_HLL>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops,
+                         ph, sa);
+    } else if (flux == FR_FLUX_HLL) {
+      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_HLL>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops, ph,
+                         sa);
+    } else if (flux == FR_FLUX_RUSANOV) {
+      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_RUSANOV>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops,
+                         ph, sa);
+    } else {
+      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_ROE>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops, ph,
+                         sa);
+    }
+    FR_LAUNCHED("fr1d_kernel");
+    return FR_OK;
+  }
+
+  // batched (mode 0): whole work list in one launch; looped (mode 1): one launch per member
+  int stage(const Stage1D &sa, int mode, hipStream_t st) const {
+    const int v = tindex(threads);
+    if (mode == 0) return launch(sa, threads, grid, 0, nwork[v], st);
+    for (size_t b = 0; b < probs.size(); ++b) {
+      int rc = launch(sa, threads, 0, member_first[v][b], member_count[v][b], st);
+      if (rc) return rc;
+    }
+    return FR_OK;
+  }
+
+  int ensure_stage_buffers(bool acc) {
+    if (!d_s1) FR_HIP(hipMalloc(&d_s1, len * sizeof(double)));
+    if (!d_s2) FR_HIP(hipMalloc(&d_s2, len * sizeof(double)));
+    if (acc && !d_acc) FR_HIP(hipMalloc(&d_acc, len * sizeof(double)));
+    return FR_OK;
+  }
+
+  int enqueue_steps(int scheme, int mode, double dt, int nsteps, hipStream_t st) {
+    double *U = d_u, *S1 = d_s1, *S2 = d_s2, *A = d_acc;
+    auto stg = [&](const double *u0, const double *us, double *out, double a0, double a1, double b,
+                   const double *ai, double *ao, double b2) {
+      Stage1D s{us, out, u0, ai, ao, a0, a1, b, b2, 0};
+      return stage(s, mode, st);
+    };
+    for (int n = 0; n < nsteps; ++n) {
+      int rc = FR_OK;
+      switch (scheme) {
+        case FR_RK_SSPRK3:
+          if ((rc = stg(U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
+          if ((rc = stg(U, S1, S2, 0.75, 0.25, 0.25 * dt, nullptr, nullptr, 0.0))) return rc;
+          if ((rc = stg(U, S2, U, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, nullptr, nullptr, 0.0))) return rc;
+          break;
+        case FR_RK_RK4:
+          if ((rc = stg(U, U, S2, 1.0, 0.0, 0.5 * dt, U, A, dt / 6.0))) return rc;
+          if ((rc = stg(U, S2, S1, 1.0, 0.0, 0.5 * dt, A, A, dt / 3.0))) return rc;
+          if ((rc = stg(U, S1, S2, 1.0, 0.0, dt, A, A, dt / 3.0))) return rc;
+          if ((rc = stg(A, S2, U, 1.0, 0.0, dt / 6.0, nullptr, nullptr, 0.0))) return rc;
+          break;
+        case FR_RK_MIDPOINT:
+          if ((rc = stg(U, U, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0))) return rc;
+          if ((rc = stg(U, S1, U, 1.0, 0.0, dt, nullptr, nullptr, 0.0))) return rc;
+          break;
+        case FR_RK_EULER:
+          if ((rc = stg(U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
+          FR_HIP(hipMemcpyAsync(U, S1, len * sizeof(double), hipMemcpyDeviceToDevice, st));
+          break;
+        default:
+          return fail(FR_E_INVALID, "fr1d_step: unknown scheme");
+      }
+    }
+    return FR_OK;
+  }
+
+  int residual(const double *d_u_, double *d_du_, hipStream_t st) const {
+    Stage1D s{};
+    s.u = d_u_;
+    s.out = d_du_;
+    s.residual = 1;
+    return stage(s, 0, st);
+  }
+};
+
+}  // namespace fr
+
+using namespace fr;
+
+struct fr1d_ctx : Batch1D {};
+struct fr_ens_ctx : Batch1D {};
+
+extern "C" {
+
+int fr1d_create(fr1d_ctx **out, const fr1d_desc *d) {
+  FR_REQUIRE(out && d, FR_E_INVALID, "fr1d_create: null argument");
+  *out = nullptr;
+  FR_REQUIRE(d->equation == FR_EQ_EULER || d->equation == FR_EQ_ADVECTION, FR_E_INVALID, "fr1d_create: unknown equation");
+  FR_REQUIRE(d->ncell >= 2, FR_E_INVALID, "fr1d_create: ncell must be >= 2");
+  FR_REQUIRE(d->deg >= 1 && d->deg <= 4, FR_E_UNSUPPORTED, "fr1d_create: degree must be in 1..4");
+  FR_REQUIRE(d->dx > 0.0 && std::isfinite(d->dx), FR_E_INVALID, "fr1d_create: dx must be > 0");
+  FR_REQUIRE(d->bc == FR_BC1_DIRICHLET || d->bc == FR_BC1_PERIOD, FR_E_INVALID, "fr1d_create: unknown bc");
+  FR_REQUIRE(d->correction >= FR_CORR_RADAU && d->correction <= FR_CORR_HUYNH, FR_E_INVALID, "fr1d_create: unknown correction");
+  FR_REQUIRE(d->flux >= FR_FLUX_HLL && d->flux <= FR_FLUX_ROE, FR_E_INVALID, "fr1d_create: unknown flux");
+  FR_REQUIRE(d->equation == FR_EQ_ADVECTION || d->gamma > 1.0, FR_E_INVALID, "fr1d_create: gamma must be > 1");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FR_E_HIP, "fr1d_create: no HIP device visible");
+  fr1d_ctx *c = new (std::nothrow) fr1d_ctx();
+  FR_REQUIRE(c, FR_E_NOMEM, "fr1d_create: out of host memory");
+  c->bc = d->bc;
+  c->gamma = d->gamma;
+  c->a = d->a;
+  c->correction = d->correction;
+  c->probs.push_back(Prob1D{0, d->ncell, d->deg + 1, 2.0 / d->dx});
+  int rc = c->build(d->flux, d->equation);
+  if (rc) {
+    c->release();
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return FR_OK;
+}
+
+int fr1d_destroy(fr1d_ctx *c) {
+  if (!c) return FR_OK;
+  c->release();
+  delete c;
+  return FR_OK;
+}
+
+size_t fr1d_state_len(const fr1d_ctx *c) { return c ? c->len : 0; }
+
+int fr1d_rhs_dev(fr1d_ctx *c, const double *d_u, double *d_du, double t) {
+  (void)t;
+  FR_REQUIRE(c && d_u && d_du && d_u != d_du, FR_E_INVALID, "fr1d_rhs_dev: bad arguments");
+  return c->residual(d_u, d_du, c->stream);
+}
+
+int fr1d_rhs(fr1d_ctx *c, const double *u, double *du, double t) {
+  (void)t;
+  FR_REQUIRE(c && u && du, FR_E_INVALID, "fr1d_rhs: null argument");
+  const size_t bytes = c->len * sizeof(double);
+  if (!c->d_in) FR_HIP(hipMalloc(&c->d_in, bytes));
+  if (!c->d_out) FR_HIP(hipMalloc(&c->d_out, bytes));
+  FR_HIP(hipMemcpyAsync(c->d_in, u, bytes, hipMemcpyHostToDevice, c->stream));
+  int rc = c->residual(c->d_in, c->d_out, c->stream);
+  if (rc) return rc;
+  FR_HIP(hipMemcpyAsync(du, c->d_out, bytes, hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr1d_set_state(fr1d_ctx *c, const double *u) {
+  FR_REQUIRE(c && u, FR_E_INVALID, "fr1d_set_state: null argument");
+  FR_HIP(hipMemcpyAsync(c->d_u, u, c->len * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr1d_get_state(fr1d_ctx *c, double *u) {
+  FR_REQUIRE(c && u, FR_E_INVALID, "fr1d_get_state: null argument");
+  FR_HIP(hipMemcpyAsync(u, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr1d_step(fr1d_ctx *c, int scheme, double dt, int nsteps) {
+  FR_REQUIRE(c && nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr1d_step: bad arguments");
+  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr1d_step: unknown scheme");
+  int rc = c->ensure_stage_buffers(scheme == FR_RK_RK4);
+  if (rc) return rc;
+  return c->enqueue_steps(scheme, 0, dt, nsteps, c->stream);
+}
+
+// ------------------------------------------------------------------ ensembles
+int fr_ens_create(fr_ens_ctx **out, int nprob, const int *ncell, const int *deg, const double *dx, double gamma,
+                  int flux, int bc) {
+  FR_REQUIRE(out && ncell && deg && dx && nprob >= 1, FR_E_INVALID, "fr_ens_create: bad arguments");
+  *out = nullptr;
+  FR_REQUIRE(gamma > 1.0, FR_E_INVALID, "fr_ens_create: gamma must be > 1");
+  FR_REQUIRE(flux >= FR_FLUX_HLL && flux <= FR_FLUX_ROE, FR_E_INVALID, "fr_ens_create: unknown flux");
+  FR_REQUIRE(bc == FR_BC1_DIRICHLET || bc == FR_BC1_PERIOD, FR_E_INVALID, "fr_ens_create: unknown bc");
+  for (int b = 0; b < nprob; ++b) {
+    FR_REQUIRE(ncell[b] >= 2, FR_E_INVALID, "fr_ens_create: every member needs ncell >= 2");
+    FR_REQUIRE(deg[b] >= 1 && deg[b] <= 4, FR_E_UNSUPPORTED, "fr_ens_create: member degree must be in 1..4");
+    FR_REQUIRE(dx[b] > 0.0, FR_E_INVALID, "fr_ens_create: member dx must be > 0");
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FR_E_HIP, "fr_ens_create: no HIP device visible");
+  fr_ens_ctx *c = new (std::nothrow) fr_ens_ctx();
+  FR_REQUIRE(c, FR_E_NOMEM, "fr_ens_create: out of host memory");
+  c->bc = bc;
+  c->gamma = gamma;
+  for (int b = 0; b < nprob; ++b) c->probs.push_back(Prob1D{0, ncell[b], deg[b] + 1, 2.0 / dx[b]});
+  int rc = c->build(flux, FR_EQ_EULER);
+  if (rc) {
+    c->release();
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return FR_OK;
+}
+
+int fr_ens_destroy(fr_ens_ctx *c) {
+  if (!c) return FR_OK;
+  c->release();
+  delete c;
+  return FR_OK;
+}
+
+size_t fr_ens_state_len(const fr_ens_ctx *c) { return c ? c->len : 0; }
+
+size_t fr_ens_offset(const fr_ens_ctx *c, int member) {
+  if (!c || member < 0 || member >= (int)c->probs.size()) return (size_t)-1;
+  return (size_t)c->probs[member].off;
+}
+
+int fr_ens_set_state(fr_ens_ctx *c, const double *u) {
+  FR_REQUIRE(c && u, FR_E_INVALID, "fr_ens_set_state: null argument");
+  FR_HIP(hipMemcpyAsync(c->d_u, u, c->len * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr_ens_get_state(fr_ens_ctx *c, double *u) {
+  FR_REQUIRE(c && u, FR_E_INVALID, "fr_ens_get_state: null argument");
+  FR_HIP(hipMemcpyAsync(u, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr_ens_step(fr_ens_ctx *c, int mode, double dt, int nsteps) {
+  FR_REQUIRE(c && (mode == 0 || mode == 1) && nsteps >= 0 && std::isfinite(dt), FR_E_INVALID,
+             "fr_ens_step: bad arguments");
+  int rc = c->ensure_stage_buffers(false);
+  if (rc) return rc;
+  rc = c->enqueue_steps(FR_RK_SSPRK3, mode, dt, nsteps, c->stream);
+  if (rc) return rc;
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr_ens_step_graph(fr_ens_ctx *c, double dt, int nsteps, int nrep) {
+  FR_REQUIRE(c && nsteps >= 1 && nrep >= 0 && std::isfinite(dt), FR_E_INVALID, "fr_ens_step_graph: bad arguments");
+  int rc = c->ensure_stage_buffers(false);
+  if (rc) return rc;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  FR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  rc = c->enqueue_steps(FR_RK_SSPRK3, 0, dt, nsteps, c->stream);
+  hipError_t e = hipStreamEndCapture(c->stream, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  FR_HIP(e);
+  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+  }
+  for (int r = 0; r < nrep && e == hipSuccess; ++r) e = hipGraphLaunch(exec, c->stream);
+  hipError_t e2 = hipStreamSynchronize(c->stream);
+  (void)hipGraphExecDestroy(exec);
+  (void)hipGraphDestroy(graph);
+  FR_HIP(e);
+  FR_HIP(e2);
+  return FR_OK;
+}
+
+// Times candidate (threads per workgroup, grid) mappings with HIP events over a few SSP-RK3
+// steps on a scratch copy of the state, keeps the fastest, and restores the state.
+int fr_ens_autotune(fr_ens_ctx *c, double dt, int *chosen) {
+  FR_REQUIRE(c && std::isfinite(dt), FR_E_INVALID, "fr_ens_autotune: bad arguments");
+  int rc = c->ensure_stage_buffers(false);
+  if (rc) return rc;
+  int ndev = 0, dev = 0, ncu = 256;
+  (void)ndev;
+  FR_HIP(hipGetDevice(&dev));
+  FR_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  double *save = nullptr;
+  FR_HIP(hipMalloc(&save, c->len * sizeof(double)));
+  FR_HIP(hipMemcpyAsync(save, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  const int Ts[3] = {64, 128, 256};
+  const int mult[5] = {0, 1, 2, 4, 8};  // 0: one workgroup per work item
+  hipEvent_t e0, e1;
+  FR_HIP(hipEventCreate(&e0));
+  FR_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_i = 0, idx = 0;
+  const int keepT = c->threads, keepG = c->grid;
+  for (int ti = 0; ti < 3; ++ti)
+    for (int mi = 0; mi < 5; ++mi, ++idx) {
+      c->threads = Ts[ti];
+      c->grid = mult[mi] * ncu;
+      rc = c->enqueue_steps(FR_RK_SSPRK3, 0, dt, 1, c->stream);  // warm-up
+      if (rc) break;
+      FR_HIP(hipEventRecord(e0, c->stream));
+      rc = c->enqueue_steps(FR_RK_SSPRK3, 0, dt, 3, c->stream);
+      if (rc) break;
+      FR_HIP(hipEventRecord(e1, c->stream));
+      FR_HIP(hipEventSynchronize(e1));
+      float ms = 0.f;
+      FR_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_i = idx;
+      }
+      FR_HIP(hipMemcpyAsync(c->d_u, save, c->len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  FR_HIP(hipStreamSynchronize(c->stream));
+  (void)hipFree(save);
+  if (rc) {
+    c->threads = keepT;
+    c->grid = keepG;
+    return rc;
+  }
+  c->threads = Ts[best_i / 5];
+  c->grid = mult[best_i % 5] * ncu;
+  if (chosen) *chosen = best_i;
+  return FR_OK;
+}
+
+}  // extern "C"

@@ -95,7 +95,6 @@ static int finish_step(Fr2dCtx &c, hipStream_t st) {
 
 using namespace fr;
 
-struct fr2d_ctx : Fr2dCtx {};
 
 extern "C" {
 
@@ -321,12 +320,22 @@ int fr2d_sensor_filter(fr2d_ctx *c, int mode, double lambda, double S0, double k
   if (mode == FR_SENSOR_LEARNED) {
     if ((rc = fr2d_launch_mlp_sensor(*c, c->d_u, st))) return rc;
   }
-  if ((rc = fr2d_launch_sensor_filter(*c, c->d_u, mode, lambda, S0, kappa, c->d_count, st))) return rc;
+  if ((rc = fr2d_launch_sensor_filter(*c, c->d_u, mode, lambda, S0, kappa, c->d_count, nullptr, st))) return rc;
   if (n_flagged) {
     FR_HIP(hipMemcpyAsync(n_flagged, c->d_count, sizeof(int), hipMemcpyDeviceToHost, st));
     FR_HIP(hipStreamSynchronize(st));
   }
   return FR_OK;
+}
+
+int fr2d_sensor_eval(fr2d_ctx *c, int mode, double S0, double kappa, float *d_out) {
+  FR_REQUIRE(c && c->d_u && d_out, FR_E_INVALID, "fr2d_sensor_eval: null argument or no state");
+  FR_REQUIRE(mode == FR_SENSOR_MODAL || mode == FR_SENSOR_LEARNED, FR_E_INVALID, "fr2d_sensor_eval: unknown mode");
+  FR_REQUIRE(mode == FR_SENSOR_MODAL || c->mlp_loaded, FR_E_INVALID, "fr2d_sensor_eval: no MLP loaded");
+  hipStream_t st = stream_of(*c);
+  int rc = FR_OK;
+  if (mode == FR_SENSOR_LEARNED && (rc = fr2d_launch_mlp_sensor(*c, c->d_u, st))) return rc;
+  return fr2d_launch_sensor_filter(*c, c->d_u, mode, 1.0, S0, kappa, nullptr, d_out, st);
 }
 
 int fr2d_positivity(fr2d_ctx *c) {

@@ -29,7 +29,8 @@ __global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int 
                                                                  const double *__restrict__ V,
                                                                  const double *__restrict__ iV, int mode,
                                                                  double lambda, double S0, double kappa,
-                                                                 const float *__restrict__ strength, int *count) {
+                                                                 const float *__restrict__ strength, int *count,
+                                                                 float *eval_out) {
   constexpr int NP = NSP * NSP;
   const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (e >= ncell) return;
@@ -52,6 +53,10 @@ __global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int 
     lam = shock_switch(log10(su), S0, kappa) ? lambda : 0.0;
   } else {
     lam = lambda * (double)strength[e];
+  }
+  if (eval_out) {  // evaluation only: per-cell strength multiplier, state untouched
+    eval_out[e] = (lambda > 0.0) ? (float)(lam / lambda) : (lam > 0.0 ? 1.0f : 0.0f);
+    return;
   }
   if (!(lam > 0.0)) return;
   if (count) atomicAdd(count, 1);
@@ -267,23 +272,23 @@ int fr2d_launch_copy_ghost(const Fr2dCtx &c, const double *src, double *dst, hip
 
 template <int NSP>
 static int sensor_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int *cnt,
-                      hipStream_t st) {
+                      float *eval_out, hipStream_t st) {
   const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
   const int ncell = (int)ldp;
   DissOps o = fr2d_diss_ops(c);
   hipLaunchKernelGGL(fr2d_sensor_filter_kernel<NSP>, dim3((ncell + 255) / 256), dim3(256), 0, st, d_u, ncell, ldp,
-                     o.V, o.iV, mode, lambda, S0, kappa, (const float *)c.d_strength, cnt);
+                     o.V, o.iV, mode, lambda, S0, kappa, (const float *)c.d_strength, cnt, eval_out);
   FR_LAUNCHED("fr2d_sensor_filter_kernel");
   return FR_OK;
 }
 
 int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
-                              int *d_count, hipStream_t st) {
+                              int *d_count, float *eval_out, hipStream_t st) {
   switch (c.nsp) {
-    case 2: return sensor_nsp<2>(c, d_u, mode, lambda, S0, kappa, d_count, st);
-    case 3: return sensor_nsp<3>(c, d_u, mode, lambda, S0, kappa, d_count, st);
-    case 4: return sensor_nsp<4>(c, d_u, mode, lambda, S0, kappa, d_count, st);
-    case 5: return sensor_nsp<5>(c, d_u, mode, lambda, S0, kappa, d_count, st);
+    case 2: return sensor_nsp<2>(c, d_u, mode, lambda, S0, kappa, d_count, eval_out, st);
+    case 3: return sensor_nsp<3>(c, d_u, mode, lambda, S0, kappa, d_count, eval_out, st);
+    case 4: return sensor_nsp<4>(c, d_u, mode, lambda, S0, kappa, d_count, eval_out, st);
+    case 5: return sensor_nsp<5>(c, d_u, mode, lambda, S0, kappa, d_count, eval_out, st);
     default: return fail(FR_E_UNSUPPORTED, "sensor: degree out of range");
   }
 }

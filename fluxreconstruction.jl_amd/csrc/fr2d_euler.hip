@@ -21,7 +21,13 @@
 
 namespace fr {
 
-constexpr int TX = 16, TY = 4, TE = TX * TY;
+// tile shape (elements along x and y); TX * TY = 64 so that one lane holds one element
+#ifndef FR2D_TX
+#define FR2D_TX 16
+#endif
+constexpr int TX = FR2D_TX, TY = 64 / FR2D_TX, TE = TX * TY;
+static_assert(TE == 64 && (TX & (TX - 1)) == 0, "tile must hold 64 elements, TX a power of two");
+static_assert(2 * (TX + TY) <= 64, "halo line tasks must fit in one pass of the workgroup");
 
 template <int NSP>
 struct Ops2D {
@@ -56,7 +62,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
 
   const int tid = threadIdx.x;
   const int line = tid >> 6, lane = tid & 63;
-  const int ex = lane & (TX - 1), ey = lane >> 4;
+  const int ex = lane % TX, ey = lane / TX;
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % geo.ntx, by = tile / geo.ntx;
   const int i0 = 1 + bx * TX, j0 = 1 + by * TY;
@@ -88,6 +94,22 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int k = 0; k < NSP; ++k) ur[k][m] = (m == 0 || m == 3) ? 1.0 : 0.0;
+  }
+
+  // stage epilogue operand a0*u0 + a1*u of the own row, formed while the row is in registers
+  double pre[NSP][4];
+  if constexpr (EPI == EPI_STAGE) {
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) {
+          const long long idx = e + ldp * (k + NSP * (line + NSP * m));
+          double v = sa.a1 * ur[k][m];
+          if (sa.a0 != 0.0) v = sa.a0 * sa.u0[idx] + v;
+          pre[k][m] = v;
+        }
+    }
   }
 
   // halo task: kind 0 bottom (column c, uT), 1 top (column c, uB), 2 left (row l, uR), 3 right (row l, uL)
@@ -185,7 +207,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
   }
 
   // row: LDS transpose copy, x traces, contravariant x flux, its traces and dl derivative
-  double uLx[4], fLx[4], fRx[4], dux[NSP][4];
+  double uLx[4], dux[NSP][4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     double a = 0.0, b = 0.0;
@@ -220,21 +242,20 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
         a += f1[q][m] * ops.ll[q];
         b += f1[q][m] * ops.lr[q];
       }
-      fLx[m] = a;
-      fRx[m] = b;
 #pragma unroll
       for (int k = 0; k < NSP; ++k) {
         double d = 0.0;
 #pragma unroll
         for (int q = 0; q < NSP; ++q) d += f1[q][m] * ops.dl[k + NSP * q];
-        dux[k][m] = d;
+        // derivative minus the discontinuous-flux traces of the correction terms
+        dux[k][m] = d - a * ops.dhl[k] - b * ops.dhr[k];
       }
     }
   }
   __syncthreads();  // B1: sU rows, x traces and left/bottom halo traces are visible
 
   // ---------------------------------------------------------------- P1: column sweep + x Riemann
-  double uBy[4], gB[4], gT[4], duy[NSP][4];
+  double uBy[4], duy[NSP][4];
   {
     double uc[NSP][4];
     if (act) {
@@ -278,14 +299,12 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
         a += f2[q][m] * ops.ll[q];
         b += f2[q][m] * ops.lr[q];
       }
-      gB[m] = a;
-      gT[m] = b;
 #pragma unroll
       for (int k = 0; k < NSP; ++k) {
         double d = 0.0;
 #pragma unroll
         for (int q = 0; q < NSP; ++q) d += f2[q][m] * ops.dl[k + NSP * q];
-        duy[k][m] = d;
+        duy[k][m] = d - a * ops.dhl[k] - b * ops.dhr[k];
       }
     }
   }
@@ -331,12 +350,12 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
   if (act) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const double cb = SY(m, line, ey, ex) * iJ22 - gB[m];
-      const double ct = SY(m, line, ey + 1, ex) * iJ22 - gT[m];
+      const double cb = SY(m, line, ey, ex) * iJ22;
+      const double ct = SY(m, line, ey + 1, ex) * iJ22;
 #pragma unroll
       for (int k = 0; k < NSP; ++k) SU(line, k, m) = duy[k][m] + cb * ops.dhl[k] + ct * ops.dhr[k];
-      const double cl = SX(m, line, ey, ex) * iJ11 - fLx[m];
-      const double cr = SX(m, line, ey, ex + 1) * iJ11 - fRx[m];
+      const double cl = SX(m, line, ey, ex) * iJ11;
+      const double cr = SX(m, line, ey, ex + 1) * iJ11;
 #pragma unroll
       for (int k = 0; k < NSP; ++k) dux[k][m] += cl * ops.dhl[k] + cr * ops.dhr[k];
     }
@@ -355,9 +374,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
         if constexpr (EPI == EPI_RESIDUAL) {
           out[idx] = du;
         } else {
-          double v = sa.a1 * ur[k][m];
-          if (sa.a0 != 0.0) v = sa.a0 * sa.u0[idx] + v;
-          out[idx] = v + sa.b * du;
+          out[idx] = pre[k][m] + sa.b * du;
           if (sa.acc_out) sa.acc_out[idx] = sa.acc_in[idx] + sa.b2 * du;
         }
       }

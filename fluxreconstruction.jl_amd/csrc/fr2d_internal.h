@@ -33,8 +33,10 @@ int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, 
                       double a1, double b, const double *acc_in, double *acc_out, double b2, hipStream_t st);
 int fr2d_launch_fill_ghost(const Fr2dCtx &c, double *d_u, int rule, hipStream_t st);
 int fr2d_launch_copy_ghost(const Fr2dCtx &c, const double *src, double *dst, hipStream_t st);
+// eval_out != nullptr: write the per-cell strength multiplier (modal: 0/1, learned: MLP output)
+// instead of filtering
 int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
-                              int *d_count, hipStream_t st);
+                              int *d_count, float *eval_out, hipStream_t st);
 int fr2d_launch_positivity(const Fr2dCtx &c, double *d_u, int *d_count, hipStream_t st);
 int fr2d_launch_check(const Fr2dCtx &c, const double *d_u, int *d_count, hipStream_t st);
 // learned sensor: per-cell strengths from the MLP into c.d_strength (fr_mlp.hip)
@@ -47,3 +49,6 @@ struct DissOps {
 DissOps fr2d_diss_ops(const Fr2dCtx &c);
 
 }  // namespace fr
+
+// opaque handle of the C ABI (include/fr.h)
+struct fr2d_ctx : fr::Fr2dCtx {};

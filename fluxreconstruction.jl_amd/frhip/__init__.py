@@ -31,6 +31,9 @@ from .space import (  # noqa: F401
     Linf_error,
 )
 from .euler2d import FREuler2D  # noqa: F401
+from .fr1d import FR1D, FREulerProblem, FRAdvectionProblem, Ensemble1D  # noqa: F401
 from . import ics  # noqa: F401
+from .bgk import BGK2D  # noqa: F401
+from . import sensor  # noqa: F401
 
 __version__ = "0.1.0"

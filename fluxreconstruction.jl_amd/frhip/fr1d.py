@@ -1,0 +1,175 @@
+"""1D FR problems (FREulerProblem / FRAdvectionProblem of src/Equation/eq_euler.jl:1-27 and
+src/Equation/eq_advection.jl:1-53) and batched ensembles of independent 1D Euler problems,
+on libfrhip's gfx950 kernels (include/fr.h, "1D" and "batched 1D Euler ensembles")."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, dptr
+from .space import FRPSpace1D
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+_lib.SIGNATURES.update({
+    "fr1d_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_lib.Fr1dDesc)]),
+    "fr1d_destroy": (C.c_int, [_vp]),
+    "fr1d_state_len": (C.c_size_t, [_vp]),
+    "fr1d_rhs": (C.c_int, [_vp, _dp, _dp, C.c_double]),
+    "fr1d_rhs_dev": (C.c_int, [_vp, _vp, _vp, C.c_double]),
+    "fr1d_set_state": (C.c_int, [_vp, _dp]),
+    "fr1d_get_state": (C.c_int, [_vp, _dp]),
+    "fr1d_step": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int]),
+    "fr_ens_create": (C.c_int, [C.POINTER(_vp), C.c_int, _ip, _ip, _dp, C.c_double, C.c_int, C.c_int]),
+    "fr_ens_destroy": (C.c_int, [_vp]),
+    "fr_ens_state_len": (C.c_size_t, [_vp]),
+    "fr_ens_offset": (C.c_size_t, [_vp, C.c_int]),
+    "fr_ens_set_state": (C.c_int, [_vp, _dp]),
+    "fr_ens_get_state": (C.c_int, [_vp, _dp]),
+    "fr_ens_step": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int]),
+    "fr_ens_step_graph": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
+    "fr_ens_autotune": (C.c_int, [_vp, C.c_double, _ip]),
+})
+
+
+def _bind():
+    L = _lib.lib()
+    for name in ("fr1d_create", "fr1d_destroy", "fr1d_state_len", "fr1d_rhs", "fr1d_rhs_dev", "fr1d_set_state",
+                 "fr1d_get_state", "fr1d_step", "fr_ens_create", "fr_ens_destroy", "fr_ens_state_len", "fr_ens_offset",
+                 "fr_ens_set_state", "fr_ens_get_state", "fr_ens_step", "fr_ens_step_graph", "fr_ens_autotune"):
+        res, args = _lib.SIGNATURES[name]
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+class FR1D:
+    """One 1D FR problem.  equation "euler" (u(i, p, k), 3 variables, frode_euler!) or
+    "advection" (u(i, p), frode_advection!); bc "dirichlet" or "period"."""
+
+    def __init__(self, ps: FRPSpace1D, equation: str = "euler", gamma: float = 1.4, a: float = 1.0,
+                 flux: str = "hll", bc: str = "dirichlet"):
+        L = _bind()
+        self.ps = ps
+        self.equation = equation
+        n = ps.deg + 1
+        self.shape = (ps.nx, n, 3) if equation == "euler" else (ps.nx, n)
+        d = _lib.Fr1dDesc()
+        d.equation = _lib.EQ[equation]
+        d.ncell = ps.nx
+        d.deg = ps.deg
+        d.dx = float(ps.base.dx[0])
+        d.gamma = float(gamma)
+        d.a = float(a)
+        d.flux = _lib.FLUX[flux]
+        d.bc = _lib.BC1[bc]
+        d.correction = _lib.CORR[getattr(ps, "correction", "radau")]
+        h = C.c_void_p()
+        check(L.fr1d_create(C.byref(h), C.byref(d)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            _lib.lib().fr1d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def dudt(self, du, u, p=None, t: float = 0.0):
+        check(_lib.lib().fr1d_rhs(self.h, dptr(u), dptr(du), float(t)))
+
+    def rhs(self, u):
+        u = np.asfortranarray(u, dtype=np.float64)
+        du = np.zeros(self.shape, order="F")
+        self.dudt(du, u)
+        return du
+
+    def set_state(self, u):
+        check(_lib.lib().fr1d_set_state(self.h, dptr(np.asfortranarray(u, dtype=np.float64))))
+
+    def get_state(self):
+        out = np.zeros(self.shape, order="F")
+        check(_lib.lib().fr1d_get_state(self.h, dptr(out)))
+        return out
+
+    def step(self, dt: float, nsteps: int = 1, scheme: str = "ssprk3"):
+        check(_lib.lib().fr1d_step(self.h, _lib.RK[scheme], float(dt), int(nsteps)))
+
+
+def FREulerProblem(ps: FRPSpace1D, gamma: float, bc: str = "dirichlet", flux: str = "hll") -> FR1D:
+    """src/Equation/eq_euler.jl:1-27 (bc :dirichlet or :period)."""
+    return FR1D(ps, "euler", gamma=gamma, flux=flux, bc=bc)
+
+
+def FRAdvectionProblem(ps: FRPSpace1D, a: float, bc: str = "period") -> FR1D:
+    """src/Equation/eq_advection.jl:1-26."""
+    return FR1D(ps, "advection", a=a, bc=bc)
+
+
+class Ensemble1D:
+    """Independent 1D Euler problems advanced together (one launch per SSP-RK3 stage).
+    Member b: ncell[b] cells of degree deg[b] on a uniform grid of spacing dx[b]."""
+
+    def __init__(self, ncell, deg, dx, gamma: float = 1.4, flux: str = "rusanov", bc: str = "dirichlet"):
+        L = _bind()
+        self.ncell = np.ascontiguousarray(ncell, dtype=np.int32)
+        self.deg = np.ascontiguousarray(deg, dtype=np.int32)
+        self.dx = np.ascontiguousarray(dx, dtype=np.float64)
+        h = C.c_void_p()
+        check(L.fr_ens_create(C.byref(h), int(self.ncell.size), self.ncell.ctypes.data_as(_ip),
+                              self.deg.ctypes.data_as(_ip), dptr(self.dx), float(gamma), _lib.FLUX[flux],
+                              _lib.BC1[bc]))
+        self.h = h
+        self.len = L.fr_ens_state_len(h)
+        self.offsets = np.array([L.fr_ens_offset(h, b) for b in range(self.ncell.size)], dtype=np.int64)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            _lib.lib().fr_ens_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def member(self, flat, b):
+        n, p = int(self.ncell[b]), int(self.deg[b]) + 1
+        o = int(self.offsets[b])
+        return flat[o:o + n * p * 3].reshape((n, p, 3), order="F")
+
+    def pack(self, members):
+        flat = np.zeros(self.len)
+        for b, u in enumerate(members):
+            o = int(self.offsets[b])
+            flat[o:o + u.size] = np.asarray(u, dtype=np.float64).ravel(order="F")
+        return flat
+
+    def set_state(self, flat):
+        check(_lib.lib().fr_ens_set_state(self.h, dptr(np.ascontiguousarray(flat, dtype=np.float64))))
+
+    def get_state(self):
+        out = np.zeros(self.len)
+        check(_lib.lib().fr_ens_get_state(self.h, dptr(out)))
+        return out
+
+    def step(self, dt: float, nsteps: int = 1, looped: bool = False):
+        check(_lib.lib().fr_ens_step(self.h, 1 if looped else 0, float(dt), int(nsteps)))
+
+    def step_graph(self, dt: float, nsteps: int, nrep: int = 1):
+        check(_lib.lib().fr_ens_step_graph(self.h, float(dt), int(nsteps), int(nrep)))
+
+    def autotune(self, dt: float) -> int:
+        c = C.c_int(-1)
+        check(_lib.lib().fr_ens_autotune(self.h, float(dt), C.byref(c)))
+        return c.value

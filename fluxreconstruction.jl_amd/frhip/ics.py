@@ -143,3 +143,39 @@ def cfl_steps(u, gamma, dx, cfl, t_end):
     lam = max_wave_speed(u, gamma)
     n = int(np.ceil(t_end * lam / (cfl * dx)))
     return t_end / n, n
+
+
+def prim_conserve1(prim, gamma):
+    """KitBase prim_conserve for 1D states, prim = [rho, U, lambda]."""
+    prim = np.asarray(prim, dtype=np.float64)
+    rho, U, lam = prim[..., 0], prim[..., 1], prim[..., 2]
+    w = np.empty_like(prim)
+    w[..., 0] = rho
+    w[..., 1] = rho * U
+    w[..., 2] = 0.5 * rho / lam / (gamma - 1.0) + 0.5 * rho * U * U
+    return w
+
+
+def riemann_ensemble(nprob=512, seed=2024, gamma=1.4, ncell_range=(64, 2048), degs=(2, 3, 4)):
+    """Seeded ensemble of 1D Riemann problems on [0, 1]: member b has ncell[b] uniform cells of
+    degree deg[b]; piecewise-constant states rho, p in [0.1, 1], u in [-0.5, 0.5] left / right of
+    an interface x0 in [0.3, 0.7].  Returns (ncell, deg, dx, [u_b]) with u_b(i, p, k)."""
+    rng = np.random.default_rng(seed)
+    ncell = rng.integers(ncell_range[0], ncell_range[1] + 1, size=nprob).astype(np.int32)
+    deg = rng.choice(np.asarray(degs), size=nprob).astype(np.int32)
+    states = rng.uniform(size=(nprob, 2, 3))
+    x0 = rng.uniform(0.3, 0.7, size=nprob)
+    dx = 1.0 / ncell.astype(np.float64)
+    members = []
+    for b in range(nprob):
+        n, d = int(ncell[b]), int(deg[b])
+        r, _ = np.polynomial.legendre.leggauss(d + 1)
+        xc = (np.arange(n) + 0.5) * dx[b]
+        x = xc[:, None] + 0.5 * dx[b] * r[None, :]
+        side = (x > x0[b]).astype(int)
+        rho = 0.1 + 0.9 * states[b, side, 0]
+        vel = -0.5 + states[b, side, 1]
+        pr = 0.1 + 0.9 * states[b, side, 2]
+        prim = np.stack([rho, vel, rho / (2.0 * pr)], axis=-1)
+        members.append(np.asfortranarray(prim_conserve1(prim, gamma)))
+    return ncell, deg, dx, members

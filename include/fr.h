@@ -134,12 +134,40 @@ int fr2d_fill_ghost(fr2d_ctx *ctx, int rule);
  * If n_flagged is non-NULL it receives the number of filtered cells (synchronises). */
 int fr2d_sensor_filter(fr2d_ctx *ctx, int mode, double lambda, double S0, double kappa,
                        int *n_flagged);
+/* Per-cell strength multiplier of the sensor without filtering (d_out: float per cell of the
+ * (nx+2)(ny+2) layout, device memory): modal 1 / 0 (shock_detector), learned max(MLP, 0). */
+int fr2d_sensor_eval(fr2d_ctx *ctx, int mode, double S0, double kappa, float *d_out);
 /* Positivity limiter of src/dissipation.jl:125-206 on every interior cell of the device state,
  * weights wp/4 (shock-vortex.jl:298-303).  Returns FR_E_LIMITER if an element fails the
  * reference's t1 assertion (that element is left unchanged). */
 int fr2d_positivity(fr2d_ctx *ctx);
 /* Scan the device state for NaN/Inf (FR_E_NONFINITE) or rho <= 0 / p <= 0 (FR_E_NEGATIVE_STATE). */
 int fr2d_check_state(fr2d_ctx *ctx);
+
+/* ------------------------------------------------------------------ learned troubled-cell sensor
+ * No reference counterpart (BASELINE's learned part): MLP 16 -> 32 -> 32 -> 1 (ReLU), BF16 weights
+ * and inputs on MFMA with FP32 accumulation, FP32 master parameters and Adam.  Parameter vector
+ * (fr_mlp_nparam() floats): W1[32][16] b1[32] W2[32][32] b2[32] W3[32] b3.  Inputs: per cell the
+ * 16 density modal coefficients iV*rho normalised to unit 2-norm, as BF16 (fr2d_sensor_features). */
+typedef struct fr_mlp fr_mlp;
+int fr_mlp_nparam(void);
+int fr_mlp_create(fr_mlp **out);
+int fr_mlp_destroy(fr_mlp *m);
+int fr_mlp_set_params(fr_mlp *m, const float *params);  /* host; resets the Adam state */
+int fr_mlp_get_params(fr_mlp *m, float *params);        /* host */
+/* s = max(MLP(x), 0) for n rows of d_x (BF16 bits, 16 per row); device pointers */
+int fr_mlp_forward(fr_mlp *m, const uint16_t *d_x, float *d_out, long long n);
+/* gradients of L = mean (MLP(x) - y)^2 over n rows into host grads[nparam]; *loss = L */
+int fr_mlp_grad(fr_mlp *m, const uint16_t *d_x, const float *d_y, long long n, float *grads, float *loss);
+/* nsteps Adam steps on consecutive batches of `batch` rows; *loss = mean of the last batch losses */
+int fr_mlp_train(fr_mlp *m, const uint16_t *d_x, const float *d_y, long long n, int batch, int nsteps, float lr,
+                 float *loss);
+/* checks the MFMA operand/result lane layout the kernels assume; *max_err must be 0 */
+int fr_mfma_selftest(float *max_err);
+/* use m as the learned sensor of ctx (NULL detaches; the context keeps a pointer, m must outlive it) */
+int fr2d_set_sensor_mlp(fr2d_ctx *ctx, fr_mlp *m);
+/* features of every cell of the device state into d_x ((nx+2)(ny+2) rows of 16 BF16) */
+int fr2d_sensor_features(fr2d_ctx *ctx, uint16_t *d_x);
 
 /* ------------------------------------------------------------------ 1D (FREulerProblem / FRAdvectionProblem) */
 typedef struct fr1d_ctx fr1d_ctx;
@@ -183,6 +211,36 @@ int fr_ens_step(fr_ens_ctx *ctx, int mode, double dt, int nsteps);
 int fr_ens_step_graph(fr_ens_ctx *ctx, double dt, int nsteps, int nrep);
 /* time candidate work mappings on the device and keep the fastest; returns its index */
 int fr_ens_autotune(fr_ens_ctx *ctx, double dt, int *chosen);
+
+/* ------------------------------------------------------------------ 2D discrete-velocity BGK
+ * example/bgk_wave.jl:69-129 tensorised to periodic uniform quads (Radau FR, upwind interface flux
+ * per velocity, relaxation (M - f)/tau to the 2D Maxwellian of the local moments).  State
+ * f(i, j, k, l, a, b) column-major per member, members concatenated (nmem > 1: a batched ensemble
+ * whose members differ in tau). */
+typedef struct fr_bgk_ctx fr_bgk_ctx;
+typedef struct {
+  int nx, ny;        /* periodic cells, no ghost layer */
+  int deg;           /* 1..4 */
+  int nu, nv;        /* velocities per direction */
+  int nmem;          /* ensemble members */
+  double dx, dy;     /* cell sizes (J = dx/2, dy/2) */
+  double gamma;      /* of KitBase conserve_prim for the moments (2 for a 2D monatomic gas) */
+  const double *u, *v, *wu, *wv;  /* abscissae and quadrature weights per velocity direction */
+  const double *tau;              /* nmem relaxation times */
+} fr_bgk_desc;
+int fr_bgk_create(fr_bgk_ctx **out, const fr_bgk_desc *desc);
+int fr_bgk_destroy(fr_bgk_ctx *ctx);
+size_t fr_bgk_state_len(const fr_bgk_ctx *ctx);
+/* mol!(du, u, p, t) for every member (host pointers) */
+int fr_bgk_rhs(fr_bgk_ctx *ctx, const double *f, double *df);
+int fr_bgk_set_state(fr_bgk_ctx *ctx, const double *f);
+int fr_bgk_get_state(fr_bgk_ctx *ctx, double *f);
+/* nsteps SSP-RK3 steps of all members, one moments + one transport launch per stage */
+int fr_bgk_step(fr_bgk_ctx *ctx, double dt, int nsteps);
+/* conserved moments of the device state, w[member][c][point] (host) */
+int fr_bgk_moments(fr_bgk_ctx *ctx, double *w);
+/* Gauss-Hermite nodes / weights for the weight exp(-x^2) (host, native) */
+int fr_gauss_hermite(int n, double *x, double *w);
 
 #ifdef __cplusplus
 }

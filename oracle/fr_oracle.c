@@ -791,6 +791,41 @@ int or_fr2d_sensor_filter(const or_fr2d *d, const double *V, const double *iV, d
   return count;
 }
 
+/* The same L2 modal filter with a per-cell strength lam[i + (nx+2) j] (cells with lam <= 0 are
+ * left unchanged): the filter step of the learned-sensor path. */
+int or_fr2d_filter_cells(const or_fr2d *d, const double *V, const double *iV, double *u, const double *lam) {
+  const int nx = d->nx, ny = d->ny, nsp = d->nsp;
+  const size_t NX2 = nx + 2, NY2 = ny + 2, NSP = nsp;
+  const int np = nsp * nsp;
+  int count = 0;
+  for (int i = 0; i <= nx + 1; ++i)
+    for (int j = 0; j <= ny + 1; ++j) {
+      const double l = lam[i + NX2 * j];
+      if (!(l > 0.0)) continue;
+      ++count;
+      double nod[64], mod[64];
+      for (int m = 0; m < 4; ++m) {
+        for (int ll = 0; ll < nsp; ++ll)
+          for (int k = 0; k < nsp; ++k) nod[k + nsp * ll] = U5(u, i, j, k, ll, m);
+        for (int a = 0; a < np; ++a) {
+          double s = 0.0;
+          for (int b = 0; b < np; ++b) s += iV[a + np * b] * nod[b];
+          mod[a] = s;
+        }
+        or_modal_filter_l2(mod, np, l);
+        for (int a = 0; a < np; ++a) {
+          double s = 0.0;
+          for (int b = 0; b < np; ++b) s += V[a + np * b] * mod[b];
+          nod[a] = s;
+        }
+        for (int ll = 0; ll < nsp; ++ll)
+          for (int k = 0; k < nsp; ++k) U5(u, i, j, k, ll, m) = nod[k + nsp * ll];
+      }
+    }
+  (void)NY2;
+  return count;
+}
+
 /* tj_equation (src/dissipation.jl:208-215): p(t) - eps of the state t (u - ubar) + ubar.
  * The reference solves it by NewtonRaphson from t = 1 with tol 1e-9 (dissipation.jl:184,193);
  * here the root is taken in closed form: rho(t), m(t), E(t) are affine in t, so
@@ -1111,6 +1146,98 @@ void or_fr1d_advection_rhs(int ncell, int nsp, const double *J, const double *ll
   free(ff);
   free(fi);
   free(rhs1);
+}
+
+/* =====================================================================
+ * 2D discrete-velocity BGK -- example/bgk_wave.jl:69-129 tensorised to quads: moments and
+ * Maxwellian per solution point (:77-81, KitBase moments_conserve / conserve_prim / maxwellian in
+ * 2D), f_flux = v f / J per direction (:85-88), face interpolation (:99-101), upwind interface
+ * flux through the Heaviside blend (:103-107), derivative (:114-116) and correction plus the
+ * relaxation (M - f) / tau (:122-128).  Periodic in both directions.
+ * ===================================================================== */
+size_t or_bgk2d_len(const or_bgk2d *d) {
+  return (size_t)d->nx * d->ny * d->nsp * d->nsp * d->nu * d->nv;
+}
+
+void or_bgk2d_moments(const or_bgk2d *d, const double *f, double *w) {
+  const size_t npts = (size_t)d->nx * d->ny * d->nsp * d->nsp;
+#pragma omp parallel for schedule(static)
+  for (size_t p = 0; p < npts; ++p) {
+    double m0 = 0, m1 = 0, m2 = 0, m3 = 0;
+    for (int b = 0; b < d->nv; ++b)
+      for (int a = 0; a < d->nu; ++a) {
+        const double fv = f[p + npts * ((size_t)a + (size_t)d->nu * b)];
+        const double wt = d->wu[a] * d->wv[b];
+        m0 += wt * fv;
+        m1 += wt * d->uv[a] * fv;
+        m2 += wt * d->vv[b] * fv;
+        m3 += wt * 0.5 * (d->uv[a] * d->uv[a] + d->vv[b] * d->vv[b]) * fv;
+      }
+    w[p] = m0;
+    w[p + npts] = m1;
+    w[p + 2 * npts] = m2;
+    w[p + 3 * npts] = m3;
+  }
+}
+
+void or_bgk2d_rhs(const or_bgk2d *d, const double *f, double *df) {
+  const int nx = d->nx, ny = d->ny, n = d->nsp;
+  const size_t npts = (size_t)nx * ny * n * n;
+  const size_t ncell = (size_t)nx * ny;
+  double *w = (double *)malloc(sizeof(double) * npts * 4);
+  or_bgk2d_moments(d, f, w);
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int b = 0; b < d->nv; ++b)
+    for (int a = 0; a < d->nu; ++a) {
+      const double ua = d->uv[a], vb = d->vv[b];
+      const double *fv = f + npts * ((size_t)a + (size_t)d->nu * b);
+      double *dv = df + npts * ((size_t)a + (size_t)d->nu * b);
+#define FV(arr, i, j, k, l) (arr)[(size_t)(i) + (size_t)nx * ((size_t)(j) + (size_t)ny * ((size_t)(k) + (size_t)n * (size_t)(l)))]
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i) {
+          const int il = (i + nx - 1) % nx, ir = (i + 1) % nx, jb = (j + ny - 1) % ny, jt = (j + 1) % ny;
+          for (int l = 0; l < n; ++l)
+            for (int k = 0; k < n; ++k) {
+              /* x direction, row l: flux traces of this cell and of its neighbours */
+              double dfx = 0, fL = 0, fR = 0, nbR = 0, nbL = 0;
+              for (int q = 0; q < n; ++q) {
+                const double g = ua * FV(fv, i, j, q, l) * d->iJx;
+                dfx += g * d->dl[k + n * q];
+                fL += g * d->ll[q];
+                fR += g * d->lr[q];
+                nbR += ua * FV(fv, il, j, q, l) * d->iJx * d->lr[q]; /* right trace of the left cell */
+                nbL += ua * FV(fv, ir, j, q, l) * d->iJx * d->ll[q]; /* left trace of the right cell */
+              }
+              const double dx_pos = ua > 0.0 ? 1.0 : 0.0; /* heaviside */
+              const double FsL = fL * (1.0 - dx_pos) + nbR * dx_pos;
+              const double FsR = nbL * (1.0 - dx_pos) + fR * dx_pos;
+              double dfy = 0, gB = 0, gT = 0, nbT = 0, nbB = 0;
+              for (int q = 0; q < n; ++q) {
+                const double g = vb * FV(fv, i, j, k, q) * d->iJy;
+                dfy += g * d->dl[l + n * q];
+                gB += g * d->ll[q];
+                gT += g * d->lr[q];
+                nbT += vb * FV(fv, i, jb, k, q) * d->iJy * d->lr[q];
+                nbB += vb * FV(fv, i, jt, k, q) * d->iJy * d->ll[q];
+              }
+              const double dy_pos = vb > 0.0 ? 1.0 : 0.0;
+              const double GsB = gB * (1.0 - dy_pos) + nbT * dy_pos;
+              const double GsT = nbB * (1.0 - dy_pos) + gT * dy_pos;
+              const size_t pt = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * ((size_t)k + (size_t)n * (size_t)l));
+              double wp[4] = {w[pt], w[pt + npts], w[pt + 2 * npts], w[pt + 3 * npts]}, prim[4];
+              or_conserve_prim2(wp, d->gamma, prim);
+              const double du = ua - prim[1], dvv = vb - prim[2];
+              const double M = prim[0] * (prim[3] / PI) * exp(-prim[3] * (du * du + dvv * dvv));
+              const double fc = FV(fv, i, j, k, l);
+              FV(dv, i, j, k, l) = -(dfx + (FsL - fL) * d->dhl[k] + (FsR - fR) * d->dhr[k] + dfy +
+                                     (GsB - gB) * d->dhl[l] + (GsT - gT) * d->dhr[l]) +
+                                   (M - fc) / d->tau;
+            }
+        }
+#undef FV
+    }
+  (void)ncell;
+  free(w);
 }
 
 /* =====================================================================

@@ -88,6 +88,7 @@ void or_filter_exp1d(int N, int s, int Nc, double *diag);
    returns the number of filtered cells, writes the per-cell flag if flags != NULL */
 int or_fr2d_sensor_filter(const or_fr2d *d, const double *V, const double *iV, double *u,
                           double lambda, double S0, double kappa, int *flags);
+int or_fr2d_filter_cells(const or_fr2d *d, const double *V, const double *iV, double *u, const double *lam);
 /* 2D tensor positivity limiter on one element u(k,l,m) (dissipation.jl:125-206).
    weights(k,l) as passed by the caller (the reference passes wp/4).  Returns 0, or -1 if the
    reference's @assert on t1 fails, with u untouched in that case. */
@@ -105,6 +106,20 @@ void or_fr1d_euler_rhs(int ncell, int nsp, const double *J, const double *ll, co
 void or_fr1d_advection_rhs(int ncell, int nsp, const double *J, const double *ll, const double *lr,
                            const double *dl, const double *dgl, const double *dgr, double a, int bc,
                            const double *u, double *du);
+
+/* ---------------- 2D discrete-velocity BGK (example/bgk_wave.jl:69-129, tensorised) -----------
+ * f(i, j, k, l, a, b): periodic nx x ny cells (no ghost layer), nsp x nsp points, nu x nv
+ * velocities; column-major, f[i + nx*(j + ny*(k + nsp*(l + nsp*(a + nu*b))))]. */
+typedef struct {
+  int nx, ny, nsp, nu, nv;
+  double iJx, iJy, tau, gamma;            /* 2/dx, 2/dy; relaxation time; gamma of conserve_prim */
+  const double *uv, *vv, *wu, *wv;        /* abscissae and quadrature weights per direction */
+  const double *ll, *lr, *dl, *dhl, *dhr;
+} or_bgk2d;
+size_t or_bgk2d_len(const or_bgk2d *d);
+/* moments w = (rho, rho U, rho V, E) at every solution point: w[pt + npts*c] */
+void or_bgk2d_moments(const or_bgk2d *d, const double *f, double *w);
+void or_bgk2d_rhs(const or_bgk2d *d, const double *f, double *df);
 
 /* ---------------- metrics (src/tools.jl:43-53) ---------------- */
 double or_L1_error(const double *u, const double *ue, size_t n, double dx);

@@ -279,6 +279,13 @@ class Fr2d:
                                          flags.ctypes.data_as(C.POINTER(C.c_int)))
         return n, flags.reshape((self.nx + 2, self.ny + 2), order="F")
 
+    def filter_cells(self, u, lam):
+        """L2 modal filter with per-cell strength lam (shape (nx+2, ny+2)); returns #filtered."""
+        V = np.ascontiguousarray(self.ops.V.ravel(order="F"))
+        iV = np.ascontiguousarray(self.ops.iV.ravel(order="F"))
+        lam = np.asfortranarray(lam, dtype=np.float64)
+        return lib().or_fr2d_filter_cells(C.byref(self.d), _p(V), _p(iV), _p(self._u(u)), _p(lam))
+
     def positivity(self, u):
         wp4 = np.ascontiguousarray((self.ops.wp / 4.0).ravel(order="F"))
         return lib().or_fr2d_positivity(C.byref(self.d), _p(wp4), _p(self._u(u)))
@@ -344,3 +351,24 @@ def Linf_error(u, ue, dx):
     a = _f64(u)
     b = _f64(ue)
     return lib().or_Linf_error(_p(a), _p(b), a.size, dx)
+
+
+# ----------------------------------------------------------------- 2D BGK
+class OrBgk2d(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nsp", C.c_int), ("nu", C.c_int), ("nv", C.c_int),
+                ("iJx", C.c_double), ("iJy", C.c_double), ("tau", C.c_double), ("gamma", C.c_double),
+                ("uv", _dp), ("vv", _dp), ("wu", _dp), ("wv", _dp),
+                ("ll", _dp), ("lr", _dp), ("dl", _dp), ("dhl", _dp), ("dhr", _dp)]
+
+
+def bgk2d_rhs(f, nx, ny, deg, u, wu, v, wv, tau, Lx=1.0, Ly=1.0, gamma=2.0):
+    ops = Ops(deg)
+    keep = [_f64(a) for a in (u, v, wu, wv, ops.ll, ops.lr, ops.dl, ops.dhl, ops.dhr)]
+    d = OrBgk2d()
+    d.nx, d.ny, d.nsp, d.nu, d.nv = nx, ny, deg + 1, len(u), len(v)
+    d.iJx, d.iJy, d.tau, d.gamma = 2.0 / (Lx / nx), 2.0 / (Ly / ny), tau, gamma
+    d.uv, d.vv, d.wu, d.wv, d.ll, d.lr, d.dl, d.dhl, d.dhr = [_p(a) for a in keep]
+    f = _f64(f)
+    df = np.zeros_like(f)
+    lib().or_bgk2d_rhs(C.byref(d), _p(f), _p(df))
+    return df

@@ -172,3 +172,33 @@ def test_frpspace2d_geometry():
     assert np.isclose(ps.xpg[i, j, k, l, 0], ps.x[i, j] + ps.xpl[k] * 0.25)
     assert np.isclose(ps.xpg[i, j, k, l, 1], ps.y[i, j] + ps.xpl[l] * 0.25)
     assert ps.dx[0, 0] == 0.5 and ps.nx == 4
+
+
+def test_gauss_hermite_vs_numpy():
+    from frhip import bgk
+
+    for n in (4, 8, 24):
+        x, w = bgk.gauss_hermite(n)
+        xn, wn = np.polynomial.hermite.hermgauss(n)
+        assert np.allclose(x, xn, atol=1e-13) and np.allclose(w, wn, rtol=1e-12, atol=1e-300)
+    u, wu = bgk.velocity_grid(24)
+    # moments of a Maxwellian with lambda = 1: exact for the Gauss-Hermite grid
+    M = np.exp(-u ** 2) / np.sqrt(np.pi)
+    assert abs(np.sum(wu * M) - 1.0) < 1e-13 and abs(np.sum(wu * u * u * M) - 0.5) < 1e-13
+
+
+def test_bgk_oracle_equilibrium_and_conservation():
+    from frhip import bgk
+
+    u, wu = bgk.velocity_grid(8)
+    f0 = bgk.maxwellian_state(4, 3, 2, u, u, amp=0.0)
+    assert np.max(np.abs(O.bgk2d_rhs(f0, 4, 3, 2, u, wu, u, wu, 1e-2))) < 1e-12
+    f = bgk.maxwellian_state(5, 4, 2, u, u, amp=0.1)
+    f *= 1.0 + 0.01 * np.sin(np.arange(f.size))
+    df = O.bgk2d_rhs(f, 5, 4, 2, u, wu, u, wu, 1e-2)
+    # mass: quadrature over points and velocities of df vanishes on the periodic grid
+    wq = np.polynomial.legendre.leggauss(3)[1]
+    wpt = np.einsum("k,l->kl", wq, wq).ravel(order="F")
+    d = df.reshape((5 * 4, 9, 8, 8), order="F")
+    tot = np.einsum("cpab,p,a,b->", d, wpt, wu, wu)
+    assert abs(tot) < 1e-10 * np.abs(d).sum()

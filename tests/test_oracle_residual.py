@@ -141,6 +141,42 @@ def test_advection_upwind_equals_nodal_dg():
     assert np.max(np.abs(du - ref)) < 1e-5 * np.max(np.abs(ref))
 
 
+def test_c1_conditioning():
+    """C1 (advection, 100 cells, p = 2, RK4, dt = 0.1 dx, t = 1): a relative perturbation of
+    1e-15 in u0 changes the oracle's own final state by about 1e-7, because the reference's
+    regularised flux divides face jumps of 1e-7 .. 5e-6 by (jump + 1e-8).  This bounds what any
+    two correct FP64 implementations can agree to on this run."""
+    import frhip
+
+    ncell, deg = 100, 2
+    ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
+    ops = O.Ops(deg)
+    u0 = np.asfortranarray(np.sin(2 * np.pi * ps.xpg))
+    dt = 0.1 / ncell
+
+    def run(pert):
+        L = lambda v: O.fr1d_advection_rhs(v, ps.J, ops, 1.0, "period")  # noqa: E731
+        uc = u0 * (1 + pert)
+        for _ in range(1000):
+            k1 = L(uc)
+            y = uc + (0.5 * dt) * k1
+            acc = uc + (dt / 6.0) * k1
+            k2 = L(y)
+            y = uc + (0.5 * dt) * k2
+            acc = acc + (dt / 3.0) * k2
+            k3 = L(y)
+            y = uc + dt * k3
+            acc = acc + (dt / 3.0) * k3
+            uc = acc + (dt / 6.0) * L(y)
+        return uc
+
+    a, b = run(0.0), run(1e-15)
+    d = np.linalg.norm(a - b) / np.linalg.norm(a)
+    assert 1e-9 < d < 1e-6
+    exact = np.sin(2 * np.pi * (ps.xpg - 1.0))
+    assert np.sum(np.abs(a - exact)) / (ncell * (deg + 1)) < 1e-4
+
+
 def test_euler1d_free_stream_and_conservation():
     deg, ncell = 2, 8
     ops = O.Ops(deg)
