@@ -283,7 +283,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
     double f2[NSP][4];
 #pragma unroll
     for (int q = 0; q < NSP; ++q) {
-      const double r = 1.0 / uc[q][0];
+      const double r = fr_rcp(uc[q][0]);
       const double uu = uc[q][1] * r, vv = uc[q][2] * r;
       const double p = gm1 * (uc[q][3] - 0.5 * (uc[q][1] * uu + uc[q][2] * vv));
       f2[q][0] = iJ22 * uc[q][2];

@@ -368,6 +368,13 @@ int fr_bgk_set_state(fr_bgk_ctx *c, const double *f) {
   return FR_OK;
 }
 
+int fr_bgk_set_state_dev(fr_bgk_ctx *c, const double *d_f) {
+  FR_REQUIRE(c && d_f, FR_E_INVALID, "fr_bgk_set_state_dev: null argument");
+  FR_HIP(hipMemcpyAsync(c->d_u, d_f, c->len * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
 int fr_bgk_get_state(fr_bgk_ctx *c, double *f) {
   FR_REQUIRE(c && f, FR_E_INVALID, "fr_bgk_get_state: null argument");
   FR_HIP(hipMemcpyAsync(f, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToHost, c->stream));

@@ -59,6 +59,37 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
   return (x < rem) ? x * (per + 1) + loc : rem * (per + 1) + (x - rem) * per + loc;
 }
 
+// ------------------------------------------------------------------ FP64 reciprocal / square root
+// With FR_FAST_RCP the hardware estimates v_rcp_f64 / v_rsq_f64 are refined by two Newton-Raphson
+// (reciprocal) and one Goldschmidt (square root) iteration in FMA arithmetic, which leaves about
+// one ulp of error instead of the correctly rounded division / sqrt sequences; otherwise the
+// IEEE operations are used.
+__device__ __forceinline__ double fr_rcp(double x) {
+#ifdef FR_FAST_RCP
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+#else
+  return 1.0 / x;
+#endif
+}
+
+__device__ __forceinline__ double fr_sqrt(double x) {
+#ifdef FR_FAST_RCP
+  const double y = __builtin_amdgcn_rsq(x);
+  double s = x * y, h = 0.5 * y;
+  const double r = fma(-s, h, 0.5);
+  s = fma(s, r, s);
+  h = fma(h, r, h);
+  const double d = fma(-s, s, x);
+  return fma(d, h, s);
+#else
+  return sqrt(x);
+#endif
+}
+
 // ------------------------------------------------------------------ Euler physics (device)
 // Conserved 2D state w = [rho, rho u, rho v, E]; p = (g-1)(E - |m|^2 / (2 rho)).
 
@@ -69,7 +100,7 @@ struct XState {
 };
 
 __device__ __forceinline__ void xstate(const double *w, double gm1, XState &s) {
-  s.rinv = 1.0 / w[0];
+  s.rinv = fr_rcp(w[0]);
   s.u = w[1] * s.rinv;
   const double v = w[2] * s.rinv;
   s.p = gm1 * (w[3] - 0.5 * (w[1] * s.u + w[2] * v));
@@ -89,10 +120,10 @@ __device__ __forceinline__ void riemann_x4(const double *wL, const double *wR, d
   xstate(wL, gm1, L);
   xstate(wR, gm1, R);
   if constexpr (FLUX == FR_FLUX_HLL) {
-    const double cL = sqrt(g * L.p * L.rinv), cR = sqrt(g * R.p * R.rinv);
+    const double cL = fr_sqrt(g * L.p * L.rinv), cR = fr_sqrt(g * R.p * R.rinv);
     const double lmin = fmin(L.u - cL, R.u - cR);
     const double lmax = fmax(L.u + cL, R.u + cR);
-    const double factor = 1.0 / (lmax - lmin);
+    const double factor = fr_rcp(lmax - lmin);
     const bool left = lmin >= 0.0, right = lmax <= 0.0;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
@@ -100,7 +131,7 @@ __device__ __forceinline__ void riemann_x4(const double *wL, const double *wR, d
       fw[m] = left ? L.F[m] : (right ? R.F[m] : h);
     }
   } else if constexpr (FLUX == FR_FLUX_RUSANOV) {
-    const double cL = sqrt(g * L.p * L.rinv), cR = sqrt(g * R.p * R.rinv);
+    const double cL = fr_sqrt(g * L.p * L.rinv), cR = fr_sqrt(g * R.p * R.rinv);
     const double smax = fmax(fabs(L.u) + cL, fabs(R.u) + cR);
 #pragma unroll
     for (int m = 0; m < 4; ++m) fw[m] = 0.5 * (L.F[m] + R.F[m]) - 0.5 * smax * (wR[m] - wL[m]);

@@ -25,6 +25,7 @@ _lib.SIGNATURES.update({
     "fr_bgk_state_len": (C.c_size_t, [_vp]),
     "fr_bgk_rhs": (C.c_int, [_vp, _dp, _dp]),
     "fr_bgk_set_state": (C.c_int, [_vp, _dp]),
+    "fr_bgk_set_state_dev": (C.c_int, [_vp, _vp]),
     "fr_bgk_get_state": (C.c_int, [_vp, _dp]),
     "fr_bgk_step": (C.c_int, [_vp, C.c_double, C.c_int]),
     "fr_bgk_moments": (C.c_int, [_vp, _dp]),
@@ -35,6 +36,7 @@ _lib.SIGNATURES.update({
 def _bind():
     L = _lib.lib()
     for name in ("fr_bgk_create", "fr_bgk_destroy", "fr_bgk_state_len", "fr_bgk_rhs", "fr_bgk_set_state",
+                 "fr_bgk_set_state_dev",
                  "fr_bgk_get_state", "fr_bgk_step", "fr_bgk_moments", "fr_gauss_hermite"):
         res, args = _lib.SIGNATURES[name]
         fn = getattr(L, name)
@@ -96,7 +98,13 @@ class BGK2D:
         return df
 
     def set_state(self, f):
-        check(_lib.lib().fr_bgk_set_state(self.h, dptr(np.ascontiguousarray(np.asarray(f, dtype=np.float64).ravel(order="F")))))
+        """Host array (any shape, column-major order) or a flat float64 torch GPU tensor."""
+        if type(f).__module__.startswith("torch"):
+            assert f.is_cuda and str(f.dtype) == "torch.float64" and f.is_contiguous() and f.numel() == self.len
+            check(_lib.lib().fr_bgk_set_state_dev(self.h, C.c_void_p(f.data_ptr())))
+            return
+        flat = np.ascontiguousarray(np.asarray(f, dtype=np.float64).ravel(order="F"))
+        check(_lib.lib().fr_bgk_set_state(self.h, dptr(flat)))
 
     def get_state(self):
         f = np.zeros(self.len)

@@ -234,6 +234,7 @@ size_t fr_bgk_state_len(const fr_bgk_ctx *ctx);
 /* mol!(du, u, p, t) for every member (host pointers) */
 int fr_bgk_rhs(fr_bgk_ctx *ctx, const double *f, double *df);
 int fr_bgk_set_state(fr_bgk_ctx *ctx, const double *f);
+int fr_bgk_set_state_dev(fr_bgk_ctx *ctx, const double *d_f);  /* device pointer, synchronises */
 int fr_bgk_get_state(fr_bgk_ctx *ctx, double *f);
 /* nsteps SSP-RK3 steps of all members, one moments + one transport launch per stage */
 int fr_bgk_step(fr_bgk_ctx *ctx, double dt, int nsteps);
