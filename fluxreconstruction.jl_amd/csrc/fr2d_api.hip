@@ -1,6 +1,8 @@
 // fr2d_api.hip -- C ABI of the 2D Euler solver: context, drop-in residual, device-resident
 // integrator, dissipation calls (declared in include/fr.h).
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 
@@ -107,8 +109,9 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
   FR_REQUIRE(d->bc == FR_BC_GHOST || d->bc == FR_BC_PERIODIC, FR_E_INVALID, "fr2d_create: unknown bc");
   FR_REQUIRE(d->gamma > 1.0 && std::isfinite(d->gamma), FR_E_INVALID, "fr2d_create: gamma must be > 1");
   FR_REQUIRE(d->iJ11 > 0.0 && d->iJ22 > 0.0, FR_E_INVALID, "fr2d_create: iJ11, iJ22 must be > 0");
-  FR_REQUIRE((double)(d->nx + 2) * (d->ny + 2) * (d->deg + 1) * (d->deg + 1) * 4 < 9.0e18, FR_E_INVALID,
-             "fr2d_create: grid too large");
+  // the kernels address one (k, l, m) plane with a 32-bit byte offset per element
+  FR_REQUIRE((double)(d->nx + 2) * (d->ny + 2) * 8.0 < 4294967296.0, FR_E_INVALID,
+             "fr2d_create: grid too large ((nx+2)(ny+2) must be below 2^29)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(FR_E_HIP, "fr2d_create: no HIP device visible");
@@ -138,6 +141,8 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
   if (d->V) std::memcpy(c->ops.V, d->V, sizeof(double) * np * np);
   if (d->iV) std::memcpy(c->ops.iV, d->iV, sizeof(double) * np * np);
   hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, c->device);
+  if (const char *tpw = std::getenv("FR2D_TILES_PER_WG")) c->tiles_per_wg = std::max(0, std::atoi(tpw));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->d_count, 4 * sizeof(int));
   const size_t nd = 2 * (size_t)np * np + np + 2 * n;

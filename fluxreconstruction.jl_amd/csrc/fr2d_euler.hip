@@ -27,7 +27,15 @@ namespace fr {
 #endif
 constexpr int TX = FR2D_TX, TY = 64 / FR2D_TX, TE = TX * TY;
 static_assert(TE == 64 && (TX & (TX - 1)) == 0, "tile must hold 64 elements, TX a power of two");
-static_assert(2 * (TX + TY) <= 64, "halo line tasks must fit in one pass of the workgroup");
+// 1: persistent workgroups loop over tiles and load the next tile's rows during the current one
+#ifndef FR2D_PERSIST
+#define FR2D_PERSIST 0
+#endif
+// global accesses: 0 = flat global addressing, 1 = one scalar buffer resource per plane
+#ifndef FR2D_BUFFER
+#define FR2D_BUFFER 0
+#endif
+static_assert(2 * (TX + TY) <= 64, "the halo tasks of one line must fit in one wavefront");
 
 template <int NSP>
 struct Ops2D {
@@ -55,24 +63,22 @@ struct StageArgs {
 template <int NSP, int FLUX, int EPI>
 __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const Ops2D<NSP> ops,
                                                         const StageArgs sa) {
-  constexpr int NL = NSP * 4;  // values of one line (NSP points x 4 variables)
   __shared__ double sU[NSP * NSP * 4 * TE];
   __shared__ double sX[4 * NSP * TY * (TX + 1)];
   __shared__ double sY[4 * NSP * (TY + 1) * TX];
 
   const int tid = threadIdx.x;
-  const int line = tid >> 6, lane = tid & 63;
+  // the line index is uniform over a wavefront, so every plane offset below is a scalar
+  const int line = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
   const int ex = lane % TX, ey = lane / TX;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int bx = tile % geo.ntx, by = tile / geo.ntx;
-  const int i0 = 1 + bx * TX, j0 = 1 + by * TY;
-  const int ncx = min(TX, geo.nx - i0 + 1), ncy = min(TY, geo.ny - j0 + 1);
-  const bool act = ex < ncx && ey < ncy;
-  const long long nxg = geo.nx + 2;
-  const long long ldp = geo.ldp;
-  const long long e = (long long)(i0 + ex) + nxg * (j0 + ey);
-  const double *__restrict__ u = sa.u;
+  const int ntiles = geo.ntx * geo.nty;
+  const unsigned nxg = geo.nx + 2;
+  long long ldp = geo.ldp;
+  const double *u = sa.u, *u0p = sa.u0, *accin = sa.acc_in;
+  double *outp = sa.out, *accout = sa.acc_out;
   const double g = geo.g, gm1 = geo.gm1, iJ11 = geo.iJ11, iJ22 = geo.iJ22;
+  const bool with_u0 = EPI == EPI_STAGE && sa.a0 != 0.0;
 
   auto SX = [&](int m, int l, int y, int fx) -> double & {
     return sX[((m * NSP + l) * TY + y) * (TX + 1) + fx];
@@ -81,305 +87,378 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
     return sY[((m * NSP + c) * (TY + 1) + fy) * TX + x];
   };
   auto SU = [&](int k, int l, int m) -> double & { return sU[((k * NSP + l) * 4 + m) * TE + lane]; };
+  // value of plane (k, l, m) at the element with byte offset bo inside the plane
+#if FR2D_BUFFER
+  const unsigned pbytes = (unsigned)(ldp * 8);
+  auto LD = [&](const double *base, int k, int l, int m, unsigned bo) -> double {
+    return bload(plane_rsrc(base + ldp * (k + NSP * (l + NSP * m)), pbytes), bo);
+  };
+  auto ST = [&](double *base, int k, int l, int m, unsigned bo, double v) {
+    bstore(plane_rsrc(base + ldp * (k + NSP * (l + NSP * m)), pbytes), bo, v);
+  };
+#else
+  auto LD = [&](const double *base, int k, int l, int m, unsigned bo) -> double {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base + ldp * (k + NSP * (l + NSP * m))) + bo);
+  };
+  auto ST = [&](double *base, int k, int l, int m, unsigned bo, double v) {
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(base + ldp * (k + NSP * (l + NSP * m))) + bo) = v;
+  };
+#endif
+  // geometry of one tile for this thread
+  struct TileGeo {
+    int i0, j0, ncx, ncy;
+    bool act;
+    unsigned ebo;  // byte offset of the own element inside a plane ((nx+2)(ny+2) < 2^29 on the host)
+  };
+  auto tile_geo = [&](int vt) {
+    TileGeo t;
+    const int tile = xcd_tile(vt, ntiles);
+    const int bx = tile % geo.ntx, by = tile / geo.ntx;
+    t.i0 = 1 + bx * TX;
+    t.j0 = 1 + by * TY;
+    t.ncx = min(TX, geo.nx - t.i0 + 1);
+    t.ncy = min(TY, geo.ny - t.j0 + 1);
+    t.act = ex < t.ncx && ey < t.ncy;
+    t.ebo = 8u * ((unsigned)(t.i0 + ex) + nxg * (unsigned)(t.j0 + ey));
+    return t;
+  };
+  // own row `line` of the element (and the u0 operand of the stage epilogue)
+  auto load_row = [&](const TileGeo &t, double (&r)[NSP][4], double (&r0)[NSP][4]) {
+    if (t.act) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) r[k][m] = LD(u, k, line, m, t.ebo);
+      if (with_u0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int k = 0; k < NSP; ++k) r0[k][m] = LD(u0p, k, line, m, t.ebo);
+      }
+    }
+  };
 
-  // ---------------------------------------------------------------- P0: own row + halo lines
-  double ur[NSP][4];
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) ur[k][m] = u[e + ldp * (k + NSP * (line + NSP * m))];
-  } else {
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) ur[k][m] = (m == 0 || m == 3) ? 1.0 : 0.0;
+  if constexpr (EPI == EPI_RESIDUAL) {
+    if (blockIdx.x == 0) {  // du vanishes on the four ghost corners (example/shock-vortex.jl:27)
+      constexpr int per = NSP * NSP * 4;
+      for (int v = tid; v < 4 * per; v += NSP * 64) {
+        const int c = v / per, o = v % per;
+        const long long ci = (c & 1) ? geo.nx + 1 : 0, cj = (c & 2) ? geo.ny + 1 : 0;
+        sa.out[ci + (long long)nxg * cj + ldp * o] = 0.0;
+      }
+    }
   }
 
-  // stage epilogue operand a0*u0 + a1*u of the own row, formed while the row is in registers
-  double pre[NSP][4];
-  if constexpr (EPI == EPI_STAGE) {
+  // Persistent loop over tiles vt = blockIdx.x, blockIdx.x + gridDim.x, ...: the own rows of the
+  // next tile are loaded while the current one is computed.  With gridDim.x a multiple of 8 every
+  // tile of a workgroup maps into the same XCD's contiguous range (xcd_tile).
+  int vt = blockIdx.x;
+  TileGeo T = tile_geo(vt);
+  double ur[NSP][4], u0r[NSP][4];
+  load_row(T, ur, u0r);
+  for (;;) {
+#if FR2D_PERSIST
+    // opaque per iteration: keeps the plane addresses from being hoisted out of the tile loop,
+    // where they would stay live in scalar registers for the whole kernel
+    asm volatile("" : "+s"(ldp), "+s"(u), "+s"(u0p), "+s"(outp), "+s"(accin), "+s"(accout));
+#endif
+    const int i0 = T.i0, j0 = T.j0, ncx = T.ncx, ncy = T.ncy;
+    const bool act = T.act;
+    const unsigned ebo = T.ebo;
+    if (!act) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) ur[k][m] = (m == 0 || m == 3) ? 1.0 : 0.0;
+    }
+
+    // -------------------------------------------------------------- P0: own row + halo lines
+    // Halo tasks of this wavefront's line: lanes [0, TX) bottom (column `line` of the element
+    // below the tile, its top trace), [TX, 2TX) top (bottom trace of the element above),
+    // [2TX, 2TX+TY) left (row `line`, right trace of the left neighbour), [2TX+TY, 2TX+2TY) right.
+    int hkind = -1, hpos = 0;
+    if (lane < TX) {
+      hkind = 0;
+      hpos = lane;
+    } else if (lane < 2 * TX) {
+      hkind = 1;
+      hpos = lane - TX;
+    } else if (lane < 2 * TX + TY) {
+      hkind = 2;
+      hpos = lane - 2 * TX;
+    } else if (lane < 2 * TX + 2 * TY) {
+      hkind = 3;
+      hpos = lane - 2 * TX - TY;
+    }
+    if (hkind >= 0 && hpos >= (hkind <= 1 ? ncx : ncy)) hkind = -1;
+    double htr[4] = {0.0, 0.0, 0.0, 0.0};
+    if (hkind >= 0) {
+      int hi, hj;
+      if (hkind == 0) {
+        hi = i0 + hpos;
+        hj = j0 - 1;
+      } else if (hkind == 1) {
+        hi = i0 + hpos;
+        hj = j0 + ncy;
+      } else if (hkind == 2) {
+        hi = i0 - 1;
+        hj = j0 + hpos;
+      } else {
+        hi = i0 + ncx;
+        hj = j0 + hpos;
+      }
+      int si = hi, sj = hj;
+      if (geo.periodic) {
+        si = (hi == 0) ? geo.nx : (hi == geo.nx + 1 ? 1 : hi);
+        sj = (hj == 0) ? geo.ny : (hj == geo.ny + 1 ? 1 : hj);
+      }
+      const unsigned hbo = 8u * ((unsigned)si + nxg * (unsigned)sj);
+      const bool lo = hkind == 0 || hkind == 2;  // weights lr (trace on the far side) or ll
+      double w[NSP];
+#pragma unroll
+      for (int q = 0; q < NSP; ++q) w[q] = lo ? ops.lr[q] : ops.ll[q];
+      if (hkind <= 1) {  // column `line`: points (k = line, l = q)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < NSP; ++q) s += LD(u, line, q, m, hbo) * w[q];
+          htr[m] = s;
+        }
+      } else {  // row `line`: points (k = q, l = line)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < NSP; ++q) s += LD(u, q, line, m, hbo) * w[q];
+          htr[m] = s;
+        }
+      }
+      if (hkind == 0) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) SY(m, line, 0, hpos) = htr[m];
+      } else if (hkind == 2) {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) SX(m, line, hpos, 0) = htr[m];
+      }
+      if constexpr (EPI == EPI_RESIDUAL) {
+        // du vanishes on the ghost layer (example/shock-vortex.jl:27)
+        const bool ghost = hi == 0 || hi == geo.nx + 1 || hj == 0 || hj == geo.ny + 1;
+        if (ghost) {
+          const unsigned gbo = 8u * ((unsigned)hi + nxg * (unsigned)hj);
+          if (hkind <= 1) {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+              for (int q = 0; q < NSP; ++q) ST(outp, line, q, m, gbo, 0.0);
+          } else {
+#pragma unroll
+            for (int m = 0; m < 4; ++m)
+#pragma unroll
+              for (int q = 0; q < NSP; ++q) ST(outp, q, line, m, gbo, 0.0);
+          }
+        }
+      }
+    }
+
+    // stage epilogue operand a0*u0 + a1*u of the own row
+    double pre[NSP][4];
+    if constexpr (EPI == EPI_STAGE) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) {
+          double v = sa.a1 * ur[k][m];
+          if (with_u0) v = sa.a0 * u0r[k][m] + v;
+          pre[k][m] = v;
+        }
+    }
+
+    // row: LDS transpose copy, x traces, contravariant x flux, its traces and dl derivative
+    double uLx[4], dux[NSP][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double a = 0.0, b = 0.0;
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) {
+        a += ur[k][m] * ops.ll[k];
+        b += ur[k][m] * ops.lr[k];
+      }
+      uLx[m] = a;
+      if (act) SX(m, line, ey, ex + 1) = b;
+    }
+    if (act) {
+#pragma unroll
+      for (int k = 0; k < NSP; ++k)
+#pragma unroll
+        for (int m = 0; m < 4; ++m) SU(k, line, m) = ur[k][m];
+    }
+    {
+      double f1[NSP][4];
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) {
+        XState xs;
+        xstate(ur[k], gm1, xs);
+#pragma unroll
+        for (int m = 0; m < 4; ++m) f1[k][m] = iJ11 * xs.F[m];
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int q = 0; q < NSP; ++q) {
+          a += f1[q][m] * ops.ll[q];
+          b += f1[q][m] * ops.lr[q];
+        }
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) {
+          double d = 0.0;
+#pragma unroll
+          for (int q = 0; q < NSP; ++q) d += f1[q][m] * ops.dl[k + NSP * q];
+          // derivative minus the discontinuous-flux traces of the correction terms
+          dux[k][m] = d - a * ops.dhl[k] - b * ops.dhr[k];
+        }
+      }
+    }
+
+    // the own rows are consumed: start loading the next tile's
+    const int nvt = vt + gridDim.x;
+    const bool more = FR2D_PERSIST && nvt < ntiles;
+    TileGeo NT = T;
+    if (more) {
+      NT = tile_geo(nvt);
+      load_row(NT, ur, u0r);
+    }
+    __syncthreads();  // B1: sU rows, x traces and left/bottom halo traces are visible
+
+    // -------------------------------------------------------------- P1: column sweep + x Riemann
+    double uBy[4], duy[NSP][4];
+    {
+      double uc[NSP][4];
+      if (act) {
+#pragma unroll
+        for (int q = 0; q < NSP; ++q)
+#pragma unroll
+          for (int m = 0; m < 4; ++m) uc[q][m] = SU(line, q, m);
+      } else {
+#pragma unroll
+        for (int q = 0; q < NSP; ++q)
+#pragma unroll
+          for (int m = 0; m < 4; ++m) uc[q][m] = (m == 0 || m == 3) ? 1.0 : 0.0;
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int q = 0; q < NSP; ++q) {
+          a += uc[q][m] * ops.ll[q];
+          b += uc[q][m] * ops.lr[q];
+        }
+        uBy[m] = a;
+        if (act) SY(m, line, ey + 1, ex) = b;
+      }
+      double f2[NSP][4];
+#pragma unroll
+      for (int q = 0; q < NSP; ++q) {
+        const double r = fr_rcp(uc[q][0]);
+        const double uu = uc[q][1] * r, vv = uc[q][2] * r;
+        const double p = gm1 * (uc[q][3] - 0.5 * (uc[q][1] * uu + uc[q][2] * vv));
+        f2[q][0] = iJ22 * uc[q][2];
+        f2[q][1] = iJ22 * (uc[q][1] * vv);
+        f2[q][2] = iJ22 * (uc[q][2] * vv + p);
+        f2[q][3] = iJ22 * ((uc[q][3] + p) * vv);
+      }
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int q = 0; q < NSP; ++q) {
+          a += f2[q][m] * ops.ll[q];
+          b += f2[q][m] * ops.lr[q];
+        }
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) {
+          double d = 0.0;
+#pragma unroll
+          for (int q = 0; q < NSP; ++q) d += f2[q][m] * ops.dl[k + NSP * q];
+          duy[k][m] = d - a * ops.dhl[k] - b * ops.dhr[k];
+        }
+      }
+    }
+    // x-face on the left of the own element: left state = right trace of the neighbour
+    if (act) {
+      double wl[4], fw[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) wl[m] = SX(m, line, ey, ex);
+      riemann_x4<FLUX>(wl, uLx, g, gm1, fw);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) SX(m, line, ey, ex) = fw[m];
+    }
+    if (hkind == 3) {  // right boundary face of the tile row, solved by the right halo task
+      double wl[4], fw[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) wl[m] = SX(m, line, hpos, ncx);
+      riemann_x4<FLUX>(wl, htr, g, gm1, fw);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) SX(m, line, hpos, ncx) = fw[m];
+    }
+    __syncthreads();  // B2: y traces visible
+
+    // -------------------------------------------------------------- P2: y Riemann
+    if (act) {
+      double wb[4], fw[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) wb[m] = SY(m, line, ey, ex);
+      riemann_y4<FLUX>(wb, uBy, g, gm1, fw);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) SY(m, line, ey, ex) = fw[m];
+    }
+    if (hkind == 1) {  // top boundary face of the tile column
+      double wb[4], fw[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) wb[m] = SY(m, line, ncy, hpos);
+      riemann_y4<FLUX>(wb, htr, g, gm1, fw);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) SY(m, line, ncy, hpos) = fw[m];
+    }
+    __syncthreads();  // B3: interface fluxes visible; sU is free
+
+    // -------------------------------------------------------------- P3: corrections
+    if (act) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const double cb = SY(m, line, ey, ex) * iJ22;
+        const double ct = SY(m, line, ey + 1, ex) * iJ22;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) SU(line, k, m) = duy[k][m] + cb * ops.dhl[k] + ct * ops.dhr[k];
+        const double cl = SX(m, line, ey, ex) * iJ11;
+        const double cr = SX(m, line, ey, ex + 1) * iJ11;
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) dux[k][m] += cl * ops.dhl[k] + cr * ops.dhr[k];
+      }
+    }
+    __syncthreads();  // B4: y contributions of every point visible
+
+    // -------------------------------------------------------------- P4: assemble + epilogue
     if (act) {
 #pragma unroll
       for (int m = 0; m < 4; ++m)
 #pragma unroll
         for (int k = 0; k < NSP; ++k) {
-          const long long idx = e + ldp * (k + NSP * (line + NSP * m));
-          double v = sa.a1 * ur[k][m];
-          if (sa.a0 != 0.0) v = sa.a0 * sa.u0[idx] + v;
-          pre[k][m] = v;
-        }
-    }
-  }
-
-  // halo task: kind 0 bottom (column c, uT), 1 top (column c, uB), 2 left (row l, uR), 3 right (row l, uL)
-  int hkind = -1, hpos = 0, hline = 0;
-  {
-    int t = tid;
-    if (t < TX * NSP) {
-      hkind = 0;
-      hpos = t % TX;
-      hline = t / TX;
-    } else if ((t -= TX * NSP) < TX * NSP) {
-      hkind = 1;
-      hpos = t % TX;
-      hline = t / TX;
-    } else if ((t -= TX * NSP) < TY * NSP) {
-      hkind = 2;
-      hpos = t % TY;
-      hline = t / TY;
-    } else if ((t -= TY * NSP) < TY * NSP) {
-      hkind = 3;
-      hpos = t % TY;
-      hline = t / TY;
-    }
-    if (hkind == 0 || hkind == 1) {
-      if (hpos >= ncx) hkind = -1;
-    } else if (hkind >= 2) {
-      if (hpos >= ncy) hkind = -1;
-    }
-  }
-  double htr[4] = {0.0, 0.0, 0.0, 0.0};
-  if (hkind >= 0) {
-    int hi, hj;
-    if (hkind == 0) {
-      hi = i0 + hpos;
-      hj = j0 - 1;
-    } else if (hkind == 1) {
-      hi = i0 + hpos;
-      hj = j0 + ncy;
-    } else if (hkind == 2) {
-      hi = i0 - 1;
-      hj = j0 + hpos;
-    } else {
-      hi = i0 + ncx;
-      hj = j0 + hpos;
-    }
-    int si = hi, sj = hj;
-    if (geo.periodic) {
-      si = (hi == 0) ? geo.nx : (hi == geo.nx + 1 ? 1 : hi);
-      sj = (hj == 0) ? geo.ny : (hj == geo.ny + 1 ? 1 : hj);
-    }
-    const long long he = (long long)si + nxg * sj;
-    const bool column = hkind <= 1;
-    const double *w = (hkind == 0 || hkind == 2) ? ops.lr : ops.ll;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double s = 0.0;
-#pragma unroll
-      for (int q = 0; q < NSP; ++q) {
-        const long long off = column ? (hline + NSP * (q + NSP * m)) : (q + NSP * (hline + NSP * m));
-        s += u[he + ldp * off] * w[q];
-      }
-      htr[m] = s;
-    }
-    if (hkind == 0) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) SY(m, hline, 0, hpos) = htr[m];
-    } else if (hkind == 2) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m) SX(m, hline, hpos, 0) = htr[m];
-    }
-    if constexpr (EPI == EPI_RESIDUAL) {
-      // du vanishes on the ghost layer (example/shock-vortex.jl:27)
-      const bool ghost = hi == 0 || hi == geo.nx + 1 || hj == 0 || hj == geo.ny + 1;
-      if (ghost) {
-        const long long ge = (long long)hi + nxg * hj;
-#pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-          for (int q = 0; q < NSP; ++q) {
-            const long long off = column ? (hline + NSP * (q + NSP * m)) : (q + NSP * (hline + NSP * m));
-            sa.out[ge + ldp * off] = 0.0;
+          const double du = -(dux[k][m] + SU(k, line, m));
+          if constexpr (EPI == EPI_RESIDUAL) {
+            ST(outp, k, line, m, ebo, du);
+          } else {
+            ST(outp, k, line, m, ebo, pre[k][m] + sa.b * du);
+            if (accout) ST(accout, k, line, m, ebo, LD(accin, k, line, m, ebo) + sa.b2 * du);
           }
-      }
-    }
-  }
-  if constexpr (EPI == EPI_RESIDUAL) {
-    if (blockIdx.x == 0) {  // the four ghost corners
-      constexpr int per = NSP * NSP * 4;
-      for (int v = tid; v < 4 * per; v += NSP * 64) {
-        const int c = v / per, o = v % per;
-        const long long ci = (c & 1) ? geo.nx + 1 : 0, cj = (c & 2) ? geo.ny + 1 : 0;
-        sa.out[ci + nxg * cj + ldp * o] = 0.0;
-      }
-    }
-  }
-
-  // row: LDS transpose copy, x traces, contravariant x flux, its traces and dl derivative
-  double uLx[4], dux[NSP][4];
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    double a = 0.0, b = 0.0;
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) {
-      a += ur[k][m] * ops.ll[k];
-      b += ur[k][m] * ops.lr[k];
-    }
-    uLx[m] = a;
-    if (act) SX(m, line, ey, ex + 1) = b;
-  }
-  if (act) {
-#pragma unroll
-    for (int k = 0; k < NSP; ++k)
-#pragma unroll
-      for (int m = 0; m < 4; ++m) SU(k, line, m) = ur[k][m];
-  }
-  {
-    double f1[NSP][4];
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) {
-      XState s;
-      xstate(ur[k], gm1, s);
-#pragma unroll
-      for (int m = 0; m < 4; ++m) f1[k][m] = iJ11 * s.F[m];
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int q = 0; q < NSP; ++q) {
-        a += f1[q][m] * ops.ll[q];
-        b += f1[q][m] * ops.lr[q];
-      }
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) {
-        double d = 0.0;
-#pragma unroll
-        for (int q = 0; q < NSP; ++q) d += f1[q][m] * ops.dl[k + NSP * q];
-        // derivative minus the discontinuous-flux traces of the correction terms
-        dux[k][m] = d - a * ops.dhl[k] - b * ops.dhr[k];
-      }
-    }
-  }
-  __syncthreads();  // B1: sU rows, x traces and left/bottom halo traces are visible
-
-  // ---------------------------------------------------------------- P1: column sweep + x Riemann
-  double uBy[4], duy[NSP][4];
-  {
-    double uc[NSP][4];
-    if (act) {
-#pragma unroll
-      for (int q = 0; q < NSP; ++q)
-#pragma unroll
-        for (int m = 0; m < 4; ++m) uc[q][m] = SU(line, q, m);
-    } else {
-#pragma unroll
-      for (int q = 0; q < NSP; ++q)
-#pragma unroll
-        for (int m = 0; m < 4; ++m) uc[q][m] = (m == 0 || m == 3) ? 1.0 : 0.0;
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int q = 0; q < NSP; ++q) {
-        a += uc[q][m] * ops.ll[q];
-        b += uc[q][m] * ops.lr[q];
-      }
-      uBy[m] = a;
-      if (act) SY(m, line, ey + 1, ex) = b;
-    }
-    double f2[NSP][4];
-#pragma unroll
-    for (int q = 0; q < NSP; ++q) {
-      const double r = fr_rcp(uc[q][0]);
-      const double uu = uc[q][1] * r, vv = uc[q][2] * r;
-      const double p = gm1 * (uc[q][3] - 0.5 * (uc[q][1] * uu + uc[q][2] * vv));
-      f2[q][0] = iJ22 * uc[q][2];
-      f2[q][1] = iJ22 * (uc[q][1] * vv);
-      f2[q][2] = iJ22 * (uc[q][2] * vv + p);
-      f2[q][3] = iJ22 * ((uc[q][3] + p) * vv);
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int q = 0; q < NSP; ++q) {
-        a += f2[q][m] * ops.ll[q];
-        b += f2[q][m] * ops.lr[q];
-      }
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) {
-        double d = 0.0;
-#pragma unroll
-        for (int q = 0; q < NSP; ++q) d += f2[q][m] * ops.dl[k + NSP * q];
-        duy[k][m] = d - a * ops.dhl[k] - b * ops.dhr[k];
-      }
-    }
-  }
-  // x-face on the left of the own element: left state = right trace of the neighbour
-  if (act) {
-    double wl[4], fw[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) wl[m] = SX(m, line, ey, ex);
-    riemann_x4<FLUX>(wl, uLx, g, gm1, fw);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) SX(m, line, ey, ex) = fw[m];
-  }
-  if (hkind == 3) {  // right boundary face of the tile row, solved by the right halo task
-    double wl[4], fw[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) wl[m] = SX(m, hline, hpos, ncx);
-    riemann_x4<FLUX>(wl, htr, g, gm1, fw);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) SX(m, hline, hpos, ncx) = fw[m];
-  }
-  __syncthreads();  // B2: y traces visible
-
-  // ---------------------------------------------------------------- P2: y Riemann
-  if (act) {
-    double wb[4], fw[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) wb[m] = SY(m, line, ey, ex);
-    riemann_y4<FLUX>(wb, uBy, g, gm1, fw);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) SY(m, line, ey, ex) = fw[m];
-  }
-  if (hkind == 1) {  // top boundary face of the tile column
-    double wb[4], fw[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) wb[m] = SY(m, hline, ncy, hpos);
-    riemann_y4<FLUX>(wb, htr, g, gm1, fw);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) SY(m, hline, ncy, hpos) = fw[m];
-  }
-  __syncthreads();  // B3: interface fluxes visible; sU is free
-
-  // ---------------------------------------------------------------- P3: corrections
-  if (act) {
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      const double cb = SY(m, line, ey, ex) * iJ22;
-      const double ct = SY(m, line, ey + 1, ex) * iJ22;
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) SU(line, k, m) = duy[k][m] + cb * ops.dhl[k] + ct * ops.dhr[k];
-      const double cl = SX(m, line, ey, ex) * iJ11;
-      const double cr = SX(m, line, ey, ex + 1) * iJ11;
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) dux[k][m] += cl * ops.dhl[k] + cr * ops.dhr[k];
-    }
-  }
-  __syncthreads();  // B4: y contributions of every point visible
-
-  // ---------------------------------------------------------------- P4: assemble + epilogue
-  if (act) {
-    double *__restrict__ out = sa.out;
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) {
-        const long long idx = e + ldp * (k + NSP * (line + NSP * m));
-        const double du = -(dux[k][m] + SU(k, line, m));
-        if constexpr (EPI == EPI_RESIDUAL) {
-          out[idx] = du;
-        } else {
-          out[idx] = pre[k][m] + sa.b * du;
-          if (sa.acc_out) sa.acc_out[idx] = sa.acc_in[idx] + sa.b2 * du;
         }
-      }
+    }
+    if (!more) break;
+    vt = nvt;
+    T = NT;
+    __syncthreads();  // B5: sU is read above before the next tile writes it
   }
-  (void)NL;
 }
 
 // ------------------------------------------------------------------ ghost-layer helpers
@@ -440,8 +519,20 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
     ops.dhr[k] = c.ops.dhr[k];
   }
   for (int q = 0; q < NSP * NSP; ++q) ops.dl[q] = c.ops.dl[q];
-  const int nb = geo.ntx * geo.nty;
-  hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI>), dim3(nb), dim3(NSP * 64), 0, st, geo, ops, sa);
+  // persistent grid: as many workgroups as fit at once, spread so that every workgroup has the
+  // same number of tiles, and a multiple of 8 so that each stays in one XCD's range
+  static const int per_cu = [] {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fr2d_kernel<NSP, FLUX, EPI>, NSP * 64, 0) != hipSuccess)
+      n = 1;
+    return std::max(n, 1);
+  }();
+  const long long nb = (long long)geo.ntx * geo.nty;
+  const long long slots = (long long)per_cu * std::max(c.num_cu, 1);
+  const long long tpw = !FR2D_PERSIST ? 1 : c.tiles_per_wg > 0 ? c.tiles_per_wg : (nb + slots - 1) / slots;
+  long long grid = (nb + tpw - 1) / tpw;
+  grid = std::min(nb, (grid + 7) / 8 * 8);
+  hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI>), dim3((unsigned)grid), dim3(NSP * 64), 0, st, geo, ops, sa);
   FR_LAUNCHED("fr2d_kernel");
   return FR_OK;
 }

@@ -59,6 +59,23 @@ __device__ __forceinline__ int xcd_tile(int bid, int nb) {
   return (x < rem) ? x * (per + 1) + loc : rem * (per + 1) + (x - rem) * per + loc;
 }
 
+// ------------------------------------------------------------------ buffer access to one plane
+// A scalar buffer resource spans one plane of `bytes` bytes starting at `base`; lanes address it
+// with a 32-bit byte offset, so no per-lane 64-bit address arithmetic is spent, and the hardware
+// range check turns an access outside the plane into a zero load / dropped store.
+typedef unsigned int fr_u2 __attribute__((ext_vector_type(2)));
+constexpr int kBufferWord3 = 0x00020000;  // gfx9 raw buffer: 32-bit data format, no swizzle
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const double *base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), 0, (int)bytes, kBufferWord3);
+}
+__device__ __forceinline__ double bload(__amdgpu_buffer_rsrc_t r, unsigned byte_off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, byte_off, 0, 0));
+}
+__device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned byte_off, double v) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(fr_u2, v), r, byte_off, 0, 0);
+}
+
 // ------------------------------------------------------------------ FP64 reciprocal / square root
 // With FR_FAST_RCP the hardware estimates v_rcp_f64 / v_rsq_f64 are refined by two Newton-Raphson
 // (reciprocal) and one Goldschmidt (square root) iteration in FMA arithmetic, which leaves about

@@ -156,10 +156,12 @@ def prim_conserve1(prim, gamma):
     return w
 
 
-def riemann_ensemble(nprob=512, seed=2024, gamma=1.4, ncell_range=(64, 2048), degs=(2, 3, 4)):
+def riemann_ensemble(nprob=512, seed=2024, gamma=1.4, ncell_range=(64, 2048), degs=(2, 3, 4), smooth=0.0):
     """Seeded ensemble of 1D Riemann problems on [0, 1]: member b has ncell[b] uniform cells of
     degree deg[b]; piecewise-constant states rho, p in [0.1, 1], u in [-0.5, 0.5] left / right of
-    an interface x0 in [0.3, 0.7].  Returns (ncell, deg, dx, [u_b]) with u_b(i, p, k)."""
+    an interface x0 in [0.3, 0.7].  smooth > 0 replaces the jump by a tanh ramp of that many cells
+    (primitive variables blended), which keeps the unlimited scheme finite for parity runs.
+    Returns (ncell, deg, dx, [u_b]) with u_b(i, p, k)."""
     rng = np.random.default_rng(seed)
     ncell = rng.integers(ncell_range[0], ncell_range[1] + 1, size=nprob).astype(np.int32)
     deg = rng.choice(np.asarray(degs), size=nprob).astype(np.int32)
@@ -172,10 +174,14 @@ def riemann_ensemble(nprob=512, seed=2024, gamma=1.4, ncell_range=(64, 2048), de
         r, _ = np.polynomial.legendre.leggauss(d + 1)
         xc = (np.arange(n) + 0.5) * dx[b]
         x = xc[:, None] + 0.5 * dx[b] * r[None, :]
-        side = (x > x0[b]).astype(int)
-        rho = 0.1 + 0.9 * states[b, side, 0]
-        vel = -0.5 + states[b, side, 1]
-        pr = 0.1 + 0.9 * states[b, side, 2]
+        if smooth > 0.0:
+            h = 0.5 * (1.0 + np.tanh((x - x0[b]) / (smooth * dx[b])))
+        else:
+            h = (x > x0[b]).astype(np.float64)
+        mix = lambda k: (1.0 - h) * states[b, 0, k] + h * states[b, 1, k]  # noqa: E731
+        rho = 0.1 + 0.9 * mix(0)
+        vel = -0.5 + mix(1)
+        pr = 0.1 + 0.9 * mix(2)
         prim = np.stack([rho, vel, rho / (2.0 * pr)], axis=-1)
         members.append(np.asfortranarray(prim_conserve1(prim, gamma)))
     return ncell, deg, dx, members

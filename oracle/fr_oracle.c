@@ -983,6 +983,88 @@ int or_positive_limiter1d_scalar(double *u, int n, const double *weights, const 
   return 0;
 }
 
+/* src/dissipation.jl:61-123 on one 1D Euler element ue(p, k) (column-major nsp x 3): density
+ * squeeze toward the mean (t1), then, where lambda < eps at the two flux points (pre-squeeze
+ * traces, as in the reference) or at a solution point, the pressure squeeze t2 = min(tj, t0)
+ * with tj the closed-form root of tj_equation (or_tj_root with the y momentum zero).  The
+ * weights sum to one.  Returns -1 (element untouched) when the density assert fails. */
+int or_positive_limiter1d_euler(double *ue, int nsp, double g, const double *weights, const double *ll,
+                                const double *lr) {
+#define UE(p, k) ue[(p) + nsp * (k)]
+  double um[3];
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0;
+    for (int p = 0; p < nsp; ++p) s += UE(p, k) * weights[p];
+    um[k] = s;
+  }
+  double pm[3];
+  or_conserve_prim1(um, g, pm);
+  const double p_mean = 0.5 * um[0] / pm[2];
+  double bv[2][3];
+  for (int k = 0; k < 3; ++k) {
+    double a = 0.0, b = 0.0;
+    for (int p = 0; p < nsp; ++p) {
+      a += UE(p, k) * ll[p];
+      b += UE(p, k) * lr[p];
+    }
+    bv[0][k] = a;
+    bv[1][k] = b;
+  }
+  const double eps = fmin(1e-13, fmin(um[0], p_mean));
+  double rmin = fmin(bv[0][0], bv[1][0]);
+  for (int p = 0; p < nsp; ++p) rmin = fmin(rmin, UE(p, 0));
+  const double t1 = fmin((um[0] - eps) / (um[0] - rmin + 1e-8), 1.0);
+  if (!(t1 > 0.0 && t1 <= 1.0)) return -1;
+  for (int p = 0; p < nsp; ++p) UE(p, 0) = t1 * (UE(p, 0) - um[0]) + um[0];
+  const double um4[4] = {um[0], um[1], 0.0, um[2]};
+  int have = 0;
+  double tmin = INFINITY;
+  for (int f = 0; f < 2; ++f) {
+    double prim[3];
+    or_conserve_prim1(bv[f], g, prim);
+    if (prim[2] < eps) {
+      const double w4[4] = {bv[f][0], bv[f][1], 0.0, bv[f][2]};
+      tmin = fmin(tmin, or_tj_root(w4, um4, g, eps));
+      have = 1;
+    }
+  }
+  for (int p = 0; p < nsp; ++p) {
+    const double w[3] = {UE(p, 0), UE(p, 1), UE(p, 2)};
+    double prim[3];
+    or_conserve_prim1(w, g, prim);
+    if (prim[2] < eps) {
+      const double w4[4] = {w[0], w[1], 0.0, w[2]};
+      tmin = fmin(tmin, or_tj_root(w4, um4, g, eps));
+      have = 1;
+    }
+  }
+  if (have) {
+    const double t2 = fmin(tmin, 1.0);
+    for (int k = 0; k < 3; ++k)
+      for (int p = 0; p < nsp; ++p) UE(p, k) = t2 * (UE(p, k) - um[k]) + um[k];
+  }
+#undef UE
+  return 0;
+}
+
+/* The limiter over every cell of a 1D Euler state u(i, p, k); returns the failed count. */
+int or_fr1d_positivity(int ncell, int nsp, double g, const double *weights, const double *ll, const double *lr,
+                       double *u) {
+  int bad = 0;
+  for (int i = 0; i < ncell; ++i) {
+    double ue[3 * 8];
+    for (int k = 0; k < 3; ++k)
+      for (int p = 0; p < nsp; ++p) ue[p + nsp * k] = u[i + (size_t)ncell * (p + nsp * k)];
+    if (or_positive_limiter1d_euler(ue, nsp, g, weights, ll, lr)) {
+      ++bad;
+      continue;
+    }
+    for (int k = 0; k < 3; ++k)
+      for (int p = 0; p < nsp; ++p) u[i + (size_t)ncell * (p + nsp * k)] = ue[p + nsp * k];
+  }
+  return bad;
+}
+
 #undef U5
 
 /* =====================================================================

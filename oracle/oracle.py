@@ -193,6 +193,12 @@ def prim_conserve2(prim, g: float):
     return out
 
 
+def prim_conserve1(prim, g: float):
+    out = np.zeros(3)
+    lib().or_prim_conserve1(_p(_f64(prim)), C.c_double(g), _p(out))
+    return out
+
+
 def conserve_prim2(w, g: float):
     a = _f64(w)
     out = np.zeros(4)
@@ -298,6 +304,21 @@ def positive_limiter2d(ue, g, weights, ll, lr):
     w = _f64(weights)
     rc = lib().or_positive_limiter2d(_p(a), C.c_int(n), C.c_double(g), _p(w), _p(_f64(ll)), _p(_f64(lr)))
     return rc, a
+
+
+def positive_limiter1d_euler(ue, g, weights, ll, lr):
+    """One 1D Euler element ue(p, k) (nsp x 3); weights sum to one. Returns (rc, limited copy)."""
+    a = np.array(ue, dtype=np.float64, order="F")
+    rc = lib().or_positive_limiter1d_euler(_p(a), C.c_int(a.shape[0]), C.c_double(g), _p(_f64(weights)),
+                                           _p(_f64(ll)), _p(_f64(lr)))
+    return rc, a
+
+
+def fr1d_positivity(u, g, weights, ll, lr):
+    """The 1D limiter over every cell of u(i, p, k) in place; returns the failed-cell count."""
+    assert u.flags.f_contiguous and u.dtype == np.float64
+    return lib().or_fr1d_positivity(C.c_int(u.shape[0]), C.c_int(u.shape[1]), C.c_double(g), _p(_f64(weights)),
+                                    _p(_f64(ll)), _p(_f64(lr)), _p(u))
 
 
 def shock_detector(Se, deg, S0=None, kappa=4.0) -> bool:

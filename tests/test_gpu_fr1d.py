@@ -110,7 +110,9 @@ def test_euler_ssprk3_parity():
 
 
 def _ensemble(nprob, seed=7):
-    ncell, deg, dx, members = ics.riemann_ensemble(nprob, seed=seed, ncell_range=(64, 600))
+    # Riemann data ramped over 2 cells: the plain (unlimited) scheme stays finite on every member,
+    # which the exact piecewise-constant jumps do not guarantee at p = 4
+    ncell, deg, dx, members = ics.riemann_ensemble(nprob, seed=seed, ncell_range=(64, 600), smooth=2.0)
     return ncell, deg, dx, members
 
 

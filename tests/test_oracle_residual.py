@@ -243,3 +243,32 @@ def test_positivity_limiter_invariants():
     assert out[..., 0].min() >= eps - 1e-15
     p = 0.4 * (out[..., 3] - 0.5 * (out[..., 1] ** 2 + out[..., 2] ** 2) / out[..., 0])
     assert p.min() > -1e-12
+
+
+def test_positivity_limiter_1d_euler():
+    """dissipation.jl:61-123: cell means kept; density >= eps at solution and flux points after the
+    density squeeze; pressure >= eps after the pressure squeeze, whose factor is the root of
+    tj_equation (p(t) = eps) at the worst point; a positive element is returned untouched."""
+    deg = 4
+    ops = O.Ops(deg)
+    w = ops.w / 2.0
+    rng = np.random.default_rng(8)
+    nsp = deg + 1
+    prim = np.stack([1 + 0.2 * rng.random(nsp), 0.3 * rng.standard_normal(nsp), np.full(nsp, 0.6)], axis=-1)
+    good = np.asfortranarray(np.array([O.prim_conserve1(q, 1.4) for q in prim]))
+    rc, same = O.positive_limiter1d_euler(good, 1.4, w, ops.ll, ops.lr)
+    assert rc == 0 and np.array_equal(same, good)
+    bad = good.copy(order="F")
+    bad[1, 0] = -0.2          # negative density at one point
+    bad[3, 2] *= 0.02         # pressure driven negative at another
+    mean0 = w @ bad
+    rc, out = O.positive_limiter1d_euler(bad, 1.4, w, ops.ll, ops.lr)
+    assert rc == 0
+    assert np.allclose(w @ out, mean0, rtol=1e-13, atol=1e-15)
+    rho = np.concatenate([out[:, 0], [ops.ll @ out[:, 0], ops.lr @ out[:, 0]]])
+    eps = min(1e-13, mean0[0])
+    assert rho.min() >= eps - 1e-14
+    p = 0.4 * (out[:, 2] - 0.5 * out[:, 1] ** 2 / out[:, 0])
+    assert p.min() >= -1e-12
+    # the squeeze is the largest admissible one: the worst point sits on p = eps (up to round-off)
+    assert abs(p.min()) < 1e-10
