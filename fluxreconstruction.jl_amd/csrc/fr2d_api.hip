@@ -132,12 +132,25 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
     return fail(FR_E_INVALID, "fr2d_create: operator construction failed");
   }
   const int n = c->nsp, np = n * n;
-  // caller-provided operator data (FRPSpace2D fields) take precedence
-  if (d->ll) std::memcpy(c->ops.ll, d->ll, sizeof(double) * n);
-  if (d->lr) std::memcpy(c->ops.lr, d->lr, sizeof(double) * n);
-  if (d->dl) std::memcpy(c->ops.dl, d->dl, sizeof(double) * n * n);
-  if (d->dhl) std::memcpy(c->ops.dhl, d->dhl, sizeof(double) * n);
-  if (d->dhr) std::memcpy(c->ops.dhr, d->dhr, sizeof(double) * n);
+  // Caller-provided residual operators (FRPSpace2D fields) must be the Gauss-Legendre / Radau
+  // ones that FRPSpace2D builds (src/struct.jl:193) and that the kernels carry as compile-time
+  // constants; a caller's independently computed copy differs from them by round-off only.
+  {
+    auto same = [&](const double *mine, const double *theirs, int cnt) {
+      if (!theirs) return true;
+      double scale = 0.0;
+      for (int q = 0; q < cnt; ++q) scale = std::max(scale, std::fabs(mine[q]));
+      for (int q = 0; q < cnt; ++q)
+        if (!(std::fabs(mine[q] - theirs[q]) <= 1e-12 * scale)) return false;
+      return true;
+    };
+    if (!same(c->ops.ll, d->ll, n) || !same(c->ops.lr, d->lr, n) || !same(c->ops.dl, d->dl, n * n) ||
+        !same(c->ops.dhl, d->dhl, n) || !same(c->ops.dhr, d->dhr, n)) {
+      delete c;
+      return fail(FR_E_UNSUPPORTED,
+                  "fr2d_create: ll/lr/dl/dhl/dhr differ from the Gauss-Legendre / Radau operators of this degree");
+    }
+  }
   if (d->V) std::memcpy(c->ops.V, d->V, sizeof(double) * np * np);
   if (d->iV) std::memcpy(c->ops.iV, d->iV, sizeof(double) * np * np);
   hipError_t e = hipGetDevice(&c->device);

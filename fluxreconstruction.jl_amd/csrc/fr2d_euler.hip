@@ -18,6 +18,7 @@
 
 #include "fr_common.h"
 #include "fr2d_internal.h"
+#include "fr_ops_table.h"
 
 namespace fr {
 
@@ -27,9 +28,13 @@ namespace fr {
 #endif
 constexpr int TX = FR2D_TX, TY = 64 / FR2D_TX, TE = TX * TY;
 static_assert(TE == 64 && (TX & (TX - 1)) == 0, "tile must hold 64 elements, TX a power of two");
-// 1: persistent workgroups loop over tiles and load the next tile's rows during the current one
-#ifndef FR2D_PERSIST
-#define FR2D_PERSIST 0
+// 1: each workgroup computes two tiles and loads the second one's rows during the first
+#ifndef FR2D_PAIR
+#define FR2D_PAIR 0
+#endif
+// tile order: 0 = raster over (bx, by); W > 0 = vertical strips of W tiles
+#ifndef FR2D_STRIP
+#define FR2D_STRIP 0
 #endif
 // global accesses: 0 = flat global addressing, 1 = one scalar buffer resource per plane
 #ifndef FR2D_BUFFER
@@ -37,10 +42,6 @@ static_assert(TE == 64 && (TX & (TX - 1)) == 0, "tile must hold 64 elements, TX 
 #endif
 static_assert(2 * (TX + TY) <= 64, "the halo tasks of one line must fit in one wavefront");
 
-template <int NSP>
-struct Ops2D {
-  double ll[NSP], lr[NSP], dl[NSP * NSP], dhl[NSP], dhr[NSP];
-};
 
 struct Geo2D {
   int nx, ny, ntx, nty;
@@ -61,8 +62,10 @@ struct StageArgs {
 };
 
 template <int NSP, int FLUX, int EPI>
-__global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const Ops2D<NSP> ops,
-                                                        const StageArgs sa) {
+__global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const StageArgs sa) {
+  // operators are compile-time constants (Gauss-Legendre points, Radau correction), so they
+  // occupy no scalar registers for the length of the kernel
+  constexpr StdOps2D<NSP> ops{};
   __shared__ double sU[NSP * NSP * 4 * TE];
   __shared__ double sX[4 * NSP * TY * (TX + 1)];
   __shared__ double sY[4 * NSP * (TY + 1) * TX];
@@ -113,7 +116,26 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
   auto tile_geo = [&](int vt) {
     TileGeo t;
     const int tile = xcd_tile(vt, ntiles);
-    const int bx = tile % geo.ntx, by = tile / geo.ntx;
+    int bx, by;
+#if FR2D_STRIP > 0
+    // strip-major order: vertical strips FR2D_STRIP tiles wide, rows inside a strip, so that the
+    // tile below a tile was processed FR2D_STRIP tiles earlier (its lines are still in L2)
+    {
+      const int full = geo.ntx / FR2D_STRIP, per = FR2D_STRIP * geo.nty, s = tile / per;
+      if (s < full) {
+        const int r = tile - s * per;
+        by = r / FR2D_STRIP;
+        bx = s * FR2D_STRIP + r % FR2D_STRIP;
+      } else {
+        const int wl = geo.ntx - full * FR2D_STRIP, r = tile - full * per;
+        by = r / wl;
+        bx = full * FR2D_STRIP + r % wl;
+      }
+    }
+#else
+    bx = tile % geo.ntx;
+    by = tile / geo.ntx;
+#endif
     t.i0 = 1 + bx * TX;
     t.j0 = 1 + by * TY;
     t.ncx = min(TX, geo.nx - t.i0 + 1);
@@ -149,19 +171,10 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
     }
   }
 
-  // Persistent loop over tiles vt = blockIdx.x, blockIdx.x + gridDim.x, ...: the own rows of the
-  // next tile are loaded while the current one is computed.  With gridDim.x a multiple of 8 every
-  // tile of a workgroup maps into the same XCD's contiguous range (xcd_tile).
-  int vt = blockIdx.x;
-  TileGeo T = tile_geo(vt);
-  double ur[NSP][4], u0r[NSP][4];
-  load_row(T, ur, u0r);
-  for (;;) {
-#if FR2D_PERSIST
-    // opaque per iteration: keeps the plane addresses from being hoisted out of the tile loop,
-    // where they would stay live in scalar registers for the whole kernel
-    asm volatile("" : "+s"(ldp), "+s"(u), "+s"(u0p), "+s"(outp), "+s"(accin), "+s"(accout));
-#endif
+  // One tile of the workgroup; with `more`, the own rows of the next tile NT are loaded into
+  // nur / nu0r as soon as the current rows are consumed, so they arrive during this tile's work.
+  auto run_tile = [&](const TileGeo &T, double (&ur)[NSP][4], double (&u0r)[NSP][4], const bool more,
+                      const TileGeo &NT, double (&nur)[NSP][4], double (&nu0r)[NSP][4]) {
     const int i0 = T.i0, j0 = T.j0, ncx = T.ncx, ncy = T.ncy;
     const bool act = T.act;
     const unsigned ebo = T.ebo;
@@ -322,13 +335,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
     }
 
     // the own rows are consumed: start loading the next tile's
-    const int nvt = vt + gridDim.x;
-    const bool more = FR2D_PERSIST && nvt < ntiles;
-    TileGeo NT = T;
-    if (more) {
-      NT = tile_geo(nvt);
-      load_row(NT, ur, u0r);
-    }
+    if (more) load_row(NT, nur, nu0r);
     __syncthreads();  // B1: sU rows, x traces and left/bottom halo traces are visible
 
     // -------------------------------------------------------------- P1: column sweep + x Riemann
@@ -454,11 +461,31 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const O
           }
         }
     }
-    if (!more) break;
-    vt = nvt;
-    T = NT;
-    __syncthreads();  // B5: sU is read above before the next tile writes it
+  };
+
+  const int ntg = gridDim.x;
+  const int vt0 = blockIdx.x;
+  if (vt0 >= ntiles) return;  // whole workgroup: grid rounded up to a multiple of 8
+  double ua[NSP][4], ua0[NSP][4], ub[NSP][4], ub0[NSP][4];
+  const TileGeo T0 = tile_geo(vt0);
+  load_row(T0, ua, ua0);
+#if FR2D_PAIR
+  // two tiles per workgroup (vt0 and vt0 + gridDim.x, the same XCD range when gridDim.x is a
+  // multiple of 8): the second tile's rows are in flight while the first is computed
+  const int vt1 = vt0 + ntg;
+  const bool two = vt1 < ntiles;
+  const TileGeo T1 = tile_geo(two ? vt1 : vt0);
+  run_tile(T0, ua, ua0, two, T1, ub, ub0);
+  if (two) {
+    // the second tile recomputes its plane addresses instead of keeping the first tile's live
+    asm volatile("" : "+s"(ldp), "+s"(u), "+s"(u0p), "+s"(outp), "+s"(accin), "+s"(accout));
+    __syncthreads();  // sU is read at the end of the first tile before the second writes it
+    run_tile(T1, ub, ub0, false, T1, ua, ua0);
   }
+#else
+  (void)ntg;
+  run_tile(T0, ua, ua0, false, T0, ub, ub0);
+#endif
 }
 
 // ------------------------------------------------------------------ ghost-layer helpers
@@ -511,14 +538,6 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   geo.g = c.gamma;
   geo.gm1 = c.gamma - 1.0;
   geo.periodic = c.bc == FR_BC_PERIODIC;
-  Ops2D<NSP> ops;
-  for (int k = 0; k < NSP; ++k) {
-    ops.ll[k] = c.ops.ll[k];
-    ops.lr[k] = c.ops.lr[k];
-    ops.dhl[k] = c.ops.dhl[k];
-    ops.dhr[k] = c.ops.dhr[k];
-  }
-  for (int q = 0; q < NSP * NSP; ++q) ops.dl[q] = c.ops.dl[q];
   // persistent grid: as many workgroups as fit at once, spread so that every workgroup has the
   // same number of tiles, and a multiple of 8 so that each stays in one XCD's range
   static const int per_cu = [] {
@@ -529,10 +548,11 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   }();
   const long long nb = (long long)geo.ntx * geo.nty;
   const long long slots = (long long)per_cu * std::max(c.num_cu, 1);
-  const long long tpw = !FR2D_PERSIST ? 1 : c.tiles_per_wg > 0 ? c.tiles_per_wg : (nb + slots - 1) / slots;
+  (void)slots;
+  const long long tpw = FR2D_PAIR ? 2 : 1;
   long long grid = (nb + tpw - 1) / tpw;
   grid = std::min(nb, (grid + 7) / 8 * 8);
-  hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI>), dim3((unsigned)grid), dim3(NSP * 64), 0, st, geo, ops, sa);
+  hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI>), dim3((unsigned)grid), dim3(NSP * 64), 0, st, geo, sa);
   FR_LAUNCHED("fr2d_kernel");
   return FR_OK;
 }
