@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <string>
 #include <vector>
 
 #include "fr_common.h"
@@ -17,9 +18,11 @@
 namespace fr {
 
 constexpr int NSP_MAX = 5;
-constexpr int OPS_STRIDE = 4 * NSP_MAX + NSP_MAX * NSP_MAX;  // ll, lr, dhl, dhr, dl
-
-// device operator table: entry nsp (2..5) at OPS_STRIDE*(nsp-2): ll[5] lr[5] dhl[5] dhr[5] dl[25]
+// device operator table: entry nsp (2..5) at OPS_STRIDE*(nsp-2):
+// ll[5] lr[5] dhl[5] dhr[5] dl[25] | w/2[5] (Gauss weights summing to one) V1[25] iV1[25]
+constexpr int OFF_W = 4 * NSP_MAX + NSP_MAX * NSP_MAX;
+constexpr int OFF_V = OFF_W + NSP_MAX, OFF_IV = OFF_V + NSP_MAX * NSP_MAX;
+constexpr int OPS_STRIDE = OFF_IV + NSP_MAX * NSP_MAX;
 struct Prob1D {
   long long off;  // first double of the member in the concatenated state
   int ncell, nsp;
@@ -178,6 +181,159 @@ __global__ __launch_bounds__(256) void fr1d_kernel(const Prob1D *__restrict__ pr
   }
 }
 
+// ------------------------------------------------------------------ 1D dissipation
+// Per-cell dissipation of 1D Euler states, one thread per cell over the same work lists:
+//  mode 1: modal sensor + filter of example/euler_highlevel.jl:37-52 (rho modes u_hat = iV rho,
+//          su = u_hat[end]^2 / (sum u_hat^2 + 1e-6), shock_detector(log10 su, deg) with
+//          S0 = -3 log10(deg), kappa = 4; on a shock all variables filtered with
+//          lambda = dt exp(0.875 deg) 0.15, u_hat[n] /= 1 + lambda (n+1)^2 n^2);
+//  mode 2: positivity limiter of src/dissipation.jl:61-123 (density squeeze toward the cell
+//          mean, then the pressure squeeze with the closed-form tj_equation root).
+// count[0] += flagged (mode 1) / limited (mode 2) cells, count[1] += cells whose density
+// assert failed (left untouched).
+template <int NSP>
+__device__ __forceinline__ void diss1d_cell(double *__restrict__ u, long long n, long long e, const double *ops,
+                                            int mode, double g, double dt, bool &flag, bool &bad) {
+  const double *ll = ops, *lr = ops + NSP_MAX, *w = ops + OFF_W, *V = ops + OFF_V, *iV = ops + OFF_IV;
+  double x[NSP][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int p = 0; p < NSP; ++p) x[p][k] = u[e + n * (p + NSP * k)];
+  flag = false;
+  bad = false;
+  if (mode == 1) {
+    double den = 0.0, last = 0.0;
+#pragma unroll
+    for (int a = 0; a < NSP; ++a) {
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < NSP; ++b) s += iV[a + NSP * b] * x[b][0];
+      den += s * s;
+      if (a == NSP - 1) last = s * s;
+    }
+    const int deg = NSP - 1;
+    const double S0 = -3.0 * log10((double)deg), kappa = 4.0;
+    const double Se = log10(last / (den + 1e-6));
+    double sigma;
+    if (Se < S0 - kappa)
+      sigma = 1.0;
+    else if (Se < S0 + kappa)
+      sigma = 0.5 * (1.0 - sin(0.5 * M_PI * (Se - S0) / kappa));
+    else
+      sigma = 0.0;
+    if (!(sigma < 0.99)) return;
+    flag = true;
+    const double lambda = dt * exp(0.875 * deg) * 0.15;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double mh[NSP];
+#pragma unroll
+      for (int a = 0; a < NSP; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NSP; ++b) s += iV[a + NSP * b] * x[b][k];
+        mh[a] = a == 0 ? s : s / (1.0 + lambda * (double)((a + 1) * (a + 1) * a * a));
+      }
+#pragma unroll
+      for (int a = 0; a < NSP; ++a) {
+        double s = 0.0;
+#pragma unroll
+        for (int b = 0; b < NSP; ++b) s += V[a + NSP * b] * mh[b];
+        u[e + n * (a + NSP * k)] = s;
+      }
+    }
+    return;
+  }
+  // positivity limiter
+  double um[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double s = 0.0;
+#pragma unroll
+    for (int p = 0; p < NSP; ++p) s += x[p][k] * w[p];
+    um[k] = s;
+  }
+  const double um4[4] = {um[0], um[1], 0.0, um[2]};
+  const double p_mean = 0.5 * um[0] / prim_lambda(um4, g);
+  double bv[2][4];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int p = 0; p < NSP; ++p) {
+      a += x[p][k] * ll[p];
+      b += x[p][k] * lr[p];
+    }
+    bv[0][k < 2 ? k : 3] = a;
+    bv[1][k < 2 ? k : 3] = b;
+  }
+  bv[0][2] = bv[1][2] = 0.0;
+  const double eps = fmin(1e-13, fmin(um[0], p_mean));
+  double rmin = fmin(bv[0][0], bv[1][0]);
+#pragma unroll
+  for (int p = 0; p < NSP; ++p) rmin = fmin(rmin, x[p][0]);
+  const double t1 = fmin((um[0] - eps) / (um[0] - rmin + 1e-8), 1.0);
+  if (!(t1 > 0.0 && t1 <= 1.0)) {
+    bad = true;
+    return;
+  }
+  bool have = false;
+  double tmin = 1.0e300;
+#pragma unroll
+  for (int f = 0; f < 2; ++f)
+    if (prim_lambda(bv[f], g) < eps) {
+      have = true;
+      tmin = fmin(tmin, tj_root(bv[f], um4, g, eps));
+    }
+#pragma unroll
+  for (int p = 0; p < NSP; ++p) {
+    x[p][0] = t1 * (x[p][0] - um[0]) + um[0];
+    const double w4[4] = {x[p][0], x[p][1], 0.0, x[p][2]};
+    if (prim_lambda(w4, g) < eps) {
+      have = true;
+      tmin = fmin(tmin, tj_root(w4, um4, g, eps));
+    }
+  }
+  const double t2 = have ? fmin(tmin, 1.0) : 1.0;
+  flag = t1 < 1.0 || have;
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+#pragma unroll
+    for (int p = 0; p < NSP; ++p) {
+      const double v = have ? t2 * (x[p][k] - um[k]) + um[k] : x[p][k];
+      u[e + n * (p + NSP * k)] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void fr1d_diss_kernel(const Prob1D *__restrict__ probs, const Work1D *__restrict__ work,
+                                                        int nwork, const double *__restrict__ ops, double *u, int mode,
+                                                        double g, double dt, int *count) {
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const Work1D wk = work[wi];
+    const Prob1D pb = probs[wk.prob];
+    const int tid = threadIdx.x, T = blockDim.x;
+    const int e = wk.first - 1 + tid;
+    const bool interior = tid >= 1 && tid <= T - 2 && e < pb.ncell;
+    bool flag = false, bad = false;
+    if (interior) {
+      const double *o = ops + OPS_STRIDE * (pb.nsp - 2);
+      double *um = u + pb.off;
+      switch (pb.nsp) {
+        case 2: diss1d_cell<2>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+        case 3: diss1d_cell<3>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+        case 4: diss1d_cell<4>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+        default: diss1d_cell<5>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+      }
+    }
+    const unsigned long long bf = __ballot(flag), bb = __ballot(bad);
+    if ((tid & 63) == 0) {
+      if (bf) atomicAdd(count, __popcll(bf));
+      if (bb) atomicAdd(count + 1, __popcll(bb));
+    }
+  }
+}
+
 // ------------------------------------------------------------------ batched problem set
 struct Batch1D {
   int eq = FR_EQ_EULER, flux = FR_FLUX_HLL, bc = FR_BC1_DIRICHLET, correction = FR_CORR_RADAU;
@@ -189,6 +345,7 @@ struct Batch1D {
   double *d_ops = nullptr;
   double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_acc = nullptr;
   double *d_in = nullptr, *d_out = nullptr;
+  int *d_count = nullptr;
   // work lists per thread count (64, 128, 256): concatenated list and per-member sub-lists
   int threads = 256, grid = 0;  // grid 0 = one workgroup per work item
   Work1D *d_work[3] = {nullptr, nullptr, nullptr};
@@ -215,6 +372,11 @@ struct Batch1D {
         t[3 * NSP_MAX + q] = o.dhr[q];
       }
       for (int q = 0; q < nsp * nsp; ++q) t[4 * NSP_MAX + q] = o.dl[q];
+      for (int q = 0; q < nsp; ++q) t[OFF_W + q] = 0.5 * o.w[q];
+      for (int q = 0; q < nsp * nsp; ++q) {
+        t[OFF_V + q] = o.V1[q];
+        t[OFF_IV + q] = o.iV1[q];
+      }
     }
     FR_HIP(hipMalloc(&d_ops, tab.size() * sizeof(double)));
     FR_HIP(hipMemcpy(d_ops, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -247,6 +409,7 @@ struct Batch1D {
     for (double *b : bufs)
       if (b) (void)hipFree(b);
     if (d_probs) (void)hipFree(d_probs);
+    if (d_count) (void)hipFree(d_count);
     for (auto *w : d_work)
       if (w) (void)hipFree(w);
     if (stream) (void)hipStreamDestroy(stream);
@@ -329,6 +492,27 @@ struct Batch1D {
           return fail(FR_E_INVALID, "fr1d_step: unknown scheme");
       }
     }
+    return FR_OK;
+  }
+
+  // mode 1 sensor + filter, 2 positivity; n_out (optional) receives the two counters
+  int dissipation(int mode, double dt, int *n_out) {
+    if (!d_count) FR_HIP(hipMalloc(&d_count, 2 * sizeof(int)));
+    FR_HIP(hipMemsetAsync(d_count, 0, 2 * sizeof(int), stream));
+    const int v = tindex(threads);
+    const int g = grid > 0 ? std::min(grid, nwork[v]) : nwork[v];
+    hipLaunchKernelGGL(fr1d_diss_kernel, dim3(g), dim3(threads), 0, stream, d_probs, d_work[v], nwork[v], d_ops, d_u,
+                       mode, gamma, dt, d_count);
+    FR_LAUNCHED("fr1d_diss_kernel");
+    int h[2] = {0, 0};
+    FR_HIP(hipMemcpyAsync(h, d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
+    FR_HIP(hipStreamSynchronize(stream));
+    if (n_out) {
+      n_out[0] = h[0];
+      n_out[1] = h[1];
+    }
+    if (mode == 2 && h[1] > 0)
+      return fail(FR_E_LIMITER, "fr1d_positivity: " + std::to_string(h[1]) + " cell(s) with non-positive mean density");
     return FR_OK;
   }
 
@@ -581,6 +765,40 @@ int fr_ens_autotune(fr_ens_ctx *c, double dt, int *chosen) {
   c->grid = mult[best_i % 5] * ncu;
   if (chosen) *chosen = best_i;
   return FR_OK;
+}
+
+int fr1d_sensor_filter(fr1d_ctx *c, double dt, int *n_flagged) {
+  FR_REQUIRE(c && std::isfinite(dt) && dt >= 0.0, FR_E_INVALID, "fr1d_sensor_filter: bad arguments");
+  FR_REQUIRE(c->eq == FR_EQ_EULER, FR_E_UNSUPPORTED, "fr1d_sensor_filter: Euler states only");
+  int h[2];
+  int rc = c->dissipation(1, dt, h);
+  if (n_flagged) *n_flagged = h[0];
+  return rc;
+}
+
+int fr1d_positivity(fr1d_ctx *c, int *n_limited) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr1d_positivity: null context");
+  FR_REQUIRE(c->eq == FR_EQ_EULER, FR_E_UNSUPPORTED, "fr1d_positivity: Euler states only");
+  int h[2] = {0, 0};
+  int rc = c->dissipation(2, 0.0, h);
+  if (n_limited) *n_limited = h[0];
+  return rc;
+}
+
+int fr_ens_sensor_filter(fr_ens_ctx *c, double dt, int *n_flagged) {
+  FR_REQUIRE(c && std::isfinite(dt) && dt >= 0.0, FR_E_INVALID, "fr_ens_sensor_filter: bad arguments");
+  int h[2];
+  int rc = c->dissipation(1, dt, h);
+  if (n_flagged) *n_flagged = h[0];
+  return rc;
+}
+
+int fr_ens_positivity(fr_ens_ctx *c, int *n_limited) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr_ens_positivity: null context");
+  int h[2] = {0, 0};
+  int rc = c->dissipation(2, 0.0, h);
+  if (n_limited) *n_limited = h[0];
+  return rc;
 }
 
 }  // extern "C"

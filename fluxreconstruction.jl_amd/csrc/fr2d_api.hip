@@ -155,7 +155,6 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
   if (d->iV) std::memcpy(c->ops.iV, d->iV, sizeof(double) * np * np);
   hipError_t e = hipGetDevice(&c->device);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, c->device);
-  if (const char *tpw = std::getenv("FR2D_TILES_PER_WG")) c->tiles_per_wg = std::max(0, std::atoi(tpw));
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->d_count, 4 * sizeof(int));
   const size_t nd = 2 * (size_t)np * np + np + 2 * n;
@@ -186,7 +185,7 @@ int fr2d_destroy(fr2d_ctx *c) {
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->d_count) (void)hipFree(c->d_count);
-  if (c->d_mlp) (void)hipFree(c->d_mlp);
+  // d_mlp refers to the caller's network (fr2d_set_sensor_mlp), which the context does not own
   if (c->d_feat) (void)hipFree(c->d_feat);
   if (c->d_strength) (void)hipFree(c->d_strength);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -319,7 +318,7 @@ int fr2d_step_graph(fr2d_ctx *c, int scheme, double dt, int nsteps, int nrep) {
 
 int fr2d_fill_ghost(fr2d_ctx *c, int rule) {
   FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_fill_ghost: no state");
-  FR_REQUIRE(rule == FR_GHOST_PERIODIC || rule == FR_GHOST_OUTFLOW, FR_E_INVALID, "fr2d_fill_ghost: unknown rule");
+  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_INFLOW_LEFT, FR_E_INVALID, "fr2d_fill_ghost: unknown rule");
   return fr2d_launch_fill_ghost(*c, c->d_u, rule, stream_of(*c));
 }
 

@@ -85,39 +85,6 @@ __global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int 
   }
 }
 
-// Root of tj_equation (src/dissipation.jl:208-215) in closed form: rho, m and E are affine in t,
-// so rho(t) (p(t) - eps) = (g-1)(rho E - |m|^2/2) - eps rho is a quadratic A t^2 + B t + C; the
-// largest root in [0, 1] is returned (the one NewtonRaphson reaches from t = 1).
-__device__ __forceinline__ double tj_root(const double *ut, const double *um, double g, double eps) {
-  const double r0 = um[0], r1 = ut[0] - um[0];
-  const double x0 = um[1], x1 = ut[1] - um[1];
-  const double y0 = um[2], y1 = ut[2] - um[2];
-  const double e0 = um[3], e1 = ut[3] - um[3];
-  const double A = (g - 1.0) * (r1 * e1 - 0.5 * (x1 * x1 + y1 * y1));
-  const double B = (g - 1.0) * (r0 * e1 + r1 * e0 - (x0 * x1 + y0 * y1)) - eps * r1;
-  const double C = (g - 1.0) * (r0 * e0 - 0.5 * (x0 * x0 + y0 * y0)) - eps * r0;
-  double best = 0.0;
-  if (fabs(A) < 1e-300) {
-    if (B != 0.0) {
-      const double t = -C / B;
-      if (t >= 0.0 && t <= 1.0) best = t;
-    }
-    return best;
-  }
-  const double disc = B * B - 4.0 * A * C;
-  if (disc < 0.0) return 0.0;
-  const double sq = sqrt(disc);
-  const double qq = -0.5 * (B + (B >= 0.0 ? sq : -sq));
-  const double t1 = qq / A, t2 = (qq != 0.0) ? C / qq : t1;
-  if (t1 >= 0.0 && t1 <= 1.0 && t1 > best) best = t1;
-  if (t2 >= 0.0 && t2 <= 1.0 && t2 > best) best = t2;
-  return best;
-}
-
-// lambda = rho / (2 p) of KitBase conserve_prim
-__device__ __forceinline__ double prim_lambda(const double *w, double g) {
-  return 0.5 * w[0] / (g - 1.0) / (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
-}
 
 // Positivity limiter of src/dissipation.jl:125-206 on every interior cell, weights wp/4.
 template <int NSP>

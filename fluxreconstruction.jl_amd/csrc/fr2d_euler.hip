@@ -32,11 +32,16 @@ static_assert(TE == 64 && (TX & (TX - 1)) == 0, "tile must hold 64 elements, TX 
 #ifndef FR2D_PAIR
 #define FR2D_PAIR 0
 #endif
+// 1: the tile's right x-faces and top y-faces share one Riemann pass after the y traces
+#ifndef FR2D_MERGE_HALO
+#define FR2D_MERGE_HALO 1
+#endif
 // tile order: 0 = raster over (bx, by); W > 0 = vertical strips of W tiles
 #ifndef FR2D_STRIP
 #define FR2D_STRIP 0
 #endif
-// global accesses: 0 = flat global addressing, 1 = one scalar buffer resource per plane
+// global accesses: 0 = flat global addressing, 1 = one scalar buffer resource per plane,
+// 2 = one buffer resource per array with the plane offset as scalar offset (arrays < 4 GiB)
 #ifndef FR2D_BUFFER
 #define FR2D_BUFFER 0
 #endif
@@ -91,7 +96,18 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
   };
   auto SU = [&](int k, int l, int m) -> double & { return sU[((k * NSP + l) * 4 + m) * TE + lane]; };
   // value of plane (k, l, m) at the element with byte offset bo inside the plane
-#if FR2D_BUFFER
+#if FR2D_BUFFER == 2
+  // one buffer resource per array (arrays below 4 GiB), the plane offset in the scalar offset
+  const unsigned ldp8 = (unsigned)(ldp * 8);
+  auto LD = [&](const double *base, int k, int l, int m, unsigned bo) -> double {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(
+        plane_rsrc(base, 0xFFFFFFFFu), bo, (int)(ldp8 * (unsigned)(k + NSP * (l + NSP * m))), 0));
+  };
+  auto ST = [&](double *base, int k, int l, int m, unsigned bo, double v) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(fr_u2, v), plane_rsrc(base, 0xFFFFFFFFu), bo,
+                                          (int)(ldp8 * (unsigned)(k + NSP * (l + NSP * m))), 0);
+  };
+#elif FR2D_BUFFER
   const unsigned pbytes = (unsigned)(ldp * 8);
   auto LD = [&](const double *base, int k, int l, int m, unsigned bo) -> double {
     return bload(plane_rsrc(base + ldp * (k + NSP * (l + NSP * m)), pbytes), bo);
@@ -401,6 +417,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
 #pragma unroll
       for (int m = 0; m < 4; ++m) SX(m, line, ey, ex) = fw[m];
     }
+#if !FR2D_MERGE_HALO
     if (hkind == 3) {  // right boundary face of the tile row, solved by the right halo task
       double wl[4], fw[4];
 #pragma unroll
@@ -409,6 +426,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
 #pragma unroll
       for (int m = 0; m < 4; ++m) SX(m, line, hpos, ncx) = fw[m];
     }
+#endif
     __syncthreads();  // B2: y traces visible
 
     // -------------------------------------------------------------- P2: y Riemann
@@ -420,6 +438,33 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
 #pragma unroll
       for (int m = 0; m < 4; ++m) SY(m, line, ey, ex) = fw[m];
     }
+#if FR2D_MERGE_HALO
+    // One divergent pass for the tile's right x-faces (hkind 3, solved by the right halo task)
+    // and top y-faces (hkind 1): a y-face is solved in its normal frame n = (0, 1), i.e. on
+    // (rho, m_y, -m_x, E) with the flux rotated back as (f0, -f2, f1, f3).
+    if (hkind == 1 || hkind == 3) {
+      const bool xf = hkind == 3;
+      double *slot[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) slot[m] = xf ? &SX(m, line, hpos, ncx) : &SY(m, line, ncy, hpos);
+      double w[4], a[4], b[4], fw[4];
+#pragma unroll
+      for (int m = 0; m < 4; ++m) w[m] = *slot[m];
+      a[0] = w[0];
+      a[1] = xf ? w[1] : w[2];
+      a[2] = xf ? w[2] : -w[1];
+      a[3] = w[3];
+      b[0] = htr[0];
+      b[1] = xf ? htr[1] : htr[2];
+      b[2] = xf ? htr[2] : -htr[1];
+      b[3] = htr[3];
+      riemann_x4<FLUX>(a, b, g, gm1, fw);
+      *slot[0] = fw[0];
+      *slot[1] = xf ? fw[1] : -fw[2];
+      *slot[2] = xf ? fw[2] : fw[1];
+      *slot[3] = fw[3];
+    }
+#else
     if (hkind == 1) {  // top boundary face of the tile column
       double wb[4], fw[4];
 #pragma unroll
@@ -428,6 +473,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
 #pragma unroll
       for (int m = 0; m < 4; ++m) SY(m, line, ncy, hpos) = fw[m];
     }
+#endif
     __syncthreads();  // B3: interface fluxes visible; sU is free
 
     // -------------------------------------------------------------- P3: corrections
@@ -489,7 +535,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
 }
 
 // ------------------------------------------------------------------ ghost-layer helpers
-// rule FR_GHOST_PERIODIC / FR_GHOST_OUTFLOW over every ghost cell and every (k,l,m) plane.
+// rule FR_GHOST_PERIODIC / _OUTFLOW / _INFLOW_LEFT over every ghost cell and every (k,l,m) plane.
 __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, long long ldp, int planes, int rule) {
   const int nring = 2 * (nx + 2) + 2 * ny;
   const long long total = (long long)nring * planes;
@@ -516,8 +562,14 @@ __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, long long ldp,
     if (rule == FR_GHOST_PERIODIC) {
       si = (i == 0) ? nx : (i == nx + 1 ? 1 : i);
       sj = (j == 0) ? ny : (j == ny + 1 ? 1 : j);
-    } else {
+    } else if (rule == FR_GHOST_OUTFLOW) {
       si = (i == 0) ? 1 : (i == nx + 1 ? nx : i);
+      sj = (j == 0) ? 1 : (j == ny + 1 ? ny : j);
+    } else {
+      // bottom/top rows first (every i), then the right column (every j); the left column
+      // 1..ny is left alone, so each ghost reads a cell that this pass does not write
+      if (i == 0 && j >= 1 && j <= ny) continue;
+      si = (i == nx + 1) ? nx : i;
       sj = (j == 0) ? 1 : (j == ny + 1 ? ny : j);
     }
     u[(long long)i + nxg * j + ldp * pl] = u[(long long)si + nxg * sj + ldp * pl];
@@ -538,17 +590,9 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   geo.g = c.gamma;
   geo.gm1 = c.gamma - 1.0;
   geo.periodic = c.bc == FR_BC_PERIODIC;
-  // persistent grid: as many workgroups as fit at once, spread so that every workgroup has the
-  // same number of tiles, and a multiple of 8 so that each stays in one XCD's range
-  static const int per_cu = [] {
-    int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, fr2d_kernel<NSP, FLUX, EPI>, NSP * 64, 0) != hipSuccess)
-      n = 1;
-    return std::max(n, 1);
-  }();
+  // one tile per workgroup (two with FR2D_PAIR); a multiple of 8 workgroups keeps both tiles of a
+  // pair in one XCD's range of the tile order
   const long long nb = (long long)geo.ntx * geo.nty;
-  const long long slots = (long long)per_cu * std::max(c.num_cu, 1);
-  (void)slots;
   const long long tpw = FR2D_PAIR ? 2 : 1;
   long long grid = (nb + tpw - 1) / tpw;
   grid = std::min(nb, (grid + 7) / 8 * 8);

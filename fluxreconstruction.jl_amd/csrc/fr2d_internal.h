@@ -11,8 +11,7 @@ struct Fr2dCtx {
   HostOps ops;
   size_t len = 0;  // doubles per state array
   int device = 0;
-  int num_cu = 0;        // compute units of the device
-  int tiles_per_wg = 0;  // tiles per workgroup of the persistent 2D kernel (0 = from occupancy)
+  int num_cu = 0;  // compute units of the device
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   // device state and stage buffers of the integrator path

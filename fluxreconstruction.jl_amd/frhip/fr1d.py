@@ -33,6 +33,10 @@ _lib.SIGNATURES.update({
     "fr_ens_step": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int]),
     "fr_ens_step_graph": (C.c_int, [_vp, C.c_double, C.c_int, C.c_int]),
     "fr_ens_autotune": (C.c_int, [_vp, C.c_double, _ip]),
+    "fr1d_sensor_filter": (C.c_int, [_vp, C.c_double, _ip]),
+    "fr1d_positivity": (C.c_int, [_vp, _ip]),
+    "fr_ens_sensor_filter": (C.c_int, [_vp, C.c_double, _ip]),
+    "fr_ens_positivity": (C.c_int, [_vp, _ip]),
 })
 
 
@@ -40,7 +44,8 @@ def _bind():
     L = _lib.lib()
     for name in ("fr1d_create", "fr1d_destroy", "fr1d_state_len", "fr1d_rhs", "fr1d_rhs_dev", "fr1d_set_state",
                  "fr1d_get_state", "fr1d_step", "fr_ens_create", "fr_ens_destroy", "fr_ens_state_len", "fr_ens_offset",
-                 "fr_ens_set_state", "fr_ens_get_state", "fr_ens_step", "fr_ens_step_graph", "fr_ens_autotune"):
+                 "fr_ens_set_state", "fr_ens_get_state", "fr_ens_step", "fr_ens_step_graph", "fr_ens_autotune",
+                 "fr1d_sensor_filter", "fr1d_positivity", "fr_ens_sensor_filter", "fr_ens_positivity"):
         res, args = _lib.SIGNATURES[name]
         fn = getattr(L, name)
         fn.restype = res
@@ -103,6 +108,18 @@ class FR1D:
 
     def step(self, dt: float, nsteps: int = 1, scheme: str = "ssprk3"):
         check(_lib.lib().fr1d_step(self.h, _lib.RK[scheme], float(dt), int(nsteps)))
+
+    def sensor_filter(self, dt: float) -> int:
+        """example/euler_highlevel.jl:37-52 on the device state; returns the flagged-cell count."""
+        n = C.c_int(0)
+        check(_lib.lib().fr1d_sensor_filter(self.h, float(dt), C.byref(n)))
+        return n.value
+
+    def positivity(self) -> int:
+        """src/dissipation.jl:61-123 on every cell of the device state; returns the cells changed."""
+        n = C.c_int(0)
+        check(_lib.lib().fr1d_positivity(self.h, C.byref(n)))
+        return n.value
 
 
 def FREulerProblem(ps: FRPSpace1D, gamma: float, bc: str = "dirichlet", flux: str = "hll") -> FR1D:
@@ -168,6 +185,16 @@ class Ensemble1D:
 
     def step_graph(self, dt: float, nsteps: int, nrep: int = 1):
         check(_lib.lib().fr_ens_step_graph(self.h, float(dt), int(nsteps), int(nrep)))
+
+    def sensor_filter(self, dt: float) -> int:
+        n = C.c_int(0)
+        check(_lib.lib().fr_ens_sensor_filter(self.h, float(dt), C.byref(n)))
+        return n.value
+
+    def positivity(self) -> int:
+        n = C.c_int(0)
+        check(_lib.lib().fr_ens_positivity(self.h, C.byref(n)))
+        return n.value
 
     def autotune(self, dt: float) -> int:
         c = C.c_int(-1)

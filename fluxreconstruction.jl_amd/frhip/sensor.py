@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 from . import _lib, ics
-from ._lib import check, dptr
+from ._lib import FRError, check, dptr
 from .euler2d import FREuler2D
 from .space import FRPSpace2D
 
@@ -199,14 +199,17 @@ def shock_vortex_solver(nx, ny, deg=3, gamma=1.4, flux="hll", **ic):
 
 def run_shock_vortex(s: FREuler2D, dt, nsteps, mode="modal", lam=C3["lam"], kappa=C3["kappa"], callback=None,
                      every=0):
-    """Host loop of example/shock-vortex.jl:296-326 on the device state: SSP-RK3 step, sensor +
-    filter (mode "modal" / "learned" / "off"), outflow ghost refresh."""
+    """Host loop of example/shock-vortex.jl:296-326 on the device state: positivity limiter on
+    every cell (:298-303), SSP-RK3 step, sensor + filter (mode "modal" / "learned" / "off",
+    :306-321), ghost refresh of the bottom, top and right sides with the inflow column kept
+    (:322-325)."""
     S0 = -3.0 * np.log10(s.ps.deg)
     for n in range(nsteps):
+        s.positivity()
         s.step(dt, 1, "ssprk3")
         if mode != "off":
             _filter(s, mode, lam, S0, kappa)
-        s.fill_ghost("outflow")
+        s.fill_ghost("inflow_left")
         if callback is not None and every and (n + 1) % every == 0:
             callback(n + 1)
 
@@ -224,6 +227,7 @@ def collect_training_data(nruns=200, nx=128, ny=64, nsteps=240, every=40, seed=1
 
     rng = np.random.default_rng(seed)
     xs, ys = [], []
+    failed = 0
     for r in range(nruns):
         f = lambda: rng.uniform(0.8, 1.2)  # noqa: E731
         ic = dict(shock_mach=rng.uniform(1.05, 1.5), vortex_mach=C3["vortex_mach"] * f(),
@@ -232,6 +236,7 @@ def collect_training_data(nruns=200, nx=128, ny=64, nsteps=240, every=40, seed=1
         dt, _ = ics.cfl_steps(u0, C3["gamma"], ps.dx0, C3["cfl"], 1.0)
         gen = torch.Generator(device="cuda")
         gen.manual_seed(int(rng.integers(1 << 31)))
+        run_x, run_y = [], []
 
         def snap(_n):
             x = features(s)
@@ -241,13 +246,21 @@ def collect_training_data(nruns=200, nx=128, ny=64, nsteps=240, every=40, seed=1
             k = min(neg.numel(), max(int(max_neg_ratio * pos.numel()), 64))
             pick = neg[torch.randperm(neg.numel(), generator=gen, device="cuda")[:k]]
             idx = torch.cat([pos, pick])
-            xs.append(x[idx])
-            ys.append(y[idx])
+            run_x.append(x[idx])
+            run_y.append(y[idx])
 
-        snap(0)
-        run_shock_vortex(s, dt, nsteps, "modal", callback=snap, every=every)
+        try:
+            snap(0)
+            run_shock_vortex(s, dt, nsteps, "modal", callback=snap, every=every)
+            xs.extend(run_x)
+            ys.extend(run_y)
+        except FRError:
+            failed += 1  # a run whose limiter failed contributes no samples
         s.close()
-    return torch.cat(xs), torch.cat(ys)
+    x, y = torch.cat(xs), torch.cat(ys)
+    keep = torch.isfinite(x.float()).all(dim=1) & torch.isfinite(y)
+    collect_training_data.last_stats = dict(failed_runs=failed, dropped_samples=int((~keep).sum().item()))
+    return x[keep].contiguous(), y[keep].contiguous()
 
 
 def train_sensor(x, y, nsteps=3000, batch=4096, lr=2e-3, seed=0, val_frac=0.1):

@@ -37,6 +37,7 @@ def main():
           f"positive fraction {float(y.mean().item()):.3f}", flush=True)
     mlp, stats = S.train_sensor(x, y, nsteps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed)
     t2 = time.time()
+    stats.update(getattr(S.collect_training_data, "last_stats", {}))
     stats.update(dict(runs=args.runs, grid=[args.nx, args.ny], run_steps=args.run_steps, every=args.every,
                       steps=args.steps, batch=args.batch, lr=args.lr, seed=args.seed, samples=int(x.shape[0]),
                       data_s=t1 - t0, train_s=t2 - t1))

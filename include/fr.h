@@ -44,8 +44,9 @@ enum { FR_RK_SSPRK3 = 0, FR_RK_RK4 = 1, FR_RK_MIDPOINT = 2, FR_RK_EULER = 3 };
 /* ghost refresh rules for fr2d_fill_ghost (host loops of the reference scripts) */
 enum {
   FR_GHOST_PERIODIC = 0, /* copy the opposite interior row/column (euler2d_wave.jl:127-131, no sign flips) */
-  FR_GHOST_OUTFLOW = 1   /* copy the adjacent interior row/column on every side (shock-vortex.jl:324-326
-                            plus the left side) */
+  FR_GHOST_OUTFLOW = 1,  /* copy the adjacent interior row/column on every side */
+  FR_GHOST_INFLOW_LEFT = 2 /* shock-vortex.jl:322-325: bottom, top and right copy the adjacent row/column,
+                              the left (supersonic inflow) column keeps its values */
 };
 /* sensor modes for fr2d_sensor_filter */
 enum { FR_SENSOR_OFF = 0, FR_SENSOR_MODAL = 1, FR_SENSOR_LEARNED = 2 };
@@ -193,6 +194,15 @@ int fr1d_rhs_dev(fr1d_ctx *ctx, const double *d_u, double *d_du, double t);
 int fr1d_set_state(fr1d_ctx *ctx, const double *u);
 int fr1d_get_state(fr1d_ctx *ctx, double *u);
 int fr1d_step(fr1d_ctx *ctx, int scheme, double dt, int nsteps);
+/* The per-step host loop of example/euler_highlevel.jl:37-52 on the device state (Euler only):
+ * modal sensor on rho (su = u_hat[end]^2 / (sum u_hat^2 + 1e-6), shock_detector of
+ * src/dissipation.jl:13 with S0 = -3 log10(deg), kappa = 4) and, on a shock, modal_filter!
+ * (:l2) of every variable with lambda = dt exp(0.875 deg) 0.15.  n_flagged (optional). */
+int fr1d_sensor_filter(fr1d_ctx *ctx, double dt, int *n_flagged);
+/* positive_limiter(u::AbstractMatrix, ...) of src/dissipation.jl:61-123 on every cell (Gauss
+ * weights / 2).  Returns FR_E_LIMITER if a cell has a non-positive mean density (left as is);
+ * n_limited (optional) counts the cells changed. */
+int fr1d_positivity(fr1d_ctx *ctx, int *n_limited);
 
 /* ------------------------------------------------------------------ batched 1D Euler ensembles */
 typedef struct fr_ens_ctx fr_ens_ctx;
@@ -211,6 +221,9 @@ int fr_ens_step(fr_ens_ctx *ctx, int mode, double dt, int nsteps);
 int fr_ens_step_graph(fr_ens_ctx *ctx, double dt, int nsteps, int nrep);
 /* time candidate work mappings on the device and keep the fastest; returns its index */
 int fr_ens_autotune(fr_ens_ctx *ctx, double dt, int *chosen);
+/* fr1d_sensor_filter / fr1d_positivity over every member (each with its own degree) */
+int fr_ens_sensor_filter(fr_ens_ctx *ctx, double dt, int *n_flagged);
+int fr_ens_positivity(fr_ens_ctx *ctx, int *n_limited);
 
 /* ------------------------------------------------------------------ 2D discrete-velocity BGK
  * example/bgk_wave.jl:69-129 tensorised to periodic uniform quads (Radau FR, upwind interface flux

@@ -1047,6 +1047,46 @@ int or_positive_limiter1d_euler(double *ue, int nsp, double g, const double *wei
   return 0;
 }
 
+/* 1D modal sensor + filter of example/euler_highlevel.jl:37-52 on u(i, p, k): per cell
+ * u_hat = iV rho, su = u_hat[end]^2 / (sum(u_hat^2) + 1e-6), shock_detector(log10(su), deg) with
+ * S0 = -3 log10(deg), kappa = 4; on a shock every variable is filtered in modal space with
+ * lambda = dt exp(0.875 deg) 0.15 (modal_filter!, :l2).  Returns the number of flagged cells. */
+int or_fr1d_sensor_filter(int ncell, int nsp, const double *V, const double *iV, double dt, double *u) {
+  const int deg = nsp - 1;
+  const double S0 = -3.0 * log10((double)deg), kappa = 4.0;
+  const double lambda = dt * exp(0.875 * deg) * 0.15;
+  int count = 0;
+  for (int i = 0; i < ncell; ++i) {
+    double mod[8], nod[8];
+    double num = 0.0, den = 0.0;
+    for (int a = 0; a < nsp; ++a) {
+      double s = 0.0;
+      for (int b = 0; b < nsp; ++b) s += iV[a + nsp * b] * u[i + (size_t)ncell * b];
+      mod[a] = s;
+      den += s * s;
+    }
+    num = mod[nsp - 1] * mod[nsp - 1];
+    const double su = num / (den + 1e-6);
+    if (!or_shock_detector(log10(su), deg, S0, kappa)) continue;
+    ++count;
+    for (int k = 0; k < 3; ++k) {
+      for (int a = 0; a < nsp; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < nsp; ++b) s += iV[a + nsp * b] * u[i + (size_t)ncell * (b + nsp * k)];
+        mod[a] = s;
+      }
+      or_modal_filter_l2(mod, nsp, lambda);
+      for (int a = 0; a < nsp; ++a) {
+        double s = 0.0;
+        for (int b = 0; b < nsp; ++b) s += V[a + nsp * b] * mod[b];
+        nod[a] = s;
+      }
+      for (int a = 0; a < nsp; ++a) u[i + (size_t)ncell * (a + nsp * k)] = nod[a];
+    }
+  }
+  return count;
+}
+
 /* The limiter over every cell of a 1D Euler state u(i, p, k); returns the failed count. */
 int or_fr1d_positivity(int ncell, int nsp, double g, const double *weights, const double *ll, const double *lr,
                        double *u) {

@@ -97,6 +97,7 @@ int or_positive_limiter2d(double *ue, int nsp, double g, const double *weights, 
 int or_fr2d_positivity(const or_fr2d *d, const double *wp4, double *u);
 int or_positive_limiter1d_euler(double *ue, int nsp, double g, const double *weights, const double *ll,
                                 const double *lr);
+int or_fr1d_sensor_filter(int ncell, int nsp, const double *V, const double *iV, double dt, double *u);
 int or_fr1d_positivity(int ncell, int nsp, double g, const double *weights, const double *ll, const double *lr,
                        double *u);
 int or_positive_limiter1d_scalar(double *u, int n, const double *weights, const double *ll,

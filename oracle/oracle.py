@@ -321,6 +321,13 @@ def fr1d_positivity(u, g, weights, ll, lr):
                                     _p(_f64(ll)), _p(_f64(lr)), _p(u))
 
 
+def fr1d_sensor_filter(u, ops: "Ops", dt):
+    """example/euler_highlevel.jl:37-52 on u(i, p, k) in place; returns the flagged-cell count."""
+    assert u.flags.f_contiguous and u.dtype == np.float64
+    return lib().or_fr1d_sensor_filter(C.c_int(u.shape[0]), C.c_int(u.shape[1]), _p(_f64(ops.V1)), _p(_f64(ops.iV1)),
+                                       C.c_double(dt), _p(u))
+
+
 def shock_detector(Se, deg, S0=None, kappa=4.0) -> bool:
     if S0 is None:
         S0 = -3.0 * np.log10(deg)

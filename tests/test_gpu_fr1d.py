@@ -174,3 +174,74 @@ def test_ensemble_graph_and_autotune_keep_results():
     assert np.array_equal(e.get_state(), flat)  # autotune restores the state
     e.step(dt, 6)
     assert np.array_equal(e.get_state(), ref)
+
+
+def sod_state(ncell, deg, gamma=1.4):
+    """Sod data sampled at the solution points (the jump falls inside a cell for odd ncell)."""
+    r, _ = np.polynomial.legendre.leggauss(deg + 1)
+    x = (np.arange(ncell)[:, None] + 0.5 + 0.5 * r[None, :]) / ncell
+    left = x <= 0.5
+    prim = np.stack([np.where(left, 1.0, 0.125), np.zeros_like(x),
+                     np.where(left, 1.0, 0.125) / (2.0 * np.where(left, 1.0, 0.1))], -1)
+    return np.asfortranarray(ics.prim_conserve1(prim, gamma))
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+def test_sensor_filter_1d_parity(deg):
+    """example/euler_highlevel.jl:37-52 on the device state vs the oracle: same flagged cells,
+    states to 1e-13 (rel inf)."""
+    ncell = 301
+    ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
+    u = sod_state(ncell, deg)
+    dt = 0.05 / ncell
+    g = frhip.FREulerProblem(ps, 1.4, "dirichlet", "rusanov")
+    g.set_state(u)
+    n = g.sensor_filter(dt)
+    ref = u.copy(order="F")
+    nref = O.fr1d_sensor_filter(ref, oracle_ops(deg), dt)
+    assert n == nref and n >= 1
+    assert rel_inf(g.get_state(), ref) <= 1e-13
+
+
+@pytest.mark.parametrize("deg", [2, 3, 4])
+def test_positivity_1d_parity(deg):
+    """src/dissipation.jl:61-123 on every cell vs the oracle, on a state with a negative density
+    and a negative pressure planted in two cells; every cell ends with rho, p >= 0."""
+    ncell = 97
+    ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
+    u = euler_state(ncell, deg, seed=5)
+    u[10, 1, 0] = -0.05
+    u[40, 0, 2] *= 0.2
+    g = frhip.FREulerProblem(ps, 1.4, "dirichlet", "rusanov")
+    g.set_state(u)
+    n = g.positivity()
+    ops = oracle_ops(deg)
+    ref = u.copy(order="F")
+    bad = O.fr1d_positivity(ref, 1.4, ops.w / 2.0, ops.ll, ops.lr)
+    assert bad == 0 and n >= 2
+    out = g.get_state()
+    assert rel_inf(out, ref) <= 1e-13
+    p = 0.4 * (out[..., 2] - 0.5 * out[..., 1] ** 2 / out[..., 0])
+    assert out[..., 0].min() > 0.0 and p.min() > -1e-12
+
+
+def test_ensemble_dissipation_matches_single():
+    """fr_ens_sensor_filter / fr_ens_positivity over mixed-degree members equal the single-problem
+    calls on each member bitwise."""
+    ncell, deg, dx, members = ics.riemann_ensemble(12, seed=4, ncell_range=(64, 400))
+    e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
+    flat = e.pack(members)
+    dt = 0.05 * float(np.min(dx)) / 2.5
+    for what in ("filter", "positivity"):
+        e.set_state(flat)
+        n = e.sensor_filter(dt) if what == "filter" else e.positivity()
+        out = e.get_state()
+        tot = 0
+        for b in range(len(members)):
+            ps = frhip.FRPSpace1D(0.0, 1.0, int(ncell[b]), int(deg[b]))
+            g = frhip.FREulerProblem(ps, 1.4, "dirichlet", "rusanov")
+            g.set_state(members[b])
+            tot += g.sensor_filter(dt) if what == "filter" else g.positivity()
+            assert np.array_equal(g.get_state(), e.member(out, b)), (what, b)
+            g.close()
+        assert n == tot

@@ -171,3 +171,38 @@ def test_check_state_flags_negative_and_nan():
     with pytest.raises(frhip.FRError) as e:
         s.check_state()
     assert e.value.code == 3
+
+
+def _ghost_rule_numpy(u, rule):
+    """The ghost refresh rules written out with numpy slices (host loops of the reference scripts)."""
+    v = u.copy(order="F")
+    nx, ny = v.shape[0] - 2, v.shape[1] - 2
+    if rule == "periodic":  # example/euler2d_wave.jl:125-135
+        v[0, 1:ny + 1] = v[nx, 1:ny + 1]
+        v[nx + 1, 1:ny + 1] = v[1, 1:ny + 1]
+        v[:, 0] = v[:, ny]
+        v[:, ny + 1] = v[:, 1]
+        v[0, 0], v[0, ny + 1], v[nx + 1, 0], v[nx + 1, ny + 1] = v[nx, ny], v[nx, 1], v[1, ny], v[1, 1]
+    elif rule == "outflow":
+        v[0, 1:ny + 1] = v[1, 1:ny + 1]
+        v[nx + 1, 1:ny + 1] = v[nx, 1:ny + 1]
+        v[:, 0] = v[:, 1]
+        v[:, ny + 1] = v[:, ny]
+        v[0, 0], v[0, ny + 1], v[nx + 1, 0], v[nx + 1, ny + 1] = v[1, 1], v[1, ny], v[nx, 1], v[nx, ny]
+    else:  # example/shock-vortex.jl:322-325, in its order
+        v[:, 0] = v[:, 1]
+        v[:, ny + 1] = v[:, ny]
+        v[nx + 1, :] = v[nx, :]
+    return v
+
+
+@pytest.mark.parametrize("rule", ["periodic", "outflow", "inflow_left"])
+def test_fill_ghost_rules(rule):
+    nx, ny, deg = 13, 9, 3
+    ps = frhip.FRPSpace2D(0.0, 1.0, nx, 0.0, 1.0, ny, deg, 1, 1)
+    rng = np.random.default_rng(3)
+    u = np.asfortranarray(rng.random((nx + 2, ny + 2, deg + 1, deg + 1, 4)) + 1.0)
+    s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
+    s.set_state(u)
+    s.fill_ghost(rule)
+    assert np.array_equal(s.get_state(), _ghost_rule_numpy(u, rule))
