@@ -188,6 +188,7 @@ int fr2d_destroy(fr2d_ctx *c) {
   // d_mlp refers to the caller's network (fr2d_set_sensor_mlp), which the context does not own
   if (c->d_feat) (void)hipFree(c->d_feat);
   if (c->d_strength) (void)hipFree(c->d_strength);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
   return FR_OK;
@@ -283,36 +284,36 @@ int fr2d_step_graph(fr2d_ctx *c, int scheme, double dt, int nsteps, int nrep) {
   hipStream_t st = stream_of(*c);
   int rc = prepare_step(*c, scheme, st);
   if (rc) return rc;
-  // capture on a private stream so a caller stream in relaxed use is never put in capture mode
-  hipStream_t cap = c->own_stream;
-  if (cap != st) {
-    hipEvent_t ev;
-    FR_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-    FR_HIP(hipEventRecord(ev, st));
-    FR_HIP(hipStreamWaitEvent(cap, ev, 0));
-    FR_HIP(hipEventDestroy(ev));
-  }
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  FR_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-  rc = enqueue_steps(*c, scheme, dt, nsteps, cap);
-  hipError_t e = hipStreamEndCapture(cap, &graph);
-  if (rc) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rc;
-  }
-  FR_HIP(e);
-  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
+  // The graph holds the context's own buffers (state and stage arrays never move once allocated),
+  // so a capture stays valid while scheme, dt and nsteps are unchanged.
+  if (!(c->gexec && c->g_scheme == scheme && c->g_dt == dt && c->g_nsteps == nsteps)) {
+    if (c->gexec) {
+      (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+    // capture on the private stream so a caller stream in relaxed use is never put in capture
+    // mode; nothing runs during a capture, and the replays below are ordered on st
+    hipStream_t cap = c->own_stream;
+    hipGraph_t graph = nullptr;
+    FR_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_steps(*c, scheme, dt, nsteps, cap);
+    hipError_t e = hipStreamEndCapture(cap, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    FR_HIP(e);
+    e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    if (e != hipSuccess) {
+      c->gexec = nullptr;
+      return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    c->g_scheme = scheme;
+    c->g_dt = dt;
+    c->g_nsteps = nsteps;
   }
-  for (int r = 0; r < nrep && e == hipSuccess; ++r) e = hipGraphLaunch(exec, cap);
-  hipError_t e2 = hipStreamSynchronize(cap);
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
-  FR_HIP(e);
-  FR_HIP(e2);
+  for (int r = 0; r < nrep; ++r) FR_HIP(hipGraphLaunch(c->gexec, st));
   return finish_step(*c, st);
 }
 

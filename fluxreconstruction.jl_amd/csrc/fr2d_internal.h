@@ -27,6 +27,10 @@ struct Fr2dCtx {
   bool mlp_loaded = false;
   // device copies of V, iV, wp/4, ll, lr used by the dissipation kernels
   double *d_dops = nullptr;
+  // the last step sequence captured by fr2d_step_graph, replayed while its parameters match
+  hipGraphExec_t gexec = nullptr;
+  int g_scheme = -1, g_nsteps = 0;
+  double g_dt = 0.0;
 };
 
 int fr2d_launch_residual(const Fr2dCtx &c, const double *d_u, double *d_du, hipStream_t st);

@@ -76,6 +76,27 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, unsigned byte_o
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(fr_u2, v), r, byte_off, 0, 0);
 }
 
+// Value of v held by lane src of the same wavefront (ds_bpermute: a crossbar read, no LDS
+// storage).  Every lane of the wavefront must execute the call with the same control flow; src is
+// taken modulo 64.
+__device__ __forceinline__ double lane_fetch(double v, int src) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int a = src << 2;
+  const int lo = __builtin_amdgcn_ds_bpermute(a, (int)(b & 0xffffffffLL));
+  const int hi = __builtin_amdgcn_ds_bpermute(a, (int)(b >> 32));
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Orders LDS traffic between the lanes of one wavefront: LDS operations are not moved across it
+// by the compiler and the wavefront's own outstanding LDS operations complete before it (the
+// LDS executes one wavefront's operations in order), with no wait for the other wavefronts.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0) (vmcnt, expcnt untouched)
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ------------------------------------------------------------------ FP64 reciprocal / square root
 // With FR_FAST_RCP the hardware estimates v_rcp_f64 / v_rsq_f64 are refined by two Newton-Raphson
 // (reciprocal) and one Goldschmidt (square root) iteration in FMA arithmetic, which leaves about

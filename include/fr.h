@@ -123,7 +123,9 @@ double *fr2d_state_dev(fr2d_ctx *ctx);               /* device pointer of the st
 /* nsteps explicit steps of size dt with scheme FR_RK_*; each stage is one fused residual+update
  * launch.  In FR_BC_GHOST mode the ghost layer is held fixed (L = 0 there). */
 int fr2d_step(fr2d_ctx *ctx, int scheme, double dt, int nsteps);
-/* capture the launches of nsteps steps into a hipGraph and replay it nrep times */
+/* the launches of nsteps steps captured into a hipGraph and replayed nrep times on the context's
+ * stream (asynchronous, like fr2d_step); the context keeps the instantiated graph and reuses it
+ * while scheme, dt and nsteps are unchanged */
 int fr2d_step_graph(fr2d_ctx *ctx, int scheme, double dt, int nsteps, int nrep);
 /* refresh the ghost layer of the device state (FR_GHOST_*) */
 int fr2d_fill_ghost(fr2d_ctx *ctx, int rule);

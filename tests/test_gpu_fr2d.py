@@ -111,17 +111,22 @@ def test_stepping_parity(scheme, bc):
 
 
 def test_graph_replay_matches_eager():
+    """The context keeps its captured step graph: every replay (with the host reading the state,
+    eager steps on another context and a second replay count in between) equals the eager
+    launches bitwise; a change of dt recaptures."""
     ps, u, s, orc = setup(32, 16, 3, "rusanov", "periodic", amp=0.0)
     dt, _ = ics.cfl_steps(u, 1.4, ps.dx0, 0.05, 1.0)
     s.set_state(u)
-    s.step(dt, 6)
-    eager = s.get_state()
     s2 = frhip.FREuler2D(ps, 1.4, "rusanov", "periodic")
     s2.set_state(u)
-    for _ in range(3):  # three replays of a 2-step graph, with host synchronisations between
-        s2.step_graph(dt, 2, 1)
-    graph = s2.get_state()
-    assert np.array_equal(eager, graph)
+    for rep in (1, 1, 2, 1):
+        s.step(dt, 2 * rep)
+        eager = s.get_state()
+        s2.step_graph(dt, 2, rep)
+        assert np.array_equal(eager, s2.get_state())
+    s.step(0.5 * dt, 2)
+    s2.step_graph(0.5 * dt, 2, 1)
+    assert np.array_equal(s.get_state(), s2.get_state())
 
 
 def test_sensor_filter_parity():
