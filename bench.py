@@ -34,6 +34,22 @@ CFL = 0.05
 GRAPH_CHUNK = 100  # SSP-RK3 steps per captured graph
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "fr2d_stage_256x256_p3_hbm_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per stage launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (calibrated on a plain device copy), or None when the summary is absent."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(u0, dt, budget_s=12.0):
     """Oracle SSP-RK3 steps on the host cores for about budget_s seconds."""
     import numpy as np
@@ -262,7 +278,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(),
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT),
             "kernel": "fr2d_kernel<4,RUSANOV,STAGE>",
             "avg_launch_us": per_launch_s * 1e6,
             "alg_bytes_per_launch": alg_bytes,

@@ -98,12 +98,12 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // ------------------------------------------------------------------ FP64 reciprocal / square root
-// With FR_FAST_RCP the hardware estimates v_rcp_f64 / v_rsq_f64 are refined by two Newton-Raphson
-// (reciprocal) and one Goldschmidt (square root) iteration in FMA arithmetic, which leaves about
-// one ulp of error instead of the correctly rounded division / sqrt sequences; otherwise the
-// IEEE operations are used.
+// The hardware estimates v_rcp_f64 / v_rsq_f64 refined by two Newton-Raphson (reciprocal) and one
+// Goldschmidt + Newton (square root) iterations in FMA arithmetic: about one ulp from the
+// correctly rounded result at roughly half the instructions of the IEEE division / sqrt
+// sequences.  FR_IEEE_DIV selects the correctly rounded operations instead.
 __device__ __forceinline__ double fr_rcp(double x) {
-#ifdef FR_FAST_RCP
+#ifndef FR_IEEE_DIV
   double r = __builtin_amdgcn_rcp(x);
   double e = fma(-x, r, 1.0);
   r = fma(r, e, r);
@@ -115,7 +115,7 @@ __device__ __forceinline__ double fr_rcp(double x) {
 }
 
 __device__ __forceinline__ double fr_sqrt(double x) {
-#ifdef FR_FAST_RCP
+#ifndef FR_IEEE_DIV
   const double y = __builtin_amdgcn_rsq(x);
   double s = x * y, h = 0.5 * y;
   const double r = fma(-s, h, 0.5);
