@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <string>
 
 #include "fr2d_internal.h"
 
@@ -343,6 +344,28 @@ int fr2d_sensor_filter(fr2d_ctx *c, int mode, double lambda, double S0, double k
     FR_HIP(hipMemcpyAsync(n_flagged, c->d_count, sizeof(int), hipMemcpyDeviceToHost, st));
     FR_HIP(hipStreamSynchronize(st));
   }
+  return FR_OK;
+}
+
+int fr2d_filter_refresh_limit(fr2d_ctx *c, int mode, double lambda, double S0, double kappa, int rule,
+                              int *n_filtered) {
+  FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_filter_refresh_limit: no state");
+  FR_REQUIRE(mode >= FR_SENSOR_OFF && mode <= FR_SENSOR_LEARNED, FR_E_INVALID, "fr2d_filter_refresh_limit: unknown mode");
+  FR_REQUIRE(lambda >= 0.0 && kappa > 0.0, FR_E_INVALID, "fr2d_filter_refresh_limit: lambda >= 0 and kappa > 0 required");
+  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_INFLOW_LEFT, FR_E_INVALID, "fr2d_filter_refresh_limit: unknown rule");
+  hipStream_t st = stream_of(*c);
+  if (mode == FR_SENSOR_LEARNED && !c->mlp_loaded) mode = FR_SENSOR_MODAL;  // plain sensor is the fallback
+  FR_HIP(hipMemsetAsync(c->d_count, 0, 2 * sizeof(int), st));
+  int rc = FR_OK;
+  if (mode == FR_SENSOR_LEARNED && (rc = fr2d_launch_mlp_sensor(*c, c->d_u, st))) return rc;
+  if ((rc = fr2d_launch_filter_refresh_limit(*c, c->d_u, mode, lambda, S0, kappa, rule, c->d_count, st))) return rc;
+  int h[2] = {0, 0};
+  FR_HIP(hipMemcpyAsync(h, c->d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  FR_HIP(hipStreamSynchronize(st));
+  if (n_filtered) *n_filtered = h[0];
+  if (h[1] > 0)
+    return fail(FR_E_LIMITER, "fr2d_filter_refresh_limit: " + std::to_string(h[1]) +
+                                  " element(s) with non-positive mean density (left as filtered)");
   return FR_OK;
 }
 

@@ -174,6 +174,214 @@ __global__ __launch_bounds__(256) void fr2d_positivity_kernel(double *u, int nx,
   }
 }
 
+// Source cell of ghost cell (gi, gj) under a ghost refresh rule (as fr2d_fill_ghost_kernel);
+// keep = the rule leaves the cell as it is (the inflow column of FR_GHOST_INFLOW_LEFT).
+__device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int rule, int &si, int &sj, bool &keep) {
+  keep = false;
+  if (rule == FR_GHOST_PERIODIC) {
+    si = (gi == 0) ? nx : (gi == nx + 1 ? 1 : gi);
+    sj = (gj == 0) ? ny : (gj == ny + 1 ? 1 : gj);
+  } else if (rule == FR_GHOST_OUTFLOW) {
+    si = (gi == 0) ? 1 : (gi == nx + 1 ? nx : gi);
+    sj = (gj == 0) ? 1 : (gj == ny + 1 ? ny : gj);
+  } else {
+    keep = gi == 0 && gj >= 1 && gj <= ny;
+    si = (gi == nx + 1) ? nx : gi;
+    sj = (gj == 0) ? 1 : (gj == ny + 1 ? ny : gj);
+  }
+}
+
+// The end of one iteration of the shock-vortex host loop and the start of the next in one pass
+// (example/shock-vortex.jl:306-325, then :298-303): sensor + filter of every cell, the ghost
+// refresh `rule` written from the filtered values, and the positivity limiter of every interior
+// cell.  Each cell is read once; a ghost cell that the rule overwrites is written only by the
+// thread of its source cell, so no cell has two writers.  count[0]: filtered cells whose values
+// are kept (interior and kept ghosts), count[1]: cells whose density assert failed (left as
+// filtered).  The arithmetic is that of fr2d_sensor_filter_kernel and fr2d_positivity_kernel.
+template <int NSP>
+__global__ __launch_bounds__(256) void fr2d_filter_refresh_limit_kernel(
+    double *u, int nx, int ny, long long ldp, const double *__restrict__ V, const double *__restrict__ iV,
+    const double *__restrict__ wts, const double *__restrict__ ll, const double *__restrict__ lr, int mode,
+    double lambda, double S0, double kappa, const float *__restrict__ strength, int rule, double g, int *count) {
+  constexpr int NP = NSP * NSP;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (e >= ldp) return;
+  const int nxg = nx + 2;
+  const int i = (int)(e % nxg), j = (int)(e / nxg);
+  const bool ghost = i == 0 || i == nx + 1 || j == 0 || j == ny + 1;
+  if (ghost) {
+    int si, sj;
+    bool keep;
+    ghost_source(i, j, nx, ny, rule, si, sj, keep);
+    if (!keep) return;  // refreshed by the thread of its source cell
+  }
+  double x[NP][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m)
+#pragma unroll
+    for (int q = 0; q < NP; ++q) x[q][m] = u[e + ldp * (q + NP * m)];
+
+  // sensor + filter
+  bool changed = false;
+  if (mode != FR_SENSOR_OFF) {
+    double mod[NP];
+#pragma unroll
+    for (int a = 0; a < NP; ++a) {
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < NP; ++b) s += iV[a + NP * b] * x[b][0];
+      mod[a] = s;
+    }
+    double lam = 0.0;
+    if (mode == FR_SENSOR_MODAL) {
+      double ss = 0.0;
+#pragma unroll
+      for (int a = 0; a < NP; ++a) ss += mod[a] * mod[a];
+      const double su = mod[NP - 1] * mod[NP - 1] / ss;
+      lam = shock_switch(log10(su), S0, kappa) ? lambda : 0.0;
+    } else {
+      lam = lambda * (double)strength[e];
+    }
+    if (lam > 0.0) {
+      changed = true;
+      atomicAdd(count, 1);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        if (m > 0) {
+#pragma unroll
+          for (int a = 0; a < NP; ++a) {
+            double s = 0.0;
+#pragma unroll
+            for (int b = 0; b < NP; ++b) s += iV[a + NP * b] * x[b][m];
+            mod[a] = s;
+          }
+        }
+#pragma unroll
+        for (int a = 1; a < NP; ++a) mod[a] /= (1.0 + lam * (double)((a + 1) * (a + 1)) * (double)(a * a));
+#pragma unroll
+        for (int q = 0; q < NP; ++q) {
+          double s = 0.0;
+#pragma unroll
+          for (int b = 0; b < NP; ++b) s += V[q + NP * b] * mod[b];
+          x[q][m] = s;
+        }
+      }
+    }
+  }
+
+  // ghost refresh: copy the filtered cell into every ghost cell whose source it is
+  if (!(i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1)) {
+    const int ci[5] = {i - 1, i, i + 1, 0, nx + 1}, cj[5] = {j - 1, j, j + 1, 0, ny + 1};
+    for (int a = 0; a < 5; ++a) {
+      const int gi = ci[a];
+      if (gi < 0 || gi > nx + 1) continue;
+      bool dup = false;
+      for (int b = 0; b < a; ++b) dup = dup || ci[b] == gi;
+      if (dup) continue;
+      for (int c = 0; c < 5; ++c) {
+        const int gj = cj[c];
+        if (gj < 0 || gj > ny + 1) continue;
+        bool dupj = false;
+        for (int d = 0; d < c; ++d) dupj = dupj || cj[d] == gj;
+        if (dupj || (gi == i && gj == j)) continue;
+        if (!(gi == 0 || gi == nx + 1 || gj == 0 || gj == ny + 1)) continue;
+        int si, sj;
+        bool keep;
+        ghost_source(gi, gj, nx, ny, rule, si, sj, keep);
+        if (keep || si != i || sj != j) continue;
+        const long long ge = (long long)gi + (long long)nxg * gj;
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int q = 0; q < NP; ++q) u[ge + ldp * (q + NP * m)] = x[q][m];
+      }
+    }
+  }
+
+  // positivity limiter of src/dissipation.jl:125-206 on the interior cells (weights wp/4)
+  if (!ghost) {
+    double um[4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+      double s = 0.0;
+#pragma unroll
+      for (int l = 0; l < NSP; ++l)
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) s += x[k + NSP * l][m] * wts[k + NSP * l];
+      um[m] = s;
+    }
+    const double p_mean = 0.5 * um[0] * (1.0 / prim_lambda(um, g));
+    const double eps = fmin(1e-13, fmin(um[0], p_mean));
+    double rmin = INFINITY;
+#pragma unroll
+    for (int jj = 0; jj < NSP; ++jj)
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        double s = 0.0;
+#pragma unroll
+        for (int q = 0; q < NSP; ++q) {
+          const double w = (f == 0 || f == 3) ? ll[q] : lr[q];
+          s += ((f == 0 || f == 2) ? x[jj + NSP * q][0] : x[q + NSP * jj][0]) * w;
+        }
+        rmin = fmin(rmin, s);
+      }
+#pragma unroll
+    for (int q = 0; q < NP; ++q) rmin = fmin(rmin, x[q][0]);
+    const double t1 = fmin((um[0] - eps) / (um[0] - rmin + 1e-8), 1.0);
+    if (!(t1 > 0.0 && t1 <= 1.0)) {
+      atomicAdd(count + 1, 1);
+    } else {
+      double tmin = INFINITY;
+      bool have = false;
+#pragma unroll 1
+      for (int jj = 0; jj < NSP; ++jj)
+#pragma unroll 1
+        for (int f = 0; f < 4; ++f) {
+          double w[4];
+#pragma unroll
+          for (int m = 0; m < 4; ++m) {
+            double s = 0.0;
+#pragma unroll
+            for (int q = 0; q < NSP; ++q) {
+              const double wq = (f == 0 || f == 3) ? ll[q] : lr[q];
+              s += ((f == 0 || f == 2) ? x[jj + NSP * q][m] : x[q + NSP * jj][m]) * wq;
+            }
+            w[m] = s;
+          }
+          if (prim_lambda(w, g) < eps) {
+            have = true;
+            tmin = fmin(tmin, tj_root(w, um, g, eps));
+          }
+        }
+      if (t1 < 1.0) changed = true;
+#pragma unroll
+      for (int q = 0; q < NP; ++q) {
+        const double r = t1 * (x[q][0] - um[0]) + um[0];
+        x[q][0] = r;
+        double w[4] = {r, x[q][1], x[q][2], x[q][3]};
+        if (prim_lambda(w, g) < eps) {
+          have = true;
+          tmin = fmin(tmin, tj_root(w, um, g, eps));
+        }
+      }
+      if (have) {
+        changed = true;
+        const double t2 = fmin(tmin, 1.0);
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int q = 0; q < NP; ++q) x[q][m] = t2 * (x[q][m] - um[m]) + um[m];
+      }
+    }
+  }
+  if (changed) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int q = 0; q < NP; ++q) u[e + ldp * (q + NP * m)] = x[q][m];
+  }
+}
+
 // counts non-finite values (count[0]) and non-positive density / pressure points (count[1])
 template <int NSP>
 __global__ __launch_bounds__(256) void fr2d_check_kernel(const double *u, int nx, int ny, long long ldp, double g,
@@ -278,6 +486,29 @@ int fr2d_launch_positivity(const Fr2dCtx &c, double *d_u, int *d_count, hipStrea
     case 4: return positivity_nsp<4>(c, d_u, d_count, st);
     case 5: return positivity_nsp<5>(c, d_u, d_count, st);
     default: return fail(FR_E_UNSUPPORTED, "positivity: degree out of range");
+  }
+}
+
+template <int NSP>
+static int frl_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int rule,
+                   int *cnt, hipStream_t st) {
+  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
+  DissOps o = fr2d_diss_ops(c);
+  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((ldp + 255) / 256)), dim3(256), 0, st,
+                     d_u, c.nx, c.ny, ldp, o.V, o.iV, o.wts, o.ll, o.lr, mode, lambda, S0, kappa,
+                     (const float *)c.d_strength, rule, c.gamma, cnt);
+  FR_LAUNCHED("fr2d_filter_refresh_limit_kernel");
+  return FR_OK;
+}
+
+int fr2d_launch_filter_refresh_limit(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0,
+                                     double kappa, int rule, int *d_count, hipStream_t st) {
+  switch (c.nsp) {
+    case 2: return frl_nsp<2>(c, d_u, mode, lambda, S0, kappa, rule, d_count, st);
+    case 3: return frl_nsp<3>(c, d_u, mode, lambda, S0, kappa, rule, d_count, st);
+    case 4: return frl_nsp<4>(c, d_u, mode, lambda, S0, kappa, rule, d_count, st);
+    case 5: return frl_nsp<5>(c, d_u, mode, lambda, S0, kappa, rule, d_count, st);
+    default: return fail(FR_E_UNSUPPORTED, "filter_refresh_limit: degree out of range");
   }
 }
 

@@ -43,6 +43,10 @@ int fr2d_launch_copy_ghost(const Fr2dCtx &c, const double *src, double *dst, hip
 int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
                               int *d_count, float *eval_out, hipStream_t st);
 int fr2d_launch_positivity(const Fr2dCtx &c, double *d_u, int *d_count, hipStream_t st);
+// sensor + filter of every cell, ghost refresh `rule` from the filtered values, positivity of the
+// interior cells, in one pass (d_count[0] filtered, d_count[1] limiter failures)
+int fr2d_launch_filter_refresh_limit(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0,
+                                     double kappa, int rule, int *d_count, hipStream_t st);
 int fr2d_launch_check(const Fr2dCtx &c, const double *d_u, int *d_count, hipStream_t st);
 // learned sensor: per-cell strengths from the MLP into c.d_strength (fr_mlp.hip)
 int fr2d_launch_mlp_sensor(const Fr2dCtx &c, const double *d_u, hipStream_t st);

@@ -110,6 +110,7 @@ SIGNATURES = {
     "fr2d_fill_ghost": (C.c_int, [_vp, C.c_int]),
     "fr2d_sensor_filter": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double, _ip]),
     "fr2d_positivity": (C.c_int, [_vp]),
+    "fr2d_filter_refresh_limit": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _ip]),
     "fr2d_check_state": (C.c_int, [_vp]),
 }
 

@@ -145,6 +145,17 @@ class FREuler2D:
         check(_lib.lib().fr2d_sensor_filter(self.h, _lib.SENSOR[mode], float(lam), float(S0), float(kappa), C.byref(n)))
         return n.value
 
+    def filter_refresh_limit(self, mode: str = "modal", lam: float = 5e-4, S0: float | None = None,
+                             kappa: float = 9.0, rule: str = "inflow_left") -> int:
+        """Sensor + filter, ghost refresh `rule` and positivity limiter in one pass (the end of one
+        shock-vortex.jl:296-326 iteration and the start of the next); returns #filtered cells."""
+        if S0 is None:
+            S0 = -3.0 * np.log10(self.ps.deg)
+        n = C.c_int(0)
+        check(_lib.lib().fr2d_filter_refresh_limit(self.h, _lib.SENSOR[mode], float(lam), float(S0), float(kappa),
+                                                   _lib.GHOST[rule], C.byref(n)))
+        return n.value
+
     def positivity(self):
         """Positivity limiter of src/dissipation.jl:125-206 on every interior cell."""
         check(_lib.lib().fr2d_positivity(self.h))

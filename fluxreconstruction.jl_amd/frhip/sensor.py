@@ -198,18 +198,25 @@ def shock_vortex_solver(nx, ny, deg=3, gamma=1.4, flux="hll", **ic):
 
 
 def run_shock_vortex(s: FREuler2D, dt, nsteps, mode="modal", lam=C3["lam"], kappa=C3["kappa"], callback=None,
-                     every=0):
+                     every=0, fused=True):
     """Host loop of example/shock-vortex.jl:296-326 on the device state: positivity limiter on
     every cell (:298-303), SSP-RK3 step, sensor + filter (mode "modal" / "learned" / "off",
     :306-321), ghost refresh of the bottom, top and right sides with the inflow column kept
-    (:322-325)."""
+    (:322-325).  fused: the filter, the ghost refresh and the next iteration's limiter run as one
+    pass over the state (fr2d_filter_refresh_limit, the same arithmetic); the last iteration ends
+    with the filter and the ghost refresh."""
     S0 = -3.0 * np.log10(s.ps.deg)
+    s.positivity()
     for n in range(nsteps):
-        s.positivity()
         s.step(dt, 1, "ssprk3")
-        if mode != "off":
-            _filter(s, mode, lam, S0, kappa)
-        s.fill_ghost("inflow_left")
+        if fused and n + 1 < nsteps:
+            s.filter_refresh_limit(mode, lam, S0, kappa, "inflow_left")
+        else:
+            if mode != "off":
+                _filter(s, mode, lam, S0, kappa)
+            s.fill_ghost("inflow_left")
+            if n + 1 < nsteps:
+                s.positivity()
         if callback is not None and every and (n + 1) % every == 0:
             callback(n + 1)
 
@@ -251,7 +258,8 @@ def collect_training_data(nruns=200, nx=128, ny=64, nsteps=240, every=40, seed=1
 
         try:
             snap(0)
-            run_shock_vortex(s, dt, nsteps, "modal", callback=snap, every=every)
+            # unfused loop: the snapshots see the state after the ghost refresh, before the limiter
+            run_shock_vortex(s, dt, nsteps, "modal", callback=snap, every=every, fused=False)
             xs.extend(run_x)
             ys.extend(run_y)
         except FRError:

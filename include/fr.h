@@ -144,6 +144,15 @@ int fr2d_sensor_eval(fr2d_ctx *ctx, int mode, double S0, double kappa, float *d_
  * weights wp/4 (shock-vortex.jl:298-303).  Returns FR_E_LIMITER if an element fails the
  * reference's t1 assertion (that element is left unchanged). */
 int fr2d_positivity(fr2d_ctx *ctx);
+/* The end of one iteration of the shock-vortex host loop and the start of the next in one pass
+ * over the state (example/shock-vortex.jl:306-325 followed by :298-303): sensor + filter of every
+ * cell (mode FR_SENSOR_*, as fr2d_sensor_filter), the ghost refresh `rule` (FR_GHOST_*) written
+ * from the filtered values, and the positivity limiter on every interior cell.  Equal to
+ * fr2d_sensor_filter, fr2d_fill_ghost(rule), fr2d_positivity in sequence, reading and writing
+ * each cell once.  n_filtered (optional): filtered cells whose values are kept.  Returns
+ * FR_E_LIMITER like fr2d_positivity. */
+int fr2d_filter_refresh_limit(fr2d_ctx *ctx, int mode, double lambda, double S0, double kappa, int rule,
+                              int *n_filtered);
 /* Scan the device state for NaN/Inf (FR_E_NONFINITE) or rho <= 0 / p <= 0 (FR_E_NEGATIVE_STATE). */
 int fr2d_check_state(fr2d_ctx *ctx);
 

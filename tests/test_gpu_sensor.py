@@ -104,3 +104,30 @@ def test_solver_features_and_learned_filter():
     assert n > 0
     ug = s.get_state()
     assert np.max(np.abs(ug - uc)) <= 1e-13 * np.max(np.abs(uc))
+
+
+@pytest.mark.parametrize("mode", ["off", "modal", "learned"])
+def test_fused_filter_refresh_limit_equals_host_loop(mode):
+    """fr2d_filter_refresh_limit (filter, inflow-keeping ghost refresh and the next limiter in one
+    pass) reproduces the unfused host loop of example/shock-vortex.jl:296-326 bitwise."""
+    import os
+
+    ps = frhip.FRPSpace2D(0.0, 2.0, 48, 0.0, 1.0, 24, 3, 1, 1)
+    u0 = ics.shock_vortex(ps, 1.4, shock_mach=1.3)
+    dt, _ = ics.cfl_steps(u0, 1.4, ps.dx0, 0.05, 1.0)
+    mlp = None
+    if mode == "learned":
+        if not os.path.exists(S.WEIGHTS):
+            pytest.skip("no trained sensor weights in the package")
+        mlp = S.SensorMLP.load()
+    out = []
+    for fused in (False, True):
+        s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
+        s.set_state(u0)
+        if mlp is not None:
+            S.attach(s, mlp)
+        S.run_shock_vortex(s, dt, 12, mode, fused=fused)
+        out.append(s.get_state())
+        s.close()
+    assert np.all(np.isfinite(out[0]))
+    assert np.array_equal(out[0], out[1])
