@@ -7,14 +7,17 @@
 // Mapping: a workgroup owns a tile of TX x TY = 16 x 4 elements and has NSP wavefronts.  Lane =
 // element of the tile (16 consecutive elements along x per row, so every global access of a
 // wavefront covers four 128-byte segments of the element-fastest SoA layout), wave index = one
-// line of solution points.  Each thread first holds row `line` of its element (x sweep: flux,
-// traces, derivative), the tile is transposed through LDS, and the same thread then holds column
-// `line` (y sweep).  All face data of a line stays inside its wavefront (the neighbours of an
-// element are lanes -1/+1 along x and -TX/+TX along y; the traces of the 2*(TX+TY) halo elements
-// around the tile come from halo tasks on lanes of the same wavefront) and is exchanged through
-// per-line LDS slices that reuse the transpose buffer while it is free, so a workgroup needs
-// only the transpose's LDS.  Every element is read from HBM once; halo re-reads hit L2 (tiles are
-// ordered XCD-aware).
+// line of solution points.  Each thread first holds row `line` of its element and finishes the
+// whole x direction before the first barrier (flux, traces, derivative, the x-face Riemann
+// problems inside the row and the correction terms); the tile is transposed through LDS and the
+// same thread then holds column `line` for the y direction.  All face data of a line stays inside
+// its wavefront (the neighbours of an element are lanes -1/+1 along x and -TX/+TX along y; the
+// traces of the halo elements around the tile come from halo tasks on lanes of the same
+// wavefront), so faces need only wavefront-level LDS ordering: x faces in a per-line slice,
+// y faces in the line's own column block of the transpose buffer once that column is read.
+// At nsp = 4 a workgroup takes 40 KB of LDS and 128 VGPRs (Rusanov), so four workgroups share
+// a CU and a 256 x 256 grid (1024 tiles) is resident at once.  Every element is read from HBM
+// once; halo re-reads hit L2 (tiles are ordered XCD-aware).
 #include <algorithm>
 #include <cstring>
 #include <new>
@@ -41,7 +44,9 @@ struct Geo2D {
   int periodic;
 };
 
-enum { EPI_RESIDUAL = 0, EPI_STAGE = 1 };
+// epilogues: residual du; RK stage out = a0*u0 + a1*u + b*du (b != 0); the general stage form
+// with du kept apart, optionally with the RK4 accumulator acc_out = acc_in + b2*du
+enum { EPI_RESIDUAL = 0, EPI_STAGE = 1, EPI_ACC = 2 };
 
 struct StageArgs {
   const double *u;   // state whose residual is taken
@@ -57,11 +62,11 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
   // operators are compile-time constants (Gauss-Legendre points, Radau correction), so they
   // occupy no scalar registers for the length of the kernel
   constexpr StdOps2D<NSP> ops{};
-  // LDS: the tile transpose (rows -> columns, then the y contributions columns -> rows) and the
-  // face traces / fluxes of every line (x faces sX, y faces sY; each line owns its slices)
+  // LDS: the tile transpose sU (rows -> columns, then the y contributions columns -> rows) and
+  // the x faces of every line (sX).  The y faces of a line live in that line's own column block
+  // of sU between the column read and the write-back of the y contributions.
   __shared__ double sU[NSP * NSP * 4 * TE];
-  __shared__ double sX[4 * NSP * TY * (TX + 1)];
-  __shared__ double sY[4 * NSP * (TY + 1) * TX];
+  __shared__ double sX[4 * NSP * TY * TX];
 
   const int tid = threadIdx.x;
   // the line index is uniform over a wavefront, so every plane offset below is a scalar
@@ -71,11 +76,21 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
   const unsigned nxg = geo.nx + 2;
   const long long ldp = geo.ldp;
   const double *u = sa.u;
-  const double g = geo.g, gm1 = geo.gm1, iJ11 = geo.iJ11, iJ22 = geo.iJ22;
+  const double g = geo.g, gm1 = geo.gm1;
+  // scale of the x and y contributions in the result: -iJ (residual, RK4 form) or -b*iJ (stage,
+  // b != 0 on the host); the stage carries a1*u inside the x part as (a1 / sxs)*u
+  const double sxs = EPI == EPI_STAGE ? -sa.b * geo.iJ11 : -geo.iJ11;
+  const double sys = EPI == EPI_STAGE ? -sa.b * geo.iJ22 : -geo.iJ22;
+  const double ua = EPI == EPI_STAGE ? sa.a1 / sxs : 0.0;
 
   auto SU = [&](int k, int l, int m) -> double & { return sU[((k * NSP + l) * 4 + m) * TE + lane]; };
-  auto SX = [&](int m, int l, int y, int fx) -> double & { return sX[((m * NSP + l) * TY + y) * (TX + 1) + fx]; };
-  auto SY = [&](int m, int c, int fy, int x) -> double & { return sY[((m * NSP + c) * (TY + 1) + fy) * TX + x]; };
+  // x-face slot s of row y holds face s + 1 (between elements s and s + 1 of the row): first the
+  // right trace of element s, then the face flux
+  auto SX = [&](int m, int y, int s) -> double & { return sX[((m * NSP + line) * TY + y) * TX + s]; };
+  double *const sYl = sU + line * (NSP * 4 * TE);
+  static_assert(4 * (TY + 1) * TX <= 4 * 2 * TE, "the y faces of a line fit in its column block");
+  // y-face slot fy of column x: face between elements fy - 1 and fy of the column
+  auto SY = [&](int m, int fy, int x) -> double & { return sYl[(m * (TY + 1) + fy) * TX + x]; };
   // value of plane (k, l, m) at the element with byte offset bo inside the plane
   const unsigned ldp8 = (unsigned)(ldp * 8);
   auto LD = [&](const double *base, int k, int l, int m, unsigned bo) -> double {
@@ -128,33 +143,27 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
       for (int k = 0; k < NSP; ++k) ur[k][m] = (m == 0 || m == 3) ? 1.0 : 0.0;
   }
 
-  // u0 row of the stage epilogue: its loads are issued with the own row and the halo lines, and
-  // it is used after the halo traces, so the three reads are in flight together
-  const bool with_u0 = EPI == EPI_STAGE && sa.a0 != 0.0;
-  double u0r[NSP][4];
-  if (EPI == EPI_STAGE && act && with_u0) {
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) u0r[k][m] = LD(sa.u0, k, line, m, ebo);
-  }
-
-  // Halo tasks of this wavefront's line: lanes [0, TX) bottom (column `line` of the element below
-  // the tile, its top trace), [TX, 2TX) top (bottom trace of the element above), [2TX, 2TX+TY)
-  // left (row `line`, right trace of the left neighbour), [2TX+TY, 2TX+2TY) right (left trace).
-  int hkind = -1, hpos = 0;
-  if (lane < TX) {
-    hkind = 0;
-    hpos = lane;
-  } else if (lane < 2 * TX) {
-    hkind = 1;
-    hpos = lane - TX;
-  } else if (lane < 2 * TX + TY) {
+  // Halo tasks of this wavefront's line.  The left task (right trace of the left neighbour, row
+  // `line`) runs on the lane of the tile's first element of that row, which solves that face
+  // itself; the right task (left trace of the right neighbour) on the last lane of the row; the
+  // bottom tasks (top trace of the element below, column `line`) and top tasks (bottom trace of
+  // the element above) on the lanes between.
+  int hkind = -1, hpos = 0;  // 0 bottom, 1 top, 2 left, 3 right
+  if (ex == 0) {
     hkind = 2;
-    hpos = lane - 2 * TX;
-  } else if (lane < 2 * TX + 2 * TY) {
+    hpos = ey;
+  } else if (ex == TX - 1) {
     hkind = 3;
-    hpos = lane - 2 * TX - TY;
+    hpos = ey;
+  } else {
+    const int t = ey * (TX - 2) + ex - 1;
+    if (t < TX) {
+      hkind = 0;
+      hpos = t;
+    } else if (t < 2 * TX) {
+      hkind = 1;
+      hpos = t - TX;
+    }
   }
   if (hkind >= 0 && hpos >= (hkind <= 1 ? ncx : ncy)) hkind = -1;
   double htr[4] = {0.0, 0.0, 0.0, 0.0};
@@ -220,23 +229,10 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
     }
   }
 
-  // stage epilogue operand a0*u0 + a1*u of the own row, formed while the row is in registers
-  double pre[NSP][4];
-  if constexpr (EPI == EPI_STAGE) {
-    if (act) {
-#pragma unroll
-      for (int m = 0; m < 4; ++m)
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) {
-          double v = sa.a1 * ur[k][m];
-          if (with_u0) v = sa.a0 * u0r[k][m] + v;
-          pre[k][m] = v;
-        }
-    }
-  }
-
-  // row: traces, LDS transpose copy, contravariant x flux, its traces and dl derivative
-  double uLx[4], dux[NSP][4];
+  // x sweep of the row: traces (right traces into the line's x-face slots), LDS transpose copy,
+  // x flux and its derivative with the flux-trace parts of the corrections folded in (dc); the
+  // stage seeds it with (a1 / sxs)*u, so that sxs times it carries a1*u and the row dies here
+  double uLx[4], dxr[NSP][4];
 #pragma unroll
   for (int m = 0; m < 4; ++m) {
     double a = 0.0, b = 0.0;
@@ -246,7 +242,7 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
       b += ur[k][m] * ops.lr[k];
     }
     uLx[m] = a;
-    if (act) SX(m, line, ey, ex + 1) = b;  // right trace, at the x-slot of the face it borders
+    if (act) SX(m, ey, ex) = b;
   }
   if (act) {
 #pragma unroll
@@ -261,121 +257,135 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
       XState xs;
       xstate(ur[k], gm1, xs);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) f1[k][m] = iJ11 * xs.F[m];
+      for (int m = 0; m < 4; ++m) f1[k][m] = xs.F[m];
     }
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int q = 0; q < NSP; ++q) {
-        a += f1[q][m] * ops.ll[q];
-        b += f1[q][m] * ops.lr[q];
-      }
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int k = 0; k < NSP; ++k) {
         double d = 0.0;
 #pragma unroll
-        for (int q = 0; q < NSP; ++q) d += f1[q][m] * ops.dl[k + NSP * q];
-        // derivative minus the discontinuous-flux traces of the correction terms
-        dux[k][m] = d - a * ops.dhl[k] - b * ops.dhr[k];
+        for (int q = 0; q < NSP; ++q) d += f1[q][m] * ops.dc[k + NSP * q];
+        dxr[k][m] = EPI == EPI_STAGE ? ua * ur[k][m] + d : d;
       }
+  }
+  // RK4 form: stage operand a1*u of the own row
+  double pre[NSP][4];
+  if constexpr (EPI == EPI_ACC) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) pre[k][m] = sa.a1 * ur[k][m];
+  }
+  wave_lds_sync();  // the right traces of this line are visible to its lanes
+
+  // own left x-face: the left neighbour's right trace (the left halo on the row's first lane);
+  // the flux replaces the trace in its slot, where the left neighbour reads it
+  double cl[4] = {0.0, 0.0, 0.0, 0.0};
+  if (act) {
+    double wl[4];
+    const int s = ex > 0 ? ex - 1 : 0;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) wl[m] = ex > 0 ? SX(m, ey, s) : htr[m];
+    riemann_x4<FLUX>(wl, uLx, g, gm1, cl);
+    if (ex > 0) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) SX(m, ey, s) = cl[m];
     }
   }
-  // the left halo at x-slot 0 and the bottom halo at y-slot 0 of this line's slices
-  if (hkind == 2) {
+  wave_lds_sync();
+  // x corrections (the tile's last element of a row takes its right face after the halo pass);
+  // the x part of the result: -iJ11*x (residual, RK4 form) or a1*u - b*iJ11*x (stage)
+  double res[NSP][4];
+  {
+    const bool has_r = ex < ncx - 1;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) SX(m, line, hpos, 0) = htr[m];
-  } else if (hkind == 0) {
+    for (int m = 0; m < 4; ++m) {
+      const double cr = has_r ? SX(m, ey, ex) : 0.0;
 #pragma unroll
-    for (int m = 0; m < 4; ++m) SY(m, line, 0, hpos) = htr[m];
+      for (int k = 0; k < NSP; ++k) {
+        res[k][m] = sxs * (dxr[k][m] + cl[m] * ops.dhl[k] + cr * ops.dhr[k]);
+        fr_pin(res[k][m]);
+      }
+    }
   }
   __syncthreads();  // B1: the rows of every line are in sU
 
   // ---------------------------------------------------------------- P1: column sweep
-  double uBy[4], duy[NSP][4];
+  double uc[NSP][4];
+  if (act) {
+#pragma unroll
+    for (int q = 0; q < NSP; ++q)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) uc[q][m] = SU(line, q, m);
+  } else {
+#pragma unroll
+    for (int q = 0; q < NSP; ++q)
+#pragma unroll
+      for (int m = 0; m < 4; ++m) uc[q][m] = (m == 0 || m == 3) ? 1.0 : 0.0;
+  }
+  wave_lds_sync();  // the column block is read: it now carries this line's y faces
+  double uBy[4], dyr[NSP][4];
+#pragma unroll
+  for (int m = 0; m < 4; ++m) {
+    double a = 0.0, b = 0.0;
+#pragma unroll
+    for (int q = 0; q < NSP; ++q) {
+      a += uc[q][m] * ops.ll[q];
+      b += uc[q][m] * ops.lr[q];
+    }
+    uBy[m] = a;
+    if (act) SY(m, ey + 1, ex) = b;  // top trace, at the y-slot of the face it borders
+  }
+  if (hkind == 0) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m) SY(m, 0, hpos) = htr[m];
+  }
+  // the y part of the result, scaled like the x part
   {
-    double uc[NSP][4];
-    if (act) {
-#pragma unroll
-      for (int q = 0; q < NSP; ++q)
-#pragma unroll
-        for (int m = 0; m < 4; ++m) uc[q][m] = SU(line, q, m);
-    } else {
-#pragma unroll
-      for (int q = 0; q < NSP; ++q)
-#pragma unroll
-        for (int m = 0; m < 4; ++m) uc[q][m] = (m == 0 || m == 3) ? 1.0 : 0.0;
-    }
-#pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int q = 0; q < NSP; ++q) {
-        a += uc[q][m] * ops.ll[q];
-        b += uc[q][m] * ops.lr[q];
-      }
-      uBy[m] = a;
-      if (act) SY(m, line, ey + 1, ex) = b;  // top trace, at the y-slot of the face it borders
-    }
     double f2[NSP][4];
 #pragma unroll
     for (int q = 0; q < NSP; ++q) {
       const double r = fr_rcp(uc[q][0]);
-      const double uu = uc[q][1] * r, vv = uc[q][2] * r;
-      const double p = gm1 * (uc[q][3] - 0.5 * (uc[q][1] * uu + uc[q][2] * vv));
-      f2[q][0] = iJ22 * uc[q][2];
-      f2[q][1] = iJ22 * (uc[q][1] * vv);
-      f2[q][2] = iJ22 * (uc[q][2] * vv + p);
-      f2[q][3] = iJ22 * ((uc[q][3] + p) * vv);
+      const double vv = uc[q][2] * r;
+      const double p = gm1 * (uc[q][3] - 0.5 * (uc[q][1] * (uc[q][1] * r) + uc[q][2] * vv));
+      f2[q][0] = uc[q][2];
+      f2[q][1] = uc[q][1] * vv;
+      f2[q][2] = uc[q][2] * vv + p;
+      f2[q][3] = (uc[q][3] + p) * vv;
     }
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double a = 0.0, b = 0.0;
-#pragma unroll
-      for (int q = 0; q < NSP; ++q) {
-        a += f2[q][m] * ops.ll[q];
-        b += f2[q][m] * ops.lr[q];
-      }
+    for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int k = 0; k < NSP; ++k) {
         double d = 0.0;
 #pragma unroll
-        for (int q = 0; q < NSP; ++q) d += f2[q][m] * ops.dl[k + NSP * q];
-        duy[k][m] = d - a * ops.dhl[k] - b * ops.dhr[k];
+        for (int q = 0; q < NSP; ++q) d += f2[q][m] * ops.dc[k + NSP * q];
+        dyr[k][m] = sys * d;
+        fr_pin(dyr[k][m]);
       }
-    }
   }
+  wave_lds_sync();  // this line's y traces are visible
 
-  // ---------------------------------------------------------------- P2: interface fluxes
-  // own left x-face: the slot holds the left neighbour's right trace (or the left halo) and is
-  // replaced by the flux (each slot is read and written by one lane); its inputs were written
-  // before B1
-  if (act) {
-    double wl[4], fw[4];
-#pragma unroll
-    for (int m = 0; m < 4; ++m) wl[m] = SX(m, line, ey, ex);
-    riemann_x4<FLUX>(wl, uLx, g, gm1, fw);
-#pragma unroll
-    for (int m = 0; m < 4; ++m) SX(m, line, ey, ex) = fw[m];
-  }
-  wave_lds_sync();  // this line's y traces (written by its own lanes in P1) are visible
-  // own bottom y-face, likewise
+  // ---------------------------------------------------------------- P2: y faces, tile edges
+  // own bottom y-face: the slot holds the lower neighbour's top trace (or the bottom halo) and is
+  // replaced by the flux
   if (act) {
     double wb[4], fw[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) wb[m] = SY(m, line, ey, ex);
+    for (int m = 0; m < 4; ++m) wb[m] = SY(m, ey, ex);
     riemann_y4<FLUX>(wb, uBy, g, gm1, fw);
 #pragma unroll
-    for (int m = 0; m < 4; ++m) SY(m, line, ey, ex) = fw[m];
+    for (int m = 0; m < 4; ++m) SY(m, ey, ex) = fw[m];
   }
-  // One divergent pass for the tile's right x-faces (hkind 3, solved by the right halo task) and
-  // top y-faces (hkind 1): a y-face is solved in its normal frame n = (0, 1), i.e. on
-  // (rho, m_y, -m_x, E) with the flux rotated back as (f0, -f2, f1, f3).
+  // One divergent pass for the tile's right x-faces (right tasks) and top y-faces (top tasks): a
+  // y-face is solved in its normal frame n = (0, 1), i.e. on (rho, m_y, -m_x, E) with the flux
+  // rotated back as (f0, -f2, f1, f3).
   if (hkind == 1 || hkind == 3) {
     const bool xf = hkind == 3;
     double *slot[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) slot[m] = xf ? &SX(m, line, hpos, ncx) : &SY(m, line, ncy, hpos);
+    for (int m = 0; m < 4; ++m) slot[m] = xf ? &SX(m, hpos, ncx - 1) : &SY(m, ncy, hpos);
     double w[4], a[4], b[4], fw[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) w[m] = *slot[m];
@@ -399,39 +409,55 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
   if (act) {
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
-      const double cb = SY(m, line, ey, ex) * iJ22;
-      const double ct = SY(m, line, ey + 1, ex) * iJ22;
-      const double cl = SX(m, line, ey, ex) * iJ11;
-      const double cr = SX(m, line, ey, ex + 1) * iJ11;
+      const double cb = sys * SY(m, ey, ex), ct = sys * SY(m, ey + 1, ex);
 #pragma unroll
-      for (int k = 0; k < NSP; ++k) {
-        duy[k][m] += cb * ops.dhl[k] + ct * ops.dhr[k];
-        dux[k][m] += cl * ops.dhl[k] + cr * ops.dhr[k];
+      for (int k = 0; k < NSP; ++k) dyr[k][m] += cb * ops.dhl[k] + ct * ops.dhr[k];
+    }
+    if (ex == ncx - 1) {  // right face of the row's last element, solved by the right task
+#pragma unroll
+      for (int m = 0; m < 4; ++m) {
+        const double cr = sxs * SX(m, ey, ex);
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) res[k][m] += cr * ops.dhr[k];
       }
     }
   }
-  // the y contributions go back through sU: each thread rewrites the entries of its own column
-  // that it read in P1, so no other wavefront is touched before the barrier
+  wave_lds_sync();  // the y faces are read: the column block takes the y contributions
+  // each thread writes the entries of its own column that it read in P1, so no other wavefront
+  // is touched before the barrier
   if (act) {
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
-      for (int k = 0; k < NSP; ++k) SU(line, k, m) = duy[k][m];
+      for (int k = 0; k < NSP; ++k) SU(line, k, m) = dyr[k][m];
   }
   __syncthreads();  // B2: y contributions of every point visible
 
   // ---------------------------------------------------------------- P4: assemble + epilogue
+  // the u0 row is read last, when the live set is smallest (4 workgroups per CU at nsp = 4)
+  const bool with_u0 = EPI != EPI_RESIDUAL && sa.a0 != 0.0;
+  double u0r[NSP][4];
+  if (EPI != EPI_RESIDUAL && act && with_u0) {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) u0r[k][m] = LD(sa.u0, k, line, m, ebo);
+  }
   if (act) {
 #pragma unroll
     for (int m = 0; m < 4; ++m)
 #pragma unroll
       for (int k = 0; k < NSP; ++k) {
-        const double du = -(dux[k][m] + SU(k, line, m));
-        if constexpr (EPI == EPI_RESIDUAL) {
-          ST(sa.out, k, line, m, ebo, du);
+        const double v = SU(k, line, m) + res[k][m];
+        if constexpr (EPI == EPI_ACC) {  // v = du; out = a0*u0 + a1*u + b*du, acc += b2*du
+          double o = pre[k][m] + sa.b * v;
+          if (with_u0) o = sa.a0 * u0r[k][m] + o;
+          ST(sa.out, k, line, m, ebo, o);
+          if (sa.acc_out) ST(sa.acc_out, k, line, m, ebo, LD(sa.acc_in, k, line, m, ebo) + sa.b2 * v);
+        } else if constexpr (EPI == EPI_STAGE) {
+          ST(sa.out, k, line, m, ebo, with_u0 ? sa.a0 * u0r[k][m] + v : v);
         } else {
-          ST(sa.out, k, line, m, ebo, pre[k][m] + sa.b * du);
-          if (sa.acc_out) ST(sa.acc_out, k, line, m, ebo, LD(sa.acc_in, k, line, m, ebo) + sa.b2 * du);
+          ST(sa.out, k, line, m, ebo, v);
         }
       }
   }
@@ -507,13 +533,16 @@ static int launch_flux(const Fr2dCtx &c, const StageArgs &sa, bool residual, hip
   switch (c.flux) {
     case FR_FLUX_HLL:
       return residual ? launch_nsp<NSP, FR_FLUX_HLL, EPI_RESIDUAL>(c, sa, st)
-                      : launch_nsp<NSP, FR_FLUX_HLL, EPI_STAGE>(c, sa, st);
+             : (sa.acc_out || sa.b == 0.0) ? launch_nsp<NSP, FR_FLUX_HLL, EPI_ACC>(c, sa, st)
+                          : launch_nsp<NSP, FR_FLUX_HLL, EPI_STAGE>(c, sa, st);
     case FR_FLUX_RUSANOV:
       return residual ? launch_nsp<NSP, FR_FLUX_RUSANOV, EPI_RESIDUAL>(c, sa, st)
-                      : launch_nsp<NSP, FR_FLUX_RUSANOV, EPI_STAGE>(c, sa, st);
+             : (sa.acc_out || sa.b == 0.0) ? launch_nsp<NSP, FR_FLUX_RUSANOV, EPI_ACC>(c, sa, st)
+                          : launch_nsp<NSP, FR_FLUX_RUSANOV, EPI_STAGE>(c, sa, st);
     default:
       return residual ? launch_nsp<NSP, FR_FLUX_ROE, EPI_RESIDUAL>(c, sa, st)
-                      : launch_nsp<NSP, FR_FLUX_ROE, EPI_STAGE>(c, sa, st);
+             : (sa.acc_out || sa.b == 0.0) ? launch_nsp<NSP, FR_FLUX_ROE, EPI_ACC>(c, sa, st)
+                          : launch_nsp<NSP, FR_FLUX_ROE, EPI_STAGE>(c, sa, st);
   }
 }
 

@@ -97,6 +97,10 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Marks v as produced at this point: the instructions that compute it are not sunk by the
+// compiler towards a later use (past a barrier, into the next phase's register peak).
+__device__ __forceinline__ void fr_pin(double &v) { asm volatile("" : "+v"(v)); }
+
 // ------------------------------------------------------------------ FP64 reciprocal / square root
 // The hardware estimates v_rcp_f64 / v_rsq_f64 refined by two Newton-Raphson (reciprocal) and one
 // Goldschmidt + Newton (square root) iterations in FMA arithmetic: about one ulp from the

@@ -110,6 +110,20 @@ def test_stepping_parity(scheme, bc):
     assert rel_l2(ug, uc) <= STATE_TOL
 
 
+@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler"])
+def test_zero_dt_step_keeps_state(scheme):
+    """dt = 0 takes the general stage form (b = 0): every scheme returns the state (SSP-RK3 up to
+    the rounding of 1/3 + 2/3, the others bitwise)."""
+    ps, u, s, orc = setup(20, 8, 3, "hll", "periodic")
+    s.set_state(u)
+    s.step(0.0, 3, scheme)
+    ug = s.get_state()
+    if scheme == "ssprk3":
+        assert np.allclose(ug[1:-1, 1:-1], u[1:-1, 1:-1], rtol=1e-15, atol=0.0)
+    else:
+        assert np.array_equal(ug[1:-1, 1:-1], u[1:-1, 1:-1])
+
+
 def test_graph_replay_matches_eager():
     """The context keeps its captured step graph: every replay (with the host reading the state,
     eager steps on another context and a second replay count in between) equals the eager
