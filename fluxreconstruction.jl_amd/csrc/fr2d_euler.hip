@@ -110,6 +110,23 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
       *reinterpret_cast<double *>(reinterpret_cast<char *>(base + ldp * plane) + bo) = v;
   };
 
+  // the same with a per-lane plane pl (of the first variable) and a uniform variable index m
+  auto LDL = [&](const double *base, int pl, int m, unsigned bo) -> double {
+    if constexpr (ADDR == ADDR_BUFFER)
+      return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(plane_rsrc(base, 0xFFFFFFFFu),
+                                                                             bo + ldp8 * (unsigned)pl,
+                                                                             (int)(ldp8 * (unsigned)(NSP * NSP * m)), 0));
+    else
+      return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base + ldp * (pl + NSP * NSP * m)) + bo);
+  };
+  auto STL = [&](double *base, int pl, int m, unsigned bo, double v) {
+    if constexpr (ADDR == ADDR_BUFFER)
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(fr_u2, v), plane_rsrc(base, 0xFFFFFFFFu),
+                                            bo + ldp8 * (unsigned)pl, (int)(ldp8 * (unsigned)(NSP * NSP * m)), 0);
+    else
+      *reinterpret_cast<double *>(reinterpret_cast<char *>(base + ldp * (pl + NSP * NSP * m)) + bo) = v;
+  };
+
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int bx = tile % geo.ntx, by = tile / geo.ntx;
   const int i0 = 1 + bx * TX, j0 = 1 + by * TY;
@@ -189,42 +206,29 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
     }
     const unsigned hbo = 8u * ((unsigned)si + nxg * (unsigned)sj);
     const bool lo = hkind == 0 || hkind == 2;  // weights lr (trace on the far side) or ll
+    // point q of the task's line: (k = line, l = q) for column tasks, (k = q, l = line) for row
+    // tasks, i.e. plane pb + ps*q + NSP*NSP*m; one set of loads serves both kinds
+    const bool col = hkind <= 1;
+    const int pb = col ? line : NSP * line, ps = col ? NSP : 1;
     double w[NSP];
 #pragma unroll
     for (int q = 0; q < NSP; ++q) w[q] = lo ? ops.lr[q] : ops.ll[q];
-    if (hkind <= 1) {  // column `line`: points (k = line, l = q)
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        double s = 0.0;
+    for (int m = 0; m < 4; ++m) {
+      double s = 0.0;
 #pragma unroll
-        for (int q = 0; q < NSP; ++q) s += LD(u, line, q, m, hbo) * w[q];
-        htr[m] = s;
-      }
-    } else {  // row `line`: points (k = q, l = line)
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < NSP; ++q) s += LD(u, q, line, m, hbo) * w[q];
-        htr[m] = s;
-      }
+      for (int q = 0; q < NSP; ++q) s += LDL(u, pb + ps * q, m, hbo) * w[q];
+      htr[m] = s;
     }
     if constexpr (EPI == EPI_RESIDUAL) {
       // du vanishes on the ghost layer (example/shock-vortex.jl:27)
       const bool ghost = hi == 0 || hi == geo.nx + 1 || hj == 0 || hj == geo.ny + 1;
       if (ghost) {
         const unsigned gbo = 8u * ((unsigned)hi + nxg * (unsigned)hj);
-        if (hkind <= 1) {
 #pragma unroll
-          for (int m = 0; m < 4; ++m)
+        for (int m = 0; m < 4; ++m)
 #pragma unroll
-            for (int q = 0; q < NSP; ++q) ST(sa.out, line, q, m, gbo, 0.0);
-        } else {
-#pragma unroll
-          for (int m = 0; m < 4; ++m)
-#pragma unroll
-            for (int q = 0; q < NSP; ++q) ST(sa.out, q, line, m, gbo, 0.0);
-        }
+          for (int q = 0; q < NSP; ++q) STL(sa.out, pb + ps * q, m, gbo, 0.0);
       }
     }
   }
