@@ -1,6 +1,6 @@
 """Benchmark of the 2D Euler FR hot path on one MI355X.
 
-Workload (BASELINE configs[1]): 2D Euler (gamma = 1.4) on the periodic square [0,10]^2, 256 x 256
+Workload: 2D Euler (gamma = 1.4) on the periodic square [0,10]^2, 256 x 256
 uniform quads, p = 3 (16 Gauss points per element, 4,194,304 DOF), isentropic vortex (strength 5,
 centred) on rho = 1, u = v = 1, p = 1, Rusanov flux, SSP-RK3 at CFL 0.05 on the wave speed
 (dt = 0.05 dx / max(|u|+c, |v|+c) = 2/2999), FP64.
@@ -77,7 +77,7 @@ def cpu_baseline(u0, dt, budget_s=12.0):
 
 
 def bench_ensemble(steps=10):
-    """C4: 512 seeded 1D Euler Riemann problems (64..2048 cells, p in {2,3,4}), Rusanov, SSP-RK3;
+    """512 seeded 1D Euler Riemann problems (64..2048 cells, p in {2,3,4}), Rusanov, SSP-RK3;
     one batched launch per stage (autotuned mapping, hipGraph) vs one launch per member."""
     import numpy as np
     import torch
@@ -117,7 +117,7 @@ def bench_ensemble(steps=10):
 
 
 def bench_bgk(steps=3, ens_steps=1):
-    """C5: 2D BGK, 128^2 periodic quads p = 2 x 24^2 Gauss-Hermite velocities, Kn = tau = 1e-2;
+    """2D BGK, 128^2 periodic quads p = 2 x 24^2 Gauss-Hermite velocities, Kn = tau = 1e-2;
     plus the 64-member ensemble with tau log-uniform in [1e-3, 1e-1]."""
     import numpy as np
     import torch
@@ -148,14 +148,14 @@ def bench_bgk(steps=3, ens_steps=1):
 
 
 def bench_sensor():
-    """C3: shock-vortex 512x256 p = 3 to t = 0.4 with the modal sensor, the learned sensor (trained
+    """Shock-vortex 512x256 p = 3 to t = 0.4 with the modal sensor, the learned sensor (trained
     weights frhip/data/sensor_mlp.npz) and no sensor; density L1 vs a 2x refined modal run."""
     import os as _os
 
     from frhip import sensor as S
 
     mlp = S.SensorMLP.load() if _os.path.exists(S.WEIGHTS) else None
-    r = S.evaluate_c3(mlp)
+    r = S.evaluate_shock_vortex(mlp)
     r.pop("reference_means", None)
     if "learned" in r and "modal" in r:
         r["L1_learned_over_modal"] = r["learned"]["L1_density"] / r["modal"]["L1_density"]
@@ -297,7 +297,7 @@ def main():
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     if not args.no_extras:
-        for key, fn in (("ensemble_c4", bench_ensemble), ("bgk_c5", bench_bgk), ("sensor_c3", bench_sensor)):
+        for key, fn in (("ensemble_riemann512", bench_ensemble), ("bgk_128sq_p2_v24", bench_bgk), ("sensor_shock_vortex", bench_sensor)):
             try:
                 out[key] = fn()
             except Exception as exc:  # an extra leg must not hide the headline line

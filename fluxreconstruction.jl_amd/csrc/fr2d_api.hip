@@ -392,6 +392,27 @@ int fr2d_positivity(fr2d_ctx *c) {
   return FR_OK;
 }
 
+int fr2d_primitives_dev(fr2d_ctx *c, double *d_sol) {
+  FR_REQUIRE(c && d_sol, FR_E_INVALID, "fr2d_primitives_dev: null argument");
+  FR_REQUIRE(c->d_u, FR_E_INVALID, "fr2d_primitives_dev: no state");
+  return fr2d_launch_primitives(*c, c->d_u, d_sol, stream_of(*c));
+}
+
+int fr2d_primitives(fr2d_ctx *c, double *sol) {
+  FR_REQUIRE(c && sol, FR_E_INVALID, "fr2d_primitives: null argument");
+  FR_REQUIRE(c->d_u, FR_E_INVALID, "fr2d_primitives: no state");
+  hipStream_t st = stream_of(*c);
+  // the staging buffer of the host-pointer residual holds the ghost-inclusive state, which is
+  // larger than the interior point field
+  if (!c->d_out) FR_HIP(hipMalloc(&c->d_out, c->len * sizeof(double)));
+  const size_t n = (size_t)c->nx * c->ny * c->nsp * c->nsp * 4;
+  int rc = fr2d_launch_primitives(*c, c->d_u, c->d_out, st);
+  if (rc) return rc;
+  FR_HIP(hipMemcpyAsync(sol, c->d_out, n * sizeof(double), hipMemcpyDeviceToHost, st));
+  FR_HIP(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
 int fr2d_check_state(fr2d_ctx *c) {
   FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_check_state: no state");
   hipStream_t st = stream_of(*c);

@@ -48,6 +48,8 @@ int fr2d_launch_positivity(const Fr2dCtx &c, double *d_u, int *d_count, hipStrea
 int fr2d_launch_filter_refresh_limit(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0,
                                      double kappa, int rule, int *d_count, hipStream_t st);
 int fr2d_launch_check(const Fr2dCtx &c, const double *d_u, int *d_count, hipStream_t st);
+// primitive variables of every solution point on the global point grid (fr2d_post.hip)
+int fr2d_launch_primitives(const Fr2dCtx &c, const double *d_u, double *d_sol, hipStream_t st);
 // learned sensor: per-cell strengths from the MLP into c.d_strength (fr_mlp.hip)
 int fr2d_launch_mlp_sensor(const Fr2dCtx &c, const double *d_u, hipStream_t st);
 

@@ -35,5 +35,6 @@ from .fr1d import FR1D, FREulerProblem, FRAdvectionProblem, Ensemble1D  # noqa: 
 from . import ics  # noqa: F401
 from .bgk import BGK2D  # noqa: F401
 from . import sensor  # noqa: F401
+from . import io  # noqa: F401
 
 __version__ = "0.1.0"

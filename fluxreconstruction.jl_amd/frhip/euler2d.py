@@ -163,6 +163,31 @@ class FREuler2D:
     def check_state(self):
         check(_lib.lib().fr2d_check_state(self.h))
 
+    def primitives(self, out=None):
+        """Post-processing of example/shock-vortex.jl:333-352 on the device state.
+
+        Returns (x, y, sol): solution-point coordinates and the primitive variables
+        (rho, U, V, 1/lambda), lambda = rho/(2p), of every solution point on the global grid of
+        shape (nx*nsp, ny*nsp) (sol has a trailing axis of 4), element-major along each axis like
+        the reference.  With `out` (a torch GPU tensor of nx*nsp*ny*nsp*4 doubles) the field is
+        written on the device instead and (x, y, out) is returned."""
+        nx, ny, n = self.ps.nx, self.ps.ny, self.ps.nsp
+        xpg = np.asarray(self.ps.xpg)
+        if xpg.shape[0] == nx + 2:
+            xpg = xpg[1:-1, 1:-1]
+        x = np.transpose(xpg[..., 0], (0, 2, 1, 3)).reshape(nx * n, ny * n)
+        y = np.transpose(xpg[..., 1], (0, 2, 1, 3)).reshape(nx * n, ny * n)
+        if out is not None:
+            if not _is_torch(out) or out.numel() != nx * n * ny * n * 4:
+                raise ValueError("primitives: out must be a GPU tensor of nx*nsp*ny*nsp*4 doubles")
+            if str(out.dtype) != "torch.float64" or not out.is_cuda or not out.is_contiguous():
+                raise ValueError("primitives: out must be a contiguous float64 tensor on the GPU")
+            check(_lib.lib().fr2d_primitives_dev(self.h, C.c_void_p(out.data_ptr())))
+            return x, y, out
+        sol = np.zeros((nx * n, ny * n, 4), order="F")
+        check(_lib.lib().fr2d_primitives(self.h, dptr(sol)))
+        return x, y, sol
+
     def synchronize(self):
         import torch
 

@@ -185,19 +185,19 @@ def sensor_eval(solver: FREuler2D, mode="modal", S0=None, kappa=9.0):
 
 
 # ------------------------------------------------------------------ shock / vortex runs
-C3 = dict(x1=2.0, y1=1.0, nx=512, ny=256, deg=3, shock_mach=1.1, shock_x=0.5, vortex_mach=0.9, vortex_r=0.075,
+SHOCK_VORTEX = dict(x1=2.0, y1=1.0, nx=512, ny=256, deg=3, shock_mach=1.1, shock_x=0.5, vortex_mach=0.9, vortex_r=0.075,
           vortex_x=0.25, vortex_y=0.5, gamma=1.4, lam=5e-4, kappa=9.0, cfl=0.05)
 
 
 def shock_vortex_solver(nx, ny, deg=3, gamma=1.4, flux="hll", **ic):
-    ps = FRPSpace2D(0.0, C3["x1"], nx, 0.0, C3["y1"], ny, deg, 1, 1)
+    ps = FRPSpace2D(0.0, SHOCK_VORTEX["x1"], nx, 0.0, SHOCK_VORTEX["y1"], ny, deg, 1, 1)
     u0 = ics.shock_vortex(ps, gamma, **ic)
     s = FREuler2D(ps, gamma, flux, "ghost")
     s.set_state(u0)
     return ps, u0, s
 
 
-def run_shock_vortex(s: FREuler2D, dt, nsteps, mode="modal", lam=C3["lam"], kappa=C3["kappa"], callback=None,
+def run_shock_vortex(s: FREuler2D, dt, nsteps, mode="modal", lam=SHOCK_VORTEX["lam"], kappa=SHOCK_VORTEX["kappa"], callback=None,
                      every=0, fused=True):
     """Host loop of example/shock-vortex.jl:296-326 on the device state: positivity limiter on
     every cell (:298-303), SSP-RK3 step, sensor + filter (mode "modal" / "learned" / "off",
@@ -227,7 +227,7 @@ def _filter(s, mode, lam, S0, kappa):
 
 def collect_training_data(nruns=200, nx=128, ny=64, nsteps=240, every=40, seed=1234, max_neg_ratio=1.0):
     """Cell features and modal-sensor labels from `nruns` seeded shock / vortex runs with the vortex
-    position and strength drawn uniformly within +-20% of the C3 values and the shock Mach number
+    position and strength drawn uniformly within +-20% of the shock-vortex case values and the shock Mach number
     in [1.05, 1.5].  Every `every` steps all troubled cells and an equal number of random smooth
     cells are kept.  Returns torch GPU tensors (x bf16 (n, 16), y float32 (n,))."""
     import torch
@@ -237,17 +237,17 @@ def collect_training_data(nruns=200, nx=128, ny=64, nsteps=240, every=40, seed=1
     failed = 0
     for r in range(nruns):
         f = lambda: rng.uniform(0.8, 1.2)  # noqa: E731
-        ic = dict(shock_mach=rng.uniform(1.05, 1.5), vortex_mach=C3["vortex_mach"] * f(),
-                  vortex_r=C3["vortex_r"] * f(), vortex_x=C3["vortex_x"] * f(), vortex_y=C3["vortex_y"] * f())
+        ic = dict(shock_mach=rng.uniform(1.05, 1.5), vortex_mach=SHOCK_VORTEX["vortex_mach"] * f(),
+                  vortex_r=SHOCK_VORTEX["vortex_r"] * f(), vortex_x=SHOCK_VORTEX["vortex_x"] * f(), vortex_y=SHOCK_VORTEX["vortex_y"] * f())
         ps, u0, s = shock_vortex_solver(nx, ny, **ic)
-        dt, _ = ics.cfl_steps(u0, C3["gamma"], ps.dx0, C3["cfl"], 1.0)
+        dt, _ = ics.cfl_steps(u0, SHOCK_VORTEX["gamma"], ps.dx0, SHOCK_VORTEX["cfl"], 1.0)
         gen = torch.Generator(device="cuda")
         gen.manual_seed(int(rng.integers(1 << 31)))
         run_x, run_y = [], []
 
         def snap(_n):
             x = features(s)
-            y = sensor_eval(s, "modal", kappa=C3["kappa"])
+            y = sensor_eval(s, "modal", kappa=SHOCK_VORTEX["kappa"])
             pos = torch.nonzero(y > 0.5).squeeze(1)
             neg = torch.nonzero(y <= 0.5).squeeze(1)
             k = min(neg.numel(), max(int(max_neg_ratio * pos.numel()), 64))
@@ -288,7 +288,7 @@ def train_sensor(x, y, nsteps=3000, batch=4096, lr=2e-3, seed=0, val_frac=0.1):
     return mlp, dict(train_loss=loss, val_accuracy=acc, n_train=int(xt.shape[0]), n_val=nv)
 
 
-# ------------------------------------------------------------------ evaluation on C3
+# ------------------------------------------------------------------ evaluation on the shock-vortex case
 def cell_means(ps, u, var=0):
     """Element means of one variable over the interior cells (Gauss weights wp / 4)."""
     return np.einsum("ijkl,kl->ij", u[1:-1, 1:-1, :, :, var], ps.wp / 4.0)
@@ -299,9 +299,9 @@ def restrict_means(fine_means, factor):
     return fine_means.reshape(nx // factor, factor, ny // factor, factor).mean(axis=(1, 3))
 
 
-def evaluate_c3(mlp: SensorMLP | None, nx=C3["nx"], ny=C3["ny"], t_end=0.4, fine=2, modes=("modal", "learned", "off"),
+def evaluate_shock_vortex(mlp: SensorMLP | None, nx=SHOCK_VORTEX["nx"], ny=SHOCK_VORTEX["ny"], t_end=0.4, fine=2, modes=("modal", "learned", "off"),
                 reference=None):
-    """C3: shock-vortex interaction (Mach-1.1 stationary shock at x = 0.5, Lamb-Oseen vortex of
+    """Shock-vortex interaction (Mach-1.1 stationary shock at x = 0.5, Lamb-Oseen vortex of
     Mach 0.9 and radius 0.075 at x = 0.25) on nx x ny quads, p = 3, to t_end, with each sensor mode;
     density L1 error (src/tools.jl:43-45 on element means, sum |rho - rho_ref| dx dy) against a
     `fine`-times refined run with the modal sensor, and the wall time per step of each mode.
@@ -311,10 +311,10 @@ def evaluate_c3(mlp: SensorMLP | None, nx=C3["nx"], ny=C3["ny"], t_end=0.4, fine
     import torch
 
     out = {}
-    ic = {k: C3[k] for k in ("shock_mach", "shock_x", "vortex_mach", "vortex_r", "vortex_x", "vortex_y")}
+    ic = {k: SHOCK_VORTEX[k] for k in ("shock_mach", "shock_x", "vortex_mach", "vortex_r", "vortex_x", "vortex_y")}
     if reference is None:
         psf, u0f, sf = shock_vortex_solver(nx * fine, ny * fine, **ic)
-        dtf, nf = ics.cfl_steps(u0f, C3["gamma"], psf.dx0, C3["cfl"], t_end)
+        dtf, nf = ics.cfl_steps(u0f, SHOCK_VORTEX["gamma"], psf.dx0, SHOCK_VORTEX["cfl"], t_end)
         run_shock_vortex(sf, dtf, nf, "modal")
         reference = restrict_means(cell_means(psf, sf.get_state()), fine)
         sf.close()
@@ -325,7 +325,7 @@ def evaluate_c3(mlp: SensorMLP | None, nx=C3["nx"], ny=C3["ny"], t_end=0.4, fine
             if mlp is None:
                 continue
             attach(s, mlp)
-        dt, n = ics.cfl_steps(u0, C3["gamma"], ps.dx0, C3["cfl"], t_end)
+        dt, n = ics.cfl_steps(u0, SHOCK_VORTEX["gamma"], ps.dx0, SHOCK_VORTEX["cfl"], t_end)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         run_shock_vortex(s, dt, n, mode)

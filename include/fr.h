@@ -155,9 +155,16 @@ int fr2d_filter_refresh_limit(fr2d_ctx *ctx, int mode, double lambda, double S0,
                               int *n_filtered);
 /* Scan the device state for NaN/Inf (FR_E_NONFINITE) or rho <= 0 / p <= 0 (FR_E_NEGATIVE_STATE). */
 int fr2d_check_state(fr2d_ctx *ctx);
+/* Primitive variables of every solution point on the global point grid, the post-processing block
+ * of example/shock-vortex.jl:333-352 (its `sol`): column-major (nx*nsp, ny*nsp, 4) with
+ * sol[(i-1)*nsp + k, (j-1)*nsp + l, :] = (rho, U, V, 1/lambda), lambda = rho/(2p) as in KitBase
+ * conserve_prim.  fr2d_primitives copies to a host array (synchronises); fr2d_primitives_dev
+ * writes a device array on the context stream. */
+int fr2d_primitives(fr2d_ctx *ctx, double *sol);
+int fr2d_primitives_dev(fr2d_ctx *ctx, double *d_sol);
 
 /* ------------------------------------------------------------------ learned troubled-cell sensor
- * No reference counterpart (BASELINE's learned part): MLP 16 -> 32 -> 32 -> 1 (ReLU), BF16 weights
+ * No reference counterpart (a learned replacement of shock_detector): MLP 16 -> 32 -> 32 -> 1 (ReLU), BF16 weights
  * and inputs on MFMA with FP32 accumulation, FP32 master parameters and Adam.  Parameter vector
  * (fr_mlp_nparam() floats): W1[32][16] b1[32] W2[32][32] b2[32] W3[32] b3.  Inputs: per cell the
  * 16 density modal coefficients iV*rho normalised to unit 2-norm, as BF16 (fr2d_sensor_features). */

@@ -206,6 +206,28 @@ def conserve_prim2(w, g: float):
     return out
 
 
+def solution_field2d(u, xpg, g: float):
+    """example/shock-vortex.jl:333-352: for every interior element (i, j) and solution point
+    (k, l), x[idx, idy] = xpg[i, j, k, l, 1], y likewise, sol[idx, idy, :] = conserve_prim(u) with
+    the last entry inverted, idx = (i-1)*nsp + k, idy = (j-1)*nsp + l (1-based in the reference).
+    u and xpg carry the ghost layer (i, j = 0 .. n+1).  Loops: small inputs only."""
+    nx, ny, n = u.shape[0] - 2, u.shape[1] - 2, u.shape[2]
+    x = np.zeros((nx * n, ny * n))
+    y = np.zeros((nx * n, ny * n))
+    sol = np.zeros((nx * n, ny * n, 4))
+    for i in range(1, nx + 1):
+        for j in range(1, ny + 1):
+            idx0, idy0 = (i - 1) * n, (j - 1) * n
+            for k in range(n):
+                for l in range(n):
+                    x[idx0 + k, idy0 + l] = xpg[i, j, k, l, 0]
+                    y[idx0 + k, idy0 + l] = xpg[i, j, k, l, 1]
+                    prim = conserve_prim2(u[i, j, k, l, :], g)
+                    prim[3] = 1.0 / prim[3]
+                    sol[idx0 + k, idy0 + l, :] = prim
+    return x, y, sol
+
+
 # ----------------------------------------------------------------- 2D residual
 class OrFr2d(C.Structure):
     _fields_ = [

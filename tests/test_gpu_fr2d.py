@@ -225,3 +225,23 @@ def test_fill_ghost_rules(rule):
     s.set_state(u)
     s.fill_ghost(rule)
     assert np.array_equal(s.get_state(), _ghost_rule_numpy(u, rule))
+
+
+@pytest.mark.parametrize("deg", [1, 3, 4])
+def test_primitives_parity(deg):
+    """Post-processing of example/shock-vortex.jl:333-352: coordinates and (rho, U, V, 1/lambda)
+    of every solution point against the oracle's loop restatement (host copy and device output)."""
+    import torch
+
+    ps, u, s, orc = setup(12, 7, deg, "hll", "ghost")
+    s.set_state(u)
+    x, y, sol = s.primitives()
+    xo, yo, so = O.solution_field2d(u, ps.xpg, 1.4)
+    assert np.array_equal(x, xo) and np.array_equal(y, yo)
+    for m in range(4):
+        assert rel_inf(sol[..., m], so[..., m]) <= 1e-14
+    s.use_torch_stream()
+    d = torch.full((sol.size,), np.nan, dtype=torch.float64, device="cuda")
+    s.primitives(out=d)
+    torch.cuda.synchronize()
+    assert np.array_equal(d.cpu().numpy().reshape(sol.shape, order="F"), sol)
