@@ -424,42 +424,84 @@ int fr_bgk_moments(fr_bgk_ctx *c, double *w) {
   return FR_OK;
 }
 
-// Gauss-Hermite rule for the weight exp(-x^2): nodes ascending, weights (host, native)
+// Gauss-Hermite rule for the weight exp(-x^2): nodes ascending, weights (host, native).
+// The orthonormal Hermite functions p_j follow p_{j+1} = x sqrt(2/(j+1)) p_j - sqrt(j/(j+1)) p_{j-1},
+// p_0 = pi^(-1/4), with p_n' = sqrt(2n) p_{n-1}.  All n zeros lie in |x| < sqrt(2n+1); a sign scan
+// of p_n on a grid finer than the closest pair of zeros brackets each one, bisection narrows the
+// bracket and Newton polishes it; the weight is 2 / p_n'(x)^2.
+static void hermite_pn(int n, double x, double &pn, double &pn1) {
+  double p1 = 0.7511255444649425, p0 = 0.0;  // pi^(-1/4), p_{-1}
+  for (int j = 0; j < n; ++j) {
+    const double p2 = x * std::sqrt(2.0 / (j + 1)) * p1 - std::sqrt((double)j / (j + 1)) * p0;
+    p0 = p1;
+    p1 = p2;
+  }
+  pn = p1;
+  pn1 = p0;
+}
+
 int fr_gauss_hermite(int n, double *x, double *w) {
   FR_REQUIRE(n >= 1 && n <= 200 && x && w, FR_E_INVALID, "fr_gauss_hermite: bad arguments");
-  // Newton on the orthonormal Hermite recurrence from the standard asymptotic initial guesses
-  const double pim4 = 0.7511255444649425;  // pi^(-1/4)
-  const int m = (n + 1) / 2;
-  double z = 0.0, pp = 0.0;
-  for (int i = 0; i < m; ++i) {
-    if (i == 0)
-      z = std::sqrt(2.0 * n + 1.0) - 1.85575 * std::pow(2.0 * n + 1.0, -0.16667);
-    else if (i == 1)
-      z -= 1.14 * std::pow((double)n, 0.426) / z;
-    else if (i == 2)
-      z = 1.86 * z - 0.86 * x[n - 1];
-    else if (i == 3)
-      z = 1.91 * z - 0.91 * x[n - 2];
-    else
-      z = 2.0 * z - x[n - i + 1];
-    for (int it = 0; it < 100; ++it) {
-      double p1 = pim4, p2 = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const double p3 = p2;
-        p2 = p1;
-        p1 = z * std::sqrt(2.0 / (j + 1)) * p2 - std::sqrt((double)j / (j + 1)) * p3;
+  const double R = std::sqrt(2.0 * n + 1.0);
+  std::vector<double> lo, hi;
+  for (int cells = 64 * n; (int)lo.size() != n; cells *= 2) {
+    if (cells > (1 << 24)) return fail(FR_E_INVALID, "fr_gauss_hermite: zeros not separated");
+    lo.clear();
+    hi.clear();
+    double xa = -R, pa, q;
+    hermite_pn(n, xa, pa, q);
+    for (int c = 1; c <= cells; ++c) {
+      const double xb = -R + 2.0 * R * c / cells;
+      double pb;
+      hermite_pn(n, xb, pb, q);
+      if ((pa < 0.0) != (pb < 0.0) || pb == 0.0) {
+        lo.push_back(xa);
+        hi.push_back(xb);
+        if (pb == 0.0) {  // step past an exact zero so it is bracketed once
+          xa = xb + 1e-300;
+          hermite_pn(n, xa, pa, q);
+          continue;
+        }
       }
-      pp = std::sqrt(2.0 * n) * p2;
-      const double dz = p1 / pp;
-      z -= dz;
-      if (std::fabs(dz) < 1e-15) break;
+      xa = xb;
+      pa = pb;
     }
-    x[n - 1 - i] = z;
-    x[i] = -z;
-    w[n - 1 - i] = 2.0 / (pp * pp);
-    w[i] = w[n - 1 - i];
   }
-  if (n & 1) x[m - 1] = 0.0;
+  for (int i = 0; i < n; ++i) {
+    double a = lo[i], b = hi[i], pa, q;
+    hermite_pn(n, a, pa, q);
+    for (int it = 0; it < 200 && b - a > 1e-13 * std::max(1.0, std::fabs(a)); ++it) {
+      const double m = 0.5 * (a + b);
+      double pm;
+      hermite_pn(n, m, pm, q);
+      if ((pm < 0.0) == (pa < 0.0)) {
+        a = m;
+        pa = pm;
+      } else {
+        b = m;
+      }
+    }
+    double z = 0.5 * (a + b), d = 0.0;
+    for (int it = 0; it < 5; ++it) {
+      double pn, pn1;
+      hermite_pn(n, z, pn, pn1);
+      d = std::sqrt(2.0 * n) * pn1;
+      z -= pn / d;
+    }
+    double pn, pn1;
+    hermite_pn(n, z, pn, pn1);
+    d = std::sqrt(2.0 * n) * pn1;
+    x[i] = z;
+    w[i] = 2.0 / (d * d);
+  }
+  // symmetric rule: pair the nodes exactly
+  for (int i = 0; i < n / 2; ++i) {
+    const double z = 0.5 * (x[n - 1 - i] - x[i]), ww = 0.5 * (w[i] + w[n - 1 - i]);
+    x[i] = -z;
+    x[n - 1 - i] = z;
+    w[i] = w[n - 1 - i] = ww;
+  }
+  if (n & 1) x[n / 2] = 0.0;
   return FR_OK;
 }
 
