@@ -8,9 +8,10 @@
 //
 // State f(i, j, k, l, a, b, member), column-major: the cell index is fastest, so a wavefront of
 // consecutive cells reads contiguous memory for a fixed (point, velocity).  Two kernels per stage:
-// moments (one thread per solution point, loop over the velocities, FP64 sums) and transport +
-// relaxation + RK stage (one workgroup per (member, velocity, 16 x 16 cell tile), one thread per
-// cell; the upwind traces of the neighbours come from LDS or from the halo threads).
+// moments (one thread per solution point, loop over the velocities, FP64 sums; it also writes the
+// separable Maxwellian factors of the point) and transport + relaxation + RK stage (one workgroup
+// per (member, velocity, 16 x 16 cell tile), one thread per cell; the upwind traces of the
+// neighbours come from LDS or from the halo threads).
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -42,41 +43,103 @@ struct BgkStage {
   int residual;
 };
 
-// moments -> prim (rho, U, V, lambda) per solution point: prim[member][c][pt]
-__global__ __launch_bounds__(256) void bgk_moments_kernel(BgkGeo g, const double *__restrict__ f,
-                                                          const double *__restrict__ uv,
-                                                          const double *__restrict__ vv,
-                                                          const double *__restrict__ wu,
-                                                          const double *__restrict__ wv, double *prim) {
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= g.npts * g.nmem) return;
-  const long long mem = t / g.npts, p = t % g.npts;
-  const double *fm = f + mem * g.mstride + p;
-  double m0 = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
-  for (int b = 0; b < g.nv; ++b) {
-    const double vb = vv[b], wb = wv[b];
-    for (int a = 0; a < g.nu; ++a) {
-      const double fv = fm[g.npts * (a + (long long)g.nu * b)];
-      const double wt = wu[a] * wb;
-      const double ua = uv[a];
-      m0 += wt * fv;
-      m1 += wt * ua * fv;
-      m2 += wt * vb * fv;
-      m3 += wt * 0.5 * (ua * ua + vb * vb) * fv;
+// moments -> prim (rho, U, V, lambda) per solution point, prim[member][c][pt], and the separable
+// Maxwellian M(a, b) = Ea[a] * Eb[b] of every point: Ea[member][a][pt] = exp(-lambda (u_a - U)^2),
+// Eb[member][b][pt] = rho lambda / pi exp(-lambda (v_b - V)^2), so that the transport kernel spends
+// one multiplication per value on it.  A wavefront takes 64 consecutive points (the lanes); nq
+// wavefronts of the workgroup share a point group and split its velocity rows b between them (nq
+// > 1 when the points alone give the chip too few wavefronts).  The partial sums are combined in
+// LDS in a fixed order, and each wavefront then writes its share of the factors.
+constexpr int MQ = 4;  // wavefronts per workgroup of the moments kernel
+__global__ __launch_bounds__(64 * MQ) void bgk_moments_kernel(BgkGeo g, const double *__restrict__ f,
+                                                              const double *__restrict__ uv,
+                                                              const double *__restrict__ vv,
+                                                              const double *__restrict__ wu,
+                                                              const double *__restrict__ wv, double *prim,
+                                                              double *__restrict__ tab, int nq) {
+  __shared__ double sm[MQ][4][64];
+  __shared__ double sp[MQ][4][64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, q = w % nq, grp = w / nq;
+  const long long t = (blockIdx.x * (long long)(MQ / nq) + grp) * 64 + lane;
+  const bool act = t < g.npts * g.nmem;
+  const long long mem = act ? t / g.npts : 0, p = act ? t % g.npts : 0;
+  double m[4] = {0.0, 0.0, 0.0, 0.0};
+  if (act) {
+    const double *fm = f + mem * g.mstride + p;
+    double r4[4][4] = {};  // four independent chains per moment
+    const int b0 = q * g.nv / nq, b1 = (q + 1) * g.nv / nq;
+    for (int b = b0; b < b1; ++b) {
+      const double vb = vv[b], wb = wv[b];
+      const double *fb = fm + g.npts * (long long)g.nu * b;
+      int a = 0;
+      for (; a + 4 <= g.nu; a += 4) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double fv = fb[g.npts * (long long)(a + r)];
+          const double ua = uv[a + r], wt = wu[a + r] * wb;
+          r4[r][0] += wt * fv;
+          r4[r][1] += wt * ua * fv;
+          r4[r][2] += wt * vb * fv;
+          r4[r][3] += wt * 0.5 * (ua * ua + vb * vb) * fv;
+        }
+      }
+      for (; a < g.nu; ++a) {
+        const double fv = fb[g.npts * (long long)a];
+        const double ua = uv[a], wt = wu[a] * wb;
+        r4[0][0] += wt * fv;
+        r4[0][1] += wt * ua * fv;
+        r4[0][2] += wt * vb * fv;
+        r4[0][3] += wt * 0.5 * (ua * ua + vb * vb) * fv;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 4; ++c) m[c] = (r4[0][c] + r4[1][c]) + (r4[2][c] + r4[3][c]);
+  }
+#pragma unroll
+  for (int c = 0; c < 4; ++c) sm[w][c][lane] = m[c];
+  __syncthreads();
+  if (q == 0) {
+    double mm[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      double v = sm[w][c][lane];
+      for (int r = 1; r < nq; ++r) v += sm[w + r][c][lane];
+      mm[c] = v;
+    }
+    const double rho = mm[0], U = mm[1] / mm[0], V = mm[2] / mm[0];
+    const double lam = 0.5 * mm[0] / (g.gamma - 1.0) / (mm[3] - 0.5 * (mm[1] * mm[1] + mm[2] * mm[2]) / mm[0]);
+    sp[grp][0][lane] = rho;
+    sp[grp][1][lane] = U;
+    sp[grp][2][lane] = V;
+    sp[grp][3][lane] = lam;
+    if (act) {
+      double *pr = prim + mem * 4 * g.npts + p;
+      pr[0] = rho;
+      pr[g.npts] = U;
+      pr[2 * g.npts] = V;
+      pr[3 * g.npts] = lam;
     }
   }
-  double *pr = prim + mem * 4 * g.npts + p;
-  pr[0] = m0;
-  pr[g.npts] = m1 / m0;
-  pr[2 * g.npts] = m2 / m0;
-  pr[3 * g.npts] = 0.5 * m0 / (g.gamma - 1.0) / (m3 - 0.5 * (m1 * m1 + m2 * m2) / m0);
+  __syncthreads();
+  if (!act) return;
+  const double rho = sp[grp][0][lane], U = sp[grp][1][lane], V = sp[grp][2][lane], lam = sp[grp][3][lane];
+  double *ta = tab + mem * (long long)(g.nu + g.nv) * g.npts + p;
+  for (int a = q * g.nu / nq; a < (q + 1) * g.nu / nq; ++a) {
+    const double du = uv[a] - U;
+    ta[g.npts * (long long)a] = exp(-lam * (du * du));
+  }
+  const double c0 = rho * (lam / M_PI);
+  for (int b = q * g.nv / nq; b < (q + 1) * g.nv / nq; ++b) {
+    const double dv = vv[b] - V;
+    ta[g.npts * (long long)(g.nu + b)] = c0 * exp(-lam * (dv * dv));
+  }
 }
 
 template <int NSP>
 __global__ __launch_bounds__(BT *BT) void bgk_rhs_kernel(BgkGeo g, BgkOps<NSP> ops, const double *__restrict__ uv,
                                                          const double *__restrict__ vv,
                                                          const double *__restrict__ tau,
-                                                         const double *__restrict__ prim, BgkStage sa) {
+                                                         const double *__restrict__ tab, BgkStage sa) {
   __shared__ double sx[BT + 1][BT][NSP];  // x-face upwind trace, face fx between cells fx-1 and fx
   __shared__ double sy[BT + 1][BT][NSP];  // y-face upwind trace, face fy between rows fy-1 and fy
   const int tiles_x = (g.nx + BT - 1) / BT, tiles_y = (g.ny + BT - 1) / BT;
@@ -175,7 +238,8 @@ __global__ __launch_bounds__(BT *BT) void bgk_rhs_kernel(BgkGeo g, BgkOps<NSP> o
   __syncthreads();
   if (!act) return;
   const double itau = 1.0 / tau[mem];
-  const double *pr = prim + (long long)mem * 4 * g.npts;
+  const double *__restrict__ Ea = tab + (long long)mem * (g.nu + g.nv) * g.npts + g.npts * (long long)a;
+  const double *__restrict__ Eb = tab + (long long)mem * (g.nu + g.nv) * g.npts + g.npts * (long long)(g.nu + b);
   double *__restrict__ out = sa.out + vbase;
 #pragma unroll
   for (int l = 0; l < NSP; ++l)
@@ -193,9 +257,7 @@ __global__ __launch_bounds__(BT *BT) void bgk_rhs_kernel(BgkGeo g, BgkOps<NSP> o
         dfy += vb * w[k][q] * g.iJy * ops.dl[l + NSP * q];
       }
       const long long pt = (long long)i + (long long)g.nx * j + cells * (k + NSP * l);
-      const double rho = pr[pt], U = pr[pt + g.npts], V = pr[pt + 2 * g.npts], lam = pr[pt + 3 * g.npts];
-      const double du_ = ua - U, dv_ = vb - V;
-      const double M = rho * (lam / M_PI) * exp(-lam * (du_ * du_ + dv_ * dv_));
+      const double M = Eb[pt] * Ea[pt];
       const double fc = w[k][l];
       const double r = -(dfx + (FsL - fL[l]) * ops.dhl[k] + (FsR - fR[l]) * ops.dhr[k] + dfy +
                          (GsB - gB[k]) * ops.dhl[l] + (GsT - gT[k]) * ops.dhr[l]) +
@@ -221,6 +283,7 @@ struct BgkCtx {
   double *d_vel = nullptr;  // uv | vv | wu | wv
   double *d_tau = nullptr;
   double *d_prim = nullptr;
+  double *d_tab = nullptr;  // Maxwellian factors Ea | Eb per member
   double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_in = nullptr, *d_out = nullptr;
 
   BgkGeo geo() const {
@@ -252,17 +315,28 @@ struct BgkCtx {
     const long long tiles = (long long)((nx + BT - 1) / BT) * ((ny + BT - 1) / BT);
     const long long nb = tiles * nu * nv * nmem;
     hipLaunchKernelGGL(bgk_rhs_kernel<NSP>, dim3((unsigned)nb), dim3(BT * BT), 0, st, g, o, d_vel, d_vel + nu,
-                       d_tau, d_prim, sa);
+                       d_tau, d_tab, sa);
     FR_LAUNCHED("bgk_rhs_kernel");
     return FR_OK;
   }
 
-  int stage(const BgkStage &sa, hipStream_t st) const {
+  // moments kernel launch: nq wavefronts per group of 64 points, enough for about eight
+  // wavefronts per SIMD; MQ / nq point groups per workgroup
+  int launch_moments(const double *d_f, hipStream_t st) const {
     const BgkGeo g = geo();
-    const long long n = g.npts * nmem;
-    hipLaunchKernelGGL(bgk_moments_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, sa.u, d_vel,
-                       d_vel + nu, d_vel + nu + nv, d_vel + 2 * nu + nv, d_prim);
+    const long long n = g.npts * nmem, groups = (n + 63) / 64;
+    int nq = 1;
+    while (nq < MQ && groups * nq < 8LL * 1024) nq *= 2;
+    const long long nb = (groups + (MQ / nq) - 1) / (MQ / nq);
+    hipLaunchKernelGGL(bgk_moments_kernel, dim3((unsigned)nb), dim3(64 * MQ), 0, st, g, d_f, d_vel, d_vel + nu,
+                       d_vel + nu + nv, d_vel + 2 * nu + nv, d_prim, d_tab, nq);
     FR_LAUNCHED("bgk_moments_kernel");
+    return FR_OK;
+  }
+
+  int stage(const BgkStage &sa, hipStream_t st) const {
+    int rc0 = launch_moments(sa.u, st);
+    if (rc0) return rc0;
     switch (nsp) {
       case 2: return launch_rhs<2>(sa, st);
       case 3: return launch_rhs<3>(sa, st);
@@ -273,7 +347,7 @@ struct BgkCtx {
 
   void release() {
     if (stream) (void)hipStreamSynchronize(stream);
-    double *bufs[] = {d_vel, d_tau, d_prim, d_u, d_s1, d_s2, d_in, d_out};
+    double *bufs[] = {d_vel, d_tau, d_prim, d_tab, d_u, d_s1, d_s2, d_in, d_out};
     for (double *b : bufs)
       if (b) (void)hipFree(b);
     if (stream) (void)hipStreamDestroy(stream);
@@ -327,6 +401,7 @@ int fr_bgk_create(fr_bgk_ctx **out, const fr_bgk_desc *d) {
   if (e == hipSuccess) e = hipMalloc(&c->d_tau, d->nmem * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(c->d_tau, d->tau, d->nmem * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&c->d_prim, (size_t)g.npts * 4 * d->nmem * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->d_tab, (size_t)g.npts * (d->nu + d->nv) * d->nmem * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->d_u, c->len * sizeof(double));
   if (e != hipSuccess) {
     int rc = hip_fail(e, "fr_bgk_create", __FILE__, __LINE__);
@@ -404,9 +479,8 @@ int fr_bgk_moments(fr_bgk_ctx *c, double *w) {
   FR_REQUIRE(c && w, FR_E_INVALID, "fr_bgk_moments: null argument");
   const BgkGeo g = c->geo();
   const long long n = g.npts * c->nmem;
-  hipLaunchKernelGGL(bgk_moments_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, g, c->d_u,
-                     c->d_vel, c->d_vel + c->nu, c->d_vel + c->nu + c->nv, c->d_vel + 2 * c->nu + c->nv, c->d_prim);
-  FR_LAUNCHED("bgk_moments_kernel");
+  int rc = c->launch_moments(c->d_u, c->stream);
+  if (rc) return rc;
   std::vector<double> prim((size_t)n * 4);
   FR_HIP(hipMemcpyAsync(prim.data(), c->d_prim, prim.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   FR_HIP(hipStreamSynchronize(c->stream));
