@@ -435,10 +435,8 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
 #pragma unroll
       for (int k = 0; k < NSP; ++k) SU(line, k, m) = dyr[k][m];
   }
-  __syncthreads();  // B2: y contributions of every point visible
-
-  // ---------------------------------------------------------------- P4: assemble + epilogue
-  // the u0 row is read last, when the live set is smallest (4 workgroups per CU at nsp = 4)
+  // the u0 row is read here, when the live set is smallest (4 workgroups per CU at nsp = 4), and
+  // its loads are in flight during the wait at the barrier
   const bool with_u0 = EPI != EPI_RESIDUAL && sa.a0 != 0.0;
   double u0r[NSP][4];
   if (EPI != EPI_RESIDUAL && act && with_u0) {
@@ -447,6 +445,9 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
 #pragma unroll
       for (int k = 0; k < NSP; ++k) u0r[k][m] = LD(sa.u0, k, line, m, ebo);
   }
+  __syncthreads();  // B2: y contributions of every point visible
+
+  // ---------------------------------------------------------------- P4: assemble + epilogue
   if (act) {
 #pragma unroll
     for (int m = 0; m < 4; ++m)
