@@ -392,6 +392,36 @@ int fr2d_positivity(fr2d_ctx *c) {
   return FR_OK;
 }
 
+int fr2d_set_geometry(fr2d_ctx *c, const double *iJ, const double *n1, const double *n2) {
+  FR_REQUIRE(c && iJ, FR_E_INVALID, "fr2d_set_geometry: null argument");
+  // iJ column-major like Julia's 2x2 Matrix: iJ[0] = r_x, iJ[1] = s_x, iJ[2] = r_y, iJ[3] = s_y
+  const double a = iJ[0], b = iJ[2], cc = iJ[1], d = iJ[3];
+  for (int q = 0; q < 4; ++q) FR_REQUIRE(std::isfinite(iJ[q]), FR_E_INVALID, "fr2d_set_geometry: iJ not finite");
+  FR_REQUIRE(a * d - b * cc > 0.0, FR_E_INVALID, "fr2d_set_geometry: iJ must have a positive determinant");
+  double m1[2] = {a, b}, m2[2] = {cc, d};  // grad r, grad s: the default face normals
+  if (n1) m1[0] = n1[0], m1[1] = n1[1];
+  if (n2) m2[0] = n2[0], m2[1] = n2[1];
+  const double l1 = std::hypot(m1[0], m1[1]), l2 = std::hypot(m2[0], m2[1]);
+  FR_REQUIRE(std::isfinite(l1) && std::isfinite(l2) && l1 > 0.0 && l2 > 0.0, FR_E_INVALID,
+             "fr2d_set_geometry: face normals must be finite and non-zero");
+  c->geom = 1;
+  c->iJ11 = a;
+  c->iJ12 = b;
+  c->iJ21 = cc;
+  c->iJ22 = d;
+  c->n1[0] = m1[0] / l1;
+  c->n1[1] = m1[1] / l1;
+  c->n2[0] = m2[0] / l2;
+  c->n2[1] = m2[1] / l2;
+  // a captured step sequence holds the old geometry in its kernel arguments
+  if (c->gexec) {
+    FR_HIP(hipStreamSynchronize(stream_of(*c)));
+    FR_HIP(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  return FR_OK;
+}
+
 int fr2d_primitives_dev(fr2d_ctx *c, double *d_sol) {
   FR_REQUIRE(c && d_sol, FR_E_INVALID, "fr2d_primitives_dev: null argument");
   FR_REQUIRE(c->d_u, FR_E_INVALID, "fr2d_primitives_dev: no state");

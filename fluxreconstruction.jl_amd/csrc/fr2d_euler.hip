@@ -37,11 +37,18 @@ static_assert(2 * (TX + TY) <= 64, "the halo tasks of one line must fit in one w
 // instruction on addresses
 enum { ADDR_FLAT = 0, ADDR_BUFFER = 1 };
 
+// cell geometry: rectangles (iJ diagonal, faces normal to the axes) or uniform parallelograms
+// (full 2x2 iJ, faces solved in the frames of the unit normals n1, n2)
+enum { GEO_RECT = 0, GEO_AFFINE = 1 };
+
 struct Geo2D {
   int nx, ny, ntx, nty;
   long long ldp;  // elements per (k,l,m) plane: (nx+2)(ny+2)
   double iJ11, iJ22, g, gm1;
   int periodic;
+  double iJ12, iJ21;          // off-diagonal inverse-Jacobian entries (GEO_AFFINE)
+  double n1c, n1s, n2c, n2s;  // unit normals of the x faces and y faces (GEO_AFFINE)
+  double q1, q2;              // contravariant scales of the face fluxes, (iJ n1)_1 and (iJ n2)_2
 };
 
 // epilogues: residual du; RK stage out = a0*u0 + a1*u + b*du (b != 0); the general stage form
@@ -57,7 +64,7 @@ struct StageArgs {
   double a0, a1, b, b2;
 };
 
-template <int NSP, int FLUX, int EPI, int ADDR>
+template <int NSP, int FLUX, int EPI, int ADDR, int GEO>
 __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const StageArgs sa) {
   // operators are compile-time constants (Gauss-Legendre points, Radau correction), so they
   // occupy no scalar registers for the length of the kernel
@@ -78,9 +85,12 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
   const double *u = sa.u;
   const double g = geo.g, gm1 = geo.gm1;
   // scale of the x and y contributions in the result: -iJ (residual, RK4 form) or -b*iJ (stage,
-  // b != 0 on the host); the stage carries a1*u inside the x part as (a1 / sxs)*u
-  const double sxs = EPI == EPI_STAGE ? -sa.b * geo.iJ11 : -geo.iJ11;
-  const double sys = EPI == EPI_STAGE ? -sa.b * geo.iJ22 : -geo.iJ22;
+  // b != 0 on the host); the stage carries a1*u inside the x part as (a1 / sxs)*u.  Parallelogram
+  // cells carry the metric inside the contravariant fluxes and the face scales q1, q2 instead.
+  constexpr bool AFF = GEO == GEO_AFFINE;
+  const double bsc = EPI == EPI_STAGE ? sa.b : 1.0;
+  const double sxs = AFF ? -bsc : -bsc * geo.iJ11;
+  const double sys = AFF ? -bsc : -bsc * geo.iJ22;
   const double ua = EPI == EPI_STAGE ? sa.a1 / sxs : 0.0;
 
   auto SU = [&](int k, int l, int m) -> double & { return sU[((k * NSP + l) * 4 + m) * TE + lane]; };
@@ -260,8 +270,15 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
     for (int k = 0; k < NSP; ++k) {
       XState xs;
       xstate(ur[k], gm1, xs);
+      if constexpr (AFF) {  // contravariant flux iJ11 F + iJ12 G
+        const double v = ur[k][2] * xs.rinv;
+        const double G[4] = {ur[k][2], ur[k][1] * v, ur[k][2] * v + xs.p, (ur[k][3] + xs.p) * v};
 #pragma unroll
-      for (int m = 0; m < 4; ++m) f1[k][m] = xs.F[m];
+        for (int m = 0; m < 4; ++m) f1[k][m] = geo.iJ11 * xs.F[m] + geo.iJ12 * G[m];
+      } else {
+#pragma unroll
+        for (int m = 0; m < 4; ++m) f1[k][m] = xs.F[m];
+      }
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -291,7 +308,13 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
     const int s = ex > 0 ? ex - 1 : 0;
 #pragma unroll
     for (int m = 0; m < 4; ++m) wl[m] = ex > 0 ? SX(m, ey, s) : htr[m];
-    riemann_x4<FLUX>(wl, uLx, g, gm1, cl);
+    if constexpr (AFF) {
+      riemann_n4<FLUX>(wl, uLx, geo.n1c, geo.n1s, g, gm1, cl);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) cl[m] *= geo.q1;
+    } else {
+      riemann_x4<FLUX>(wl, uLx, g, gm1, cl);
+    }
     if (ex > 0) {
 #pragma unroll
       for (int m = 0; m < 4; ++m) SX(m, ey, s) = cl[m];
@@ -357,6 +380,12 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
       f2[q][1] = uc[q][1] * vv;
       f2[q][2] = uc[q][2] * vv + p;
       f2[q][3] = (uc[q][3] + p) * vv;
+      if constexpr (AFF) {  // contravariant flux iJ21 F + iJ22 G
+        const double uu = uc[q][1] * r;
+        const double F[4] = {uc[q][1], uc[q][1] * uu + p, uc[q][2] * uu, (uc[q][3] + p) * uu};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) f2[q][m] = geo.iJ21 * F[m] + geo.iJ22 * f2[q][m];
+      }
     }
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -378,7 +407,13 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
     double wb[4], fw[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) wb[m] = SY(m, ey, ex);
-    riemann_y4<FLUX>(wb, uBy, g, gm1, fw);
+    if constexpr (AFF) {
+      riemann_n4<FLUX>(wb, uBy, geo.n2c, geo.n2s, g, gm1, fw);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) fw[m] *= geo.q2;
+    } else {
+      riemann_y4<FLUX>(wb, uBy, g, gm1, fw);
+    }
 #pragma unroll
     for (int m = 0; m < 4; ++m) SY(m, ey, ex) = fw[m];
   }
@@ -390,22 +425,30 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
     double *slot[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) slot[m] = xf ? &SX(m, hpos, ncx - 1) : &SY(m, ncy, hpos);
-    double w[4], a[4], b[4], fw[4];
+    double w[4], fw[4];
 #pragma unroll
     for (int m = 0; m < 4; ++m) w[m] = *slot[m];
-    a[0] = w[0];
-    a[1] = xf ? w[1] : w[2];
-    a[2] = xf ? w[2] : -w[1];
-    a[3] = w[3];
-    b[0] = htr[0];
-    b[1] = xf ? htr[1] : htr[2];
-    b[2] = xf ? htr[2] : -htr[1];
-    b[3] = htr[3];
-    riemann_x4<FLUX>(a, b, g, gm1, fw);
-    *slot[0] = fw[0];
-    *slot[1] = xf ? fw[1] : -fw[2];
-    *slot[2] = xf ? fw[2] : fw[1];
-    *slot[3] = fw[3];
+    if constexpr (AFF) {
+      const double c = xf ? geo.n1c : geo.n2c, sn = xf ? geo.n1s : geo.n2s, q = xf ? geo.q1 : geo.q2;
+      riemann_n4<FLUX>(w, htr, c, sn, g, gm1, fw);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) *slot[m] = q * fw[m];
+    } else {
+      double a[4], b[4];
+      a[0] = w[0];
+      a[1] = xf ? w[1] : w[2];
+      a[2] = xf ? w[2] : -w[1];
+      a[3] = w[3];
+      b[0] = htr[0];
+      b[1] = xf ? htr[1] : htr[2];
+      b[2] = xf ? htr[2] : -htr[1];
+      b[3] = htr[3];
+      riemann_x4<FLUX>(a, b, g, gm1, fw);
+      *slot[0] = fw[0];
+      *slot[1] = xf ? fw[1] : -fw[2];
+      *slot[2] = xf ? fw[2] : fw[1];
+      *slot[3] = fw[3];
+    }
   }
   wave_lds_sync();
 
@@ -513,6 +556,7 @@ __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, long long ldp,
 // ------------------------------------------------------------------ launch dispatch
 template <int NSP, int FLUX, int EPI>
 static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
+  const bool aff = c.geom != 0;
   Geo2D geo;
   geo.nx = c.nx;
   geo.ny = c.ny;
@@ -524,11 +568,24 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   geo.g = c.gamma;
   geo.gm1 = c.gamma - 1.0;
   geo.periodic = c.bc == FR_BC_PERIODIC;
+  geo.iJ12 = c.iJ12;
+  geo.iJ21 = c.iJ21;
+  geo.n1c = c.n1[0];
+  geo.n1s = c.n1[1];
+  geo.n2c = c.n2[0];
+  geo.n2s = c.n2[1];
+  geo.q1 = c.iJ11 * c.n1[0] + c.iJ12 * c.n1[1];
+  geo.q2 = c.iJ21 * c.n2[0] + c.iJ22 * c.n2[1];
   const unsigned nb = (unsigned)(geo.ntx * geo.nty);  // one tile per workgroup
-  if (c.len * sizeof(double) < (size_t)0xFFFFFFFFu)
-    hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_BUFFER>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
+  const bool buf = c.len * sizeof(double) < (size_t)0xFFFFFFFFu;
+  if (buf && !aff)
+    hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_BUFFER, GEO_RECT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
+  else if (!aff)
+    hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_FLAT, GEO_RECT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
+  else if (buf)
+    hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_BUFFER, GEO_AFFINE>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
   else
-    hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_FLAT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
+    hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_FLAT, GEO_AFFINE>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
   FR_LAUNCHED("fr2d_kernel");
   return FR_OK;
 }

@@ -8,6 +8,11 @@ namespace fr {
 struct Fr2dCtx {
   int nx = 0, ny = 0, deg = 0, nsp = 0, flux = 0, bc = 0;
   double gamma = 1.4, iJ11 = 1.0, iJ22 = 1.0;
+  // cell geometry: 0 rectangles (iJ11, iJ22); 1 uniform parallelograms with the full inverse
+  // Jacobian and the unit normals of the x faces (n1) and y faces (n2)
+  int geom = 0;
+  double iJ12 = 0.0, iJ21 = 0.0;
+  double n1[2] = {1.0, 0.0}, n2[2] = {0.0, 1.0};
   HostOps ops;
   size_t len = 0;  // doubles per state array
   int device = 0;

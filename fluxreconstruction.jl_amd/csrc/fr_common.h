@@ -243,6 +243,22 @@ __device__ __forceinline__ double prim_lambda(const double *w, double g) {
 
 // y-direction interface flux: rotate into the face-normal frame n = (0, 1), solve, rotate back
 // (KitBase local_frame / global_frame with cos = 0, sin = 1; example/shock-vortex.jl:84-87).
+// Riemann flux across a face with unit normal n = (c, s): the states go to the face frame
+// (rho, m.n, m.t, E) with t = (-s, c), the x-direction solver runs there and the flux comes back
+// as (f0, c f1 - s f2, s f1 + c f2, f3) (KitBase local_frame / global_frame)
+template <int FLUX>
+__device__ __forceinline__ void riemann_n4(const double *wL, const double *wR, double c, double s, double g,
+                                           double gm1, double *fw) {
+  const double a[4] = {wL[0], c * wL[1] + s * wL[2], c * wL[2] - s * wL[1], wL[3]};
+  const double b[4] = {wR[0], c * wR[1] + s * wR[2], c * wR[2] - s * wR[1], wR[3]};
+  double f[4];
+  riemann_x4<FLUX>(a, b, g, gm1, f);
+  fw[0] = f[0];
+  fw[1] = c * f[1] - s * f[2];
+  fw[2] = s * f[1] + c * f[2];
+  fw[3] = f[3];
+}
+
 template <int FLUX>
 __device__ __forceinline__ void riemann_y4(const double *wB, const double *wT, double g, double gm1,
                                            double *fw) {

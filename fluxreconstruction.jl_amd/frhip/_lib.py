@@ -113,6 +113,7 @@ SIGNATURES = {
     "fr2d_filter_refresh_limit": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _ip]),
     "fr2d_check_state": (C.c_int, [_vp]),
     "fr2d_primitives": (C.c_int, [_vp, _dp]),
+    "fr2d_set_geometry": (C.c_int, [_vp, _dp, _dp, _dp]),
     "fr2d_primitives_dev": (C.c_int, [_vp, _vp]),
 }
 

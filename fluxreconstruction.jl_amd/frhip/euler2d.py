@@ -48,13 +48,27 @@ class FREuler2D:
         d.gamma = self.gamma
         d.flux = _lib.FLUX[flux]
         d.bc = _lib.BC[bc]
-        d.iJ11 = float(ps.iJ_elem[0, 0])
-        d.iJ22 = float(ps.iJ_elem[1, 1])
+        iJ = np.asarray(ps.iJ_elem, dtype=np.float64)
+        rect = iJ[0, 1] == 0.0 and iJ[1, 0] == 0.0
+        d.iJ11 = float(iJ[0, 0]) if rect else 1.0
+        d.iJ22 = float(iJ[1, 1]) if rect else 1.0
         d.ll, d.lr, d.dl, d.dhl, d.dhr, d.V, d.iV = [dptr(a) for a in self._ops]
         h = C.c_void_p()
         check(_lib.lib().fr2d_create(C.byref(h), C.byref(d)))
         self.h = h
         self.len = _lib.lib().fr2d_state_len(h)
+        if not rect:
+            self.set_geometry(iJ)
+
+    def set_geometry(self, iJ, n1=None, n2=None):
+        """Uniform parallelogram cells (dev/parallelogram.jl:78-162): iJ is the 2x2 inverse Jacobian
+        [[r_x, r_y], [s_x, s_y]] of every cell, n1 / n2 the normals of the x / y faces (None:
+        grad r / grad s).  Fluxes are solved in the face frames and scaled by (iJ n)."""
+        iJ = np.asarray(iJ, dtype=np.float64).reshape(2, 2)
+        arrs = [np.ascontiguousarray(iJ.ravel(order="F"))]
+        arrs += [None if n is None else np.ascontiguousarray(n, dtype=np.float64).reshape(2) for n in (n1, n2)]
+        self._geom = arrs  # kept alive for the duration of the call only; the library copies them
+        check(_lib.lib().fr2d_set_geometry(self.h, *[None if a is None else dptr(a) for a in arrs]))
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

@@ -155,6 +155,14 @@ int fr2d_filter_refresh_limit(fr2d_ctx *ctx, int mode, double lambda, double S0,
                               int *n_filtered);
 /* Scan the device state for NaN/Inf (FR_E_NONFINITE) or rho <= 0 / p <= 0 (FR_E_NEGATIVE_STATE). */
 int fr2d_check_state(fr2d_ctx *ctx);
+/* Uniform parallelogram cells (dev/parallelogram.jl:78-162): iJ is the 2x2 inverse Jacobian of
+ * every cell, column-major like Julia's Matrix (iJ[0] = r_x, iJ[1] = s_x, iJ[2] = r_y, iJ[3] = s_y),
+ * with a positive determinant; n1 / n2 are the normals of the x faces (r = const) and y faces
+ * (s = const), normalised here, or NULL for grad r / grad s.  The volume flux becomes
+ * iJ [F, G], the face fluxes are solved in the frames of n1 / n2 (KitBase local_frame /
+ * global_frame) and scaled by (iJ n1)_1 / (iJ n2)_2.  Without this call the cells are the
+ * rectangles of example/shock-vortex.jl (iJ11, iJ22 of fr2d_desc). */
+int fr2d_set_geometry(fr2d_ctx *ctx, const double *iJ, const double *n1, const double *n2);
 /* Primitive variables of every solution point on the global point grid, the post-processing block
  * of example/shock-vortex.jl:333-352 (its `sol`): column-major (nx*nsp, ny*nsp, 4) with
  * sol[(i-1)*nsp + k, (j-1)*nsp + l, :] = (rho, U, V, 1/lambda), lambda = rho/(2p) as in KitBase

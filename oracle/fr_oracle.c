@@ -530,20 +530,35 @@ static void or_fr2d_rhs_ghost(const or_fr2d *d, const double *u, double *du) {
           }
         }
 
+  /* face normals and the contravariant scale of the common fluxes: shock-vortex.jl uses
+   * n1 = (1, 0), n2 = (0, 1) with iJ11 / iJ22 (:73-115); parallelogram.jl rotates both face
+   * families and scales by (iJ n1)_1, (iJ n2)_2 (dev/parallelogram.jl:116-162) */
+  const double c1 = d->geom ? d->n1[0] : 1.0, s1 = d->geom ? d->n1[1] : 0.0;
+  const double c2 = d->geom ? d->n2[0] : 0.0, s2 = d->geom ? d->n2[1] : 1.0;
+  const double q1 = d->geom ? iJ11 * c1 + iJ12 * s1 : iJ11;
+  const double q2 = d->geom ? iJ21 * c2 + iJ22 * s2 : iJ22;
+
   /* x-faces: flux between (i-1, j) right trace and (i, j) left trace (:73-80) */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < nsp; ++k)
     for (int j = 1; j <= ny; ++j)
       for (int i = 1; i <= nx + 1; ++i) {
-        double wL[4], wR[4], fw[4];
+        double a[4], b[4], wL[4], wR[4], fw[4], fg[4];
         for (int m = 0; m < 4; ++m) {
-          wL[m] = UF(i - 1, j, 1, k, m);
-          wR[m] = UF(i, j, 3, k, m);
+          a[m] = UF(i - 1, j, 1, k, m);
+          b[m] = UF(i, j, 3, k, m);
         }
-        or_riemann2(d->flux, wL, wR, g, fw);
-        for (int m = 0; m < 4; ++m) FX(i, j, k, m) = fw[m];
+        if (d->geom) {
+          or_local_frame2(a, c1, s1, wL);
+          or_local_frame2(b, c1, s1, wR);
+          or_riemann2(d->flux, wL, wR, g, fw);
+          or_global_frame2(fw, c1, s1, fg);
+        } else {
+          or_riemann2(d->flux, a, b, g, fg);
+        }
+        for (int m = 0; m < 4; ++m) FX(i, j, k, m) = fg[m];
       }
-  /* y-faces in the face-normal frame n = (0, 1) (:81-89) */
+  /* y-faces in the face-normal frame n2 (:81-89) */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int k = 0; k < nsp; ++k)
     for (int j = 1; j <= ny + 1; ++j)
@@ -553,10 +568,10 @@ static void or_fr2d_rhs_ghost(const or_fr2d *d, const double *u, double *du) {
           a[m] = UF(i, j - 1, 2, k, m);
           b[m] = UF(i, j, 0, k, m);
         }
-        or_local_frame2(a, 0.0, 1.0, wL);
-        or_local_frame2(b, 0.0, 1.0, wR);
+        or_local_frame2(a, c2, s2, wL);
+        or_local_frame2(b, c2, s2, wR);
         or_riemann2(d->flux, wL, wR, g, fw);
-        or_global_frame2(fw, 0.0, 1.0, fg);
+        or_global_frame2(fw, c2, s2, fg);
         for (int m = 0; m < 4; ++m) FY(i, j, k, m) = fg[m];
       }
 
@@ -576,7 +591,8 @@ static void or_fr2d_rhs_ghost(const or_fr2d *d, const double *u, double *du) {
             R5(rhs2, i, j, k, l, m) = s2;
           }
 
-  /* correction assembly (:102-115); iJ diagonal entries at the solution point */
+  /* correction assembly (:102-115); the common fluxes scaled by q1 = iJ11, q2 = iJ22 in the
+   * shock-vortex form */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int m = 0; m < 4; ++m)
     for (int l = 0; l < nsp; ++l)
@@ -585,10 +601,10 @@ static void or_fr2d_rhs_ghost(const or_fr2d *d, const double *u, double *du) {
           for (int i = 1; i <= nx; ++i) {
             U5(du, i, j, k, l, m) =
                 -(R5(rhs1, i, j, k, l, m) + R5(rhs2, i, j, k, l, m) +
-                  (FX(i, j, l, m) * iJ11 - FF(i, j, 3, l, m, 0)) * dhl[k] +
-                  (FX(i + 1, j, l, m) * iJ11 - FF(i, j, 1, l, m, 0)) * dhr[k] +
-                  (FY(i, j, k, m) * iJ22 - FF(i, j, 0, k, m, 1)) * dhl[l] +
-                  (FY(i, j + 1, k, m) * iJ22 - FF(i, j, 2, k, m, 1)) * dhr[l]);
+                  (FX(i, j, l, m) * q1 - FF(i, j, 3, l, m, 0)) * dhl[k] +
+                  (FX(i + 1, j, l, m) * q1 - FF(i, j, 1, l, m, 0)) * dhr[k] +
+                  (FY(i, j, k, m) * q2 - FF(i, j, 0, k, m, 1)) * dhl[l] +
+                  (FY(i, j + 1, k, m) * q2 - FF(i, j, 2, k, m, 1)) * dhr[l]);
           }
 
   free(f);

@@ -68,6 +68,12 @@ typedef struct {
   int bc;   /* OR_BC_* */
   double iJ[4]; /* uniform inverse Jacobian, 2x2 column-major: iJ11, iJ21, iJ12, iJ22 */
   const double *ll, *lr, *dl, *dhl, *dhr;
+  /* geom 0: interface terms of example/shock-vortex.jl:73-115 (x faces unrotated, y faces in the
+   * frame n = (0, 1), scaled by iJ11 / iJ22).  geom 1: those of dev/parallelogram.jl:116-162 (x
+   * faces in the frame n1, y faces in the frame n2, scaled by (iJ n1)_1 / (iJ n2)_2); with
+   * n1 = (1, 0), n2 = (0, 1) the two agree bitwise. */
+  int geom;
+  double n1[2], n2[2];
 } or_fr2d;
 
 size_t or_fr2d_len(const or_fr2d *d);

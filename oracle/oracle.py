@@ -243,13 +243,22 @@ class OrFr2d(C.Structure):
         ("dl", _dp),
         ("dhl", _dp),
         ("dhr", _dp),
+        ("geom", C.c_int),
+        ("n1", C.c_double * 2),
+        ("n2", C.c_double * 2),
     ]
 
 
 class Fr2d:
-    """Oracle 2D Euler residual on a uniform rectangular grid (ghost layer of one cell)."""
+    """Oracle 2D Euler residual on a uniform grid (ghost layer of one cell).
 
-    def __init__(self, nx, ny, deg, dx, dy, gamma=1.4, flux="hll", bc="ghost", ops: Ops | None = None):
+    Rectangular cells (example/shock-vortex.jl:26-118) by default.  With `iJ` (the uniform 2x2
+    inverse Jacobian [[r_x, r_y], [s_x, s_y]] of parallelogram cells) and the face normals n1, n2,
+    the interface terms of dev/parallelogram.jl:116-162 (fluxes solved in the face-normal frames,
+    scaled by (iJ n1)_1 and (iJ n2)_2)."""
+
+    def __init__(self, nx, ny, deg, dx, dy, gamma=1.4, flux="hll", bc="ghost", ops: Ops | None = None,
+                 iJ=None, n1=None, n2=None):
         self.ops = ops or Ops(deg)
         self.nx, self.ny, self.nsp = nx, ny, deg + 1
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
@@ -259,10 +268,21 @@ class Fr2d:
         d.gamma = gamma
         d.flux = FLUX[flux]
         d.bc = BC[bc]
-        d.iJ[0] = 1.0 / (dx / 2.0)
-        d.iJ[1] = 0.0
-        d.iJ[2] = 0.0
-        d.iJ[3] = 1.0 / (dy / 2.0)
+        if iJ is None:
+            d.iJ[0] = 1.0 / (dx / 2.0)
+            d.iJ[1] = 0.0
+            d.iJ[2] = 0.0
+            d.iJ[3] = 1.0 / (dy / 2.0)
+        else:
+            iJ = np.asarray(iJ, dtype=np.float64)
+            d.iJ[0], d.iJ[1], d.iJ[2], d.iJ[3] = iJ[0, 0], iJ[1, 0], iJ[0, 1], iJ[1, 1]
+        d.geom = 0
+        if iJ is not None or n1 is not None or n2 is not None:
+            d.geom = 1
+            n1 = (1.0, 0.0) if n1 is None else n1
+            n2 = (0.0, 1.0) if n2 is None else n2
+            d.n1[0], d.n1[1] = float(n1[0]), float(n1[1])
+            d.n2[0], d.n2[1] = float(n2[0]), float(n2[1])
         d.ll, d.lr, d.dl, d.dhl, d.dhr = [_p(a) for a in self._keep]
         self.d = d
         self.shape = (nx + 2, ny + 2, deg + 1, deg + 1, 4)

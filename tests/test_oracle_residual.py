@@ -272,3 +272,65 @@ def test_positivity_limiter_1d_euler():
     assert p.min() >= -1e-12
     # the squeeze is the largest admissible one: the worst point sits on p = eps (up to round-off)
     assert abs(p.min()) < 1e-10
+
+
+# ----------------------------------------------------------------- parallelogram cells
+def _rot(u, c, s):
+    """Momentum components rotated by (c, s): (rho, c m_x - s m_y, s m_x + c m_y, E)."""
+    v = u.copy(order="F")
+    v[..., 1] = c * u[..., 1] - s * u[..., 2]
+    v[..., 2] = s * u[..., 1] + c * u[..., 2]
+    return v
+
+
+def _vortex_state(nx, ny, deg, seed):
+    from frhip import ics
+    import frhip
+
+    ps = frhip.FRPSpace2D(0.0, 1.0, nx, 0.0, 1.0, ny, deg, 1, 1)
+    u = ics.isentropic_vortex(ps, 1.4, strength=3.0, center=(0.5, 0.5))
+    return ics.perturbed(u, seed=seed, amplitude=0.01)
+
+
+def test_parallelogram_form_reduces_to_rectangles():
+    """The interface terms of dev/parallelogram.jl with iJ = diag(2/dx, 2/dy), n1 = (1, 0),
+    n2 = (0, 1) are those of example/shock-vortex.jl, bitwise."""
+    nx, ny, deg = 6, 5, 2
+    dx, dy = 1.0 / nx, 1.0 / ny
+    u = _vortex_state(nx, ny, deg, 3)
+    for flux in ("hll", "rusanov", "roe"):
+        a = O.Fr2d(nx, ny, deg, dx, dy, flux=flux, bc="periodic").rhs(u)
+        b = O.Fr2d(nx, ny, deg, dx, dy, flux=flux, bc="periodic", iJ=[[2 / dx, 0.0], [0.0, 2 / dy]],
+                   n1=(1.0, 0.0), n2=(0.0, 1.0)).rhs(u)
+        assert np.array_equal(a, b)
+
+
+def test_rotated_grid_is_the_rotated_problem():
+    """Euler is rotation invariant: the same logical grid turned by theta (iJ = D^-1 R^T, face
+    normals R e1, R e2) with the momenta turned by R gives the turned residual.  A dropped
+    off-diagonal iJ term, a wrong frame sign or a transposed iJ breaks this."""
+    nx, ny, deg = 6, 5, 3
+    dx, dy = 1.0 / nx, 0.7 / ny
+    u = _vortex_state(nx, ny, deg, 4)
+    for theta in (0.3, -1.1, 2.5):
+        c, s = np.cos(theta), np.sin(theta)
+        iJ = np.array([[2 * c / dx, 2 * s / dx], [-2 * s / dy, 2 * c / dy]])
+        for flux in ("hll", "rusanov", "roe"):
+            du = O.Fr2d(nx, ny, deg, dx, dy, flux=flux, bc="periodic").rhs(u)
+            dur = O.Fr2d(nx, ny, deg, dx, dy, flux=flux, bc="periodic", iJ=iJ, n1=(c, s), n2=(-s, c)).rhs(_rot(u, c, s))
+            ref = _rot(du, c, s)
+            assert np.max(np.abs(dur - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+def test_parallelogram_free_stream():
+    """A uniform state on sheared periodic cells (J = [a/2, b/2], a = (dx, 0), b = (t, dy)) has
+    zero residual to round-off."""
+    nx, ny, deg = 5, 4, 3
+    dx, dy, t = 0.2, 0.25, 0.13
+    iJ = np.linalg.inv(np.array([[dx / 2, t / 2], [0.0, dy / 2]]))
+    u = np.zeros((nx + 2, ny + 2, deg + 1, deg + 1, 4), order="F")
+    u[...] = O.prim_conserve2([1.2, 0.7, -0.4, 0.9], 1.4)
+    for flux in ("hll", "rusanov", "roe"):
+        du = O.Fr2d(nx, ny, deg, dx, dy, flux=flux, bc="periodic", iJ=iJ,
+                    n1=iJ[0] / np.linalg.norm(iJ[0]), n2=iJ[1] / np.linalg.norm(iJ[1])).rhs(u)
+        assert np.max(np.abs(du)) <= 1e-12 * np.max(np.abs(iJ)) * np.max(np.abs(u))
