@@ -24,7 +24,7 @@ DissOps fr2d_diss_ops(const Fr2dCtx &c) {
   return o;
 }
 
-static hipStream_t stream_of(const Fr2dCtx &c) { return c.stream ? c.stream : c.own_stream; }
+static hipStream_t stream_of(const Fr2dCtx &c) { return fr2d_stream(c); }
 
 static int ensure_state(Fr2dCtx &c) {
   if (!c.d_u) {
@@ -197,7 +197,13 @@ int fr2d_destroy(fr2d_ctx *c) {
 
 int fr2d_set_stream(fr2d_ctx *c, void *s) {
   FR_REQUIRE(c, FR_E_INVALID, "fr2d_set_stream: null context");
-  c->stream = (hipStream_t)s;
+  if (s == FR_OWN_STREAM) {
+    c->caller_stream = false;
+    c->stream = nullptr;
+  } else {
+    c->caller_stream = true;
+    c->stream = (hipStream_t)s;
+  }
   return FR_OK;
 }
 

@@ -17,6 +17,9 @@ struct Fr2dCtx {
   size_t len = 0;  // doubles per state array
   int device = 0;
   int num_cu = 0;  // compute units of the device
+  // the caller's stream (caller_stream: any hipStream_t, NULL being HIP's null stream) or the
+  // context's own non-blocking stream
+  bool caller_stream = false;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
   // device state and stage buffers of the integrator path
@@ -37,6 +40,8 @@ struct Fr2dCtx {
   int g_scheme = -1, g_nsteps = 0;
   double g_dt = 0.0;
 };
+
+inline hipStream_t fr2d_stream(const Fr2dCtx &c) { return c.caller_stream ? c.stream : c.own_stream; }
 
 int fr2d_launch_residual(const Fr2dCtx &c, const double *d_u, double *d_du, hipStream_t st);
 int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, double *d_out, double a0,

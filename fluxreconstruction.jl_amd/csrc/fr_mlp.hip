@@ -553,7 +553,7 @@ int fr2d_sensor_features(fr2d_ctx *c, uint16_t *d_x) {
   FR_REQUIRE(x->nsp == 4, FR_E_UNSUPPORTED, "fr2d_sensor_features: defined for p = 3");
   const long long ldp = (long long)(x->nx + 2) * (x->ny + 2);
   const DissOps o = fr2d_diss_ops(*x);
-  hipStream_t st = x->stream ? x->stream : x->own_stream;
+  hipStream_t st = fr2d_stream(*x);
   hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)((ldp + 255) / 256)), dim3(256), 0, st, x->d_u, (int)ldp, ldp,
                      o.iV, (unsigned short *)d_x);
   FR_LAUNCHED("mlp_features_kernel");

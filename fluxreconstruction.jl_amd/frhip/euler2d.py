@@ -83,11 +83,16 @@ class FREuler2D:
 
     # ---------------------------------------------------------------- streams
     def use_torch_stream(self, stream=None):
-        """Launch on a torch CUDA/HIP stream (default: torch's current stream)."""
+        """Launch on a torch CUDA/HIP stream (default: torch's current stream; its default stream
+        is HIP's null stream, handle 0)."""
         import torch
 
         s = stream if stream is not None else torch.cuda.current_stream()
         check(_lib.lib().fr2d_set_stream(self.h, C.c_void_p(s.cuda_stream)))
+
+    def use_own_stream(self):
+        """Launch on the context's own non-blocking stream again (the default)."""
+        check(_lib.lib().fr2d_set_stream(self.h, C.c_void_p(2 ** (8 * C.sizeof(C.c_void_p)) - 1)))
 
     def stream_handle(self) -> int:
         return _lib.lib().fr2d_get_stream(self.h) or 0

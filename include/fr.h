@@ -102,7 +102,9 @@ typedef struct {
  * all scratch and a device copy of the state for the stepping entry points). */
 int fr2d_create(fr2d_ctx **out, const fr2d_desc *desc);
 int fr2d_destroy(fr2d_ctx *ctx);
-/* run on a caller-provided hipStream_t (NULL = the context's own stream) */
+/* Run on a caller-provided hipStream_t (NULL is HIP's null stream), or on the context's own
+ * non-blocking stream again with FR_OWN_STREAM (the default after fr2d_create). */
+#define FR_OWN_STREAM ((void *)~(uintptr_t)0)
 int fr2d_set_stream(fr2d_ctx *ctx, void *hip_stream);
 void *fr2d_get_stream(fr2d_ctx *ctx);
 /* number of doubles in one state array: (nx+2)(ny+2) nsp^2 4 */

@@ -318,3 +318,69 @@ def test_set_geometry_rejects_bad_input():
         s.set_geometry([[1.0, 2.0], [2.0, 1.0]])  # negative determinant
     with pytest.raises(frhip.FRError):
         s.set_geometry([[1.0, 0.0], [0.0, 1.0]], n1=(0.0, 0.0))
+
+
+def test_flat_addressing_large_grid():
+    """State arrays of 4 GiB and more take the kernel's flat 64-bit addressing instead of buffer
+    resources.  A 32 x 32 periodic vortex tiled 91 x 91 times (2912^2 cells, 4.35 GB per state)
+    must give the tiled residual of the small problem bitwise: the arithmetic per element is the
+    same, only the addressing differs."""
+    import types
+
+    import torch
+
+    n0, K, deg = 32, 91, 3
+    ps, u, s, _ = setup(n0, n0, deg, "rusanov", "periodic")
+    du_small = s.rhs(u)
+    N = n0 * K
+    big_ps = types.SimpleNamespace(nx=N, ny=N, ngx=1, ngy=1,
+                                   **{k: getattr(ps, k) for k in ("deg", "ll", "lr", "dl", "dhl", "dhr", "V", "iV",
+                                                                  "iJ_elem")})
+    sb = frhip.FREuler2D(big_ps, 1.4, "rusanov", "periodic")
+    assert sb.len * 8 >= 2 ** 32
+    # the NaN fill and the launch share torch's default stream (HIP's null stream), so the
+    # launch is ordered after the fill
+    n = deg + 1
+    small = torch.from_numpy(u.ravel(order="F").copy()).cuda().view(4, n, n, n0 + 2, n0 + 2)
+    big = torch.zeros((4, n, n, N + 2, N + 2), dtype=torch.float64, device="cuda")
+    big[:, :, :, 1:-1, 1:-1] = small[:, :, :, 1:-1, 1:-1].repeat(1, 1, 1, K, K)
+    dbig = torch.full_like(big, float("nan"))
+    sb.use_torch_stream()
+    sb.dudt(dbig, big)
+    torch.cuda.synchronize()
+    dsm = torch.from_numpy(du_small.ravel(order="F").copy()).cuda().view(4, n, n, n0 + 2, n0 + 2)
+    for m in range(4):
+        assert torch.equal(dbig[m, :, :, 1:-1, 1:-1], dsm[m, :, :, 1:-1, 1:-1].repeat(1, 1, K, K))
+        ring = torch.cat([dbig[m, :, :, 0, :].flatten(), dbig[m, :, :, -1, :].flatten(),
+                          dbig[m, :, :, :, 0].flatten(), dbig[m, :, :, :, -1].flatten()])
+        assert torch.all(ring == 0)
+    sb.close()
+
+
+def test_stream_selection_orders_against_torch():
+    """fr2d_set_stream(NULL) is HIP's null stream (torch's default stream), a torch side stream
+    is honoured, and FR_OWN_STREAM returns to the context's stream: in every case a fill queued
+    just before the launch on the same stream is overwritten by the residual."""
+    import torch
+
+    ps, u, s, orc = setup(64, 64, 3, "rusanov", "periodic")
+    ref = orc.rhs(u)
+    ud = torch.from_numpy(u.ravel(order="F").copy()).cuda()
+    side = torch.cuda.Stream()
+    for mode in ("default", "side", "own"):
+        dud = torch.empty_like(ud)
+        if mode == "own":
+            s.use_own_stream()
+            dud.fill_(float("nan"))
+            torch.cuda.synchronize()
+            s.dudt(dud, ud)
+        else:
+            st = torch.cuda.current_stream() if mode == "default" else side
+            s.use_torch_stream(st)
+            with torch.cuda.stream(st):
+                dud.fill_(float("nan"))
+                s.dudt(dud, ud)
+        torch.cuda.synchronize()
+        du = dud.cpu().numpy().reshape(s.shape, order="F")
+        assert np.all(np.isfinite(du)), mode
+        assert rel_inf(du, ref) <= RES_TOL
