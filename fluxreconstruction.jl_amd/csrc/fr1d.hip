@@ -340,7 +340,8 @@ struct Batch1D {
   double gamma = 1.4, a = 1.0;
   std::vector<Prob1D> probs;
   size_t len = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;      // the stream every call runs on
+  hipStream_t own_stream = nullptr;  // the context's own non-blocking stream
   Prob1D *d_probs = nullptr;
   double *d_ops = nullptr;
   double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_acc = nullptr;
@@ -397,7 +398,8 @@ struct Batch1D {
       FR_HIP(hipMalloc(&d_work[v], wl.size() * sizeof(Work1D)));
       FR_HIP(hipMemcpy(d_work[v], wl.data(), wl.size() * sizeof(Work1D), hipMemcpyHostToDevice));
     }
-    FR_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    FR_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+    stream = own_stream;
     FR_HIP(hipMalloc(&d_u, len * sizeof(double)));
     FR_HIP(hipMemset(d_u, 0, len * sizeof(double)));
     return FR_OK;
@@ -412,7 +414,7 @@ struct Batch1D {
     if (d_count) (void)hipFree(d_count);
     for (auto *w : d_work)
       if (w) (void)hipFree(w);
-    if (stream) (void)hipStreamDestroy(stream);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 
   static int tindex(int T) { return T == 64 ? 0 : (T == 128 ? 1 : 2); }
@@ -572,6 +574,12 @@ int fr1d_destroy(fr1d_ctx *c) {
 }
 
 size_t fr1d_state_len(const fr1d_ctx *c) { return c ? c->len : 0; }
+
+int fr1d_set_stream(fr1d_ctx *c, void *s) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr1d_set_stream: null context");
+  c->stream = s == FR_OWN_STREAM ? c->own_stream : (hipStream_t)s;
+  return FR_OK;
+}
 
 int fr1d_rhs_dev(fr1d_ctx *c, const double *d_u, double *d_du, double t) {
   (void)t;

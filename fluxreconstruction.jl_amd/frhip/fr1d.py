@@ -21,6 +21,7 @@ _lib.SIGNATURES.update({
     "fr1d_state_len": (C.c_size_t, [_vp]),
     "fr1d_rhs": (C.c_int, [_vp, _dp, _dp, C.c_double]),
     "fr1d_rhs_dev": (C.c_int, [_vp, _vp, _vp, C.c_double]),
+    "fr1d_set_stream": (C.c_int, [_vp, _vp]),
     "fr1d_set_state": (C.c_int, [_vp, _dp]),
     "fr1d_get_state": (C.c_int, [_vp, _dp]),
     "fr1d_step": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int]),
@@ -42,7 +43,8 @@ _lib.SIGNATURES.update({
 
 def _bind():
     L = _lib.lib()
-    for name in ("fr1d_create", "fr1d_destroy", "fr1d_state_len", "fr1d_rhs", "fr1d_rhs_dev", "fr1d_set_state",
+    for name in ("fr1d_create", "fr1d_destroy", "fr1d_state_len", "fr1d_rhs", "fr1d_rhs_dev", "fr1d_set_stream",
+                 "fr1d_set_state",
                  "fr1d_get_state", "fr1d_step", "fr_ens_create", "fr_ens_destroy", "fr_ens_state_len", "fr_ens_offset",
                  "fr_ens_set_state", "fr_ens_get_state", "fr_ens_step", "fr_ens_step_graph", "fr_ens_autotune",
                  "fr1d_sensor_filter", "fr1d_positivity", "fr_ens_sensor_filter", "fr_ens_positivity"):
@@ -77,6 +79,7 @@ class FR1D:
         h = C.c_void_p()
         check(L.fr1d_create(C.byref(h), C.byref(d)))
         self.h = h
+        self.len = L.fr1d_state_len(h)
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -90,6 +93,17 @@ class FR1D:
             pass
 
     def dudt(self, du, u, p=None, t: float = 0.0):
+        """In-place du = L(u): numpy (host) arrays in the reference layout, or contiguous float64
+        torch GPU tensors of fr1d_state_len values, launched on torch's current stream."""
+        if type(u).__module__.startswith("torch"):
+            import torch
+
+            for a in (u, du):
+                if str(a.dtype) != "torch.float64" or not a.is_cuda or not a.is_contiguous() or a.numel() != self.len:
+                    raise ValueError("device arrays must be contiguous float64 GPU tensors of the state size")
+            check(_lib.lib().fr1d_set_stream(self.h, C.c_void_p(torch.cuda.current_stream().cuda_stream)))
+            check(_lib.lib().fr1d_rhs_dev(self.h, C.c_void_p(u.data_ptr()), C.c_void_p(du.data_ptr()), float(t)))
+            return None
         check(_lib.lib().fr1d_rhs(self.h, dptr(u), dptr(du), float(t)))
 
     def rhs(self, u):

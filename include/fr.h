@@ -219,6 +219,9 @@ size_t fr1d_state_len(const fr1d_ctx *ctx);
  * frode_advection!(du, u, p, t) (src/Equation/eq_advection.jl:153-175) */
 int fr1d_rhs(fr1d_ctx *ctx, const double *u, double *du, double t);
 int fr1d_rhs_dev(fr1d_ctx *ctx, const double *d_u, double *d_du, double t);
+/* the stream of every later call on the context, with fr2d_set_stream's meaning (NULL is HIP's
+ * null stream, FR_OWN_STREAM the context's own non-blocking stream, the default) */
+int fr1d_set_stream(fr1d_ctx *ctx, void *hip_stream);
 int fr1d_set_state(fr1d_ctx *ctx, const double *u);
 int fr1d_get_state(fr1d_ctx *ctx, double *u);
 int fr1d_step(fr1d_ctx *ctx, int scheme, double dt, int nsteps);

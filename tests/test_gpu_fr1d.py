@@ -245,3 +245,24 @@ def test_ensemble_dissipation_matches_single():
             assert np.array_equal(g.get_state(), e.member(out, b)), (what, b)
             g.close()
         assert n == tot
+
+
+def test_euler_residual_device_entry_point():
+    """fr1d_rhs_dev on torch tensors runs on torch's current stream (a NaN fill queued just before
+    on the same stream is overwritten) and equals the host path bitwise, on the default stream
+    and on a side stream."""
+    import torch
+
+    ncell, deg = 3000, 3
+    ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
+    u = np.asfortranarray(euler_state(ncell, deg, seed=9))
+    g = frhip.FREulerProblem(ps, 1.4, "period", "rusanov")
+    ref = g.rhs(u)
+    ud = torch.from_numpy(u.ravel(order="F").copy()).cuda()
+    for st in (torch.cuda.current_stream(), torch.cuda.Stream()):
+        dud = torch.empty_like(ud)
+        with torch.cuda.stream(st):
+            dud.fill_(float("nan"))
+            g.dudt(dud, ud)
+        torch.cuda.synchronize()
+        assert np.array_equal(dud.cpu().numpy().reshape(g.shape, order="F"), ref)
