@@ -77,74 +77,19 @@ def cpu_baseline(u0, dt, budget_s=12.0):
 
 
 def bench_ensemble(steps=10):
-    """512 seeded 1D Euler Riemann problems (64..2048 cells, p in {2,3,4}), Rusanov, SSP-RK3;
-    one batched launch per stage (autotuned mapping, hipGraph) vs one launch per member."""
-    import numpy as np
-    import torch
+    """512 seeded 1D Euler Riemann problems, batched vs looped (frhip.workloads)."""
+    from frhip import workloads
 
-    import frhip
-    from frhip import ics
-
-    ncell, deg, dx, members = ics.riemann_ensemble(512, seed=2024)
-    e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
-    flat = e.pack(members)
-    lam = max(float(np.max(np.abs(m[..., 1] / m[..., 0]) + np.sqrt(1.4 * 0.4 * (m[..., 2] - 0.5 * m[..., 1] ** 2 / m[..., 0]) / m[..., 0]))) for m in members)
-    dt = 0.05 * float(np.min(dx)) / lam
-    dof = int(sum(int(n) * (int(p) + 1) * 3 for n, p in zip(ncell, deg)))
-    e.set_state(flat)
-    choice = e.autotune(dt)
-    res = {}
-    for name, fn in (("looped", lambda: e.step(dt, steps, looped=True)),
-                     ("batched", lambda: e.step(dt, steps)),
-                     ("batched_graph", lambda: e.step_graph(dt, steps, 1))):
-        e.set_state(flat)
-        fn()  # warm-up (and graph build for the graph variant is inside the timed call)
-        e.set_state(flat)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        fn()
-        torch.cuda.synchronize()
-        res[name] = dof * 3 * steps / (time.perf_counter() - t0)
-    out = e.get_state()
-    finite = sum(bool(np.all(np.isfinite(e.member(out, b)))) for b in range(len(members)))
-    return {"workload": "512 1D Euler Riemann problems, 64..2048 cells, p in {2,3,4}, Rusanov, SSP-RK3",
-            "dof": dof, "steps": steps, "unit": "DOF·RK-stage/s", "looped": res["looped"],
-            "batched": res["batched"], "batched_graph": res["batched_graph"],
-            "speedup_vs_looped": max(res["batched"], res["batched_graph"]) / res["looped"],
-            "autotune_choice": choice, "members_finite": finite,
-            "note": "piecewise-constant jumps, plain unlimited scheme: members whose p = 4 jump overshoots "
-                    "to negative pressure go non-finite (the throughput is unaffected)"}
+    return workloads.riemann_ensemble(steps)
 
 
 def bench_bgk(steps=3, ens_steps=1):
     """2D BGK, 128^2 periodic quads p = 2 x 24^2 Gauss-Hermite velocities, Kn = tau = 1e-2;
-    plus the 64-member ensemble with tau log-uniform in [1e-3, 1e-1]."""
-    import numpy as np
-    import torch
+    plus the 64-member ensemble with tau log-uniform in [1e-3, 1e-1] (frhip.workloads)."""
+    from frhip import workloads
 
-    from frhip import bgk
-
-    u, wu = bgk.velocity_grid(24)
-    out = {"workload": "2D BGK 128x128 p=2 x 24x24 Gauss-Hermite velocities, SSP-RK3", "unit": "values·RK-stage/s"}
-    dt = 0.05 * (1.0 / 128) / float(np.max(np.abs(u)))
-    for name, taus, n in (("single", np.array([1e-2]), steps),
-                          ("ensemble64", np.exp(np.random.default_rng(5).uniform(np.log(1e-3), np.log(1e-1), 64)),
-                           ens_steps)):
-        g = bgk.BGK2D(128, 128, 2, u, wu, u, wu, taus)
-        one = torch.from_numpy(bgk.maxwellian_state(128, 128, 2, u, u)).cuda()
-        g.set_state(one.repeat(len(taus)))
-        del one
-        g.step(dt, 1)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        g.step(dt, n)
-        torch.cuda.synchronize()
-        el = time.perf_counter() - t0
-        vals = g.len
-        out[name] = {"members": len(taus), "values": int(vals), "value": vals * 3 * n / el,
-                     "alg_GBs": vals * 3 * n * (8 + 64 / 3) / el / 1e9, "steps": n}
-        g.close()
-    return out
+    return {"workload": "2D BGK 128x128 p=2 x 24x24 Gauss-Hermite velocities, SSP-RK3", "unit": "values·RK-stage/s",
+            "single": workloads.bgk(steps), "ensemble64": workloads.bgk(ens_steps, members=64)}
 
 
 def bench_sensor():

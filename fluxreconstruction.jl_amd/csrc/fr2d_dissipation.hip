@@ -333,9 +333,10 @@ __global__ __launch_bounds__(256) void fr2d_filter_refresh_limit_kernel(
     } else {
       double tmin = INFINITY;
       bool have = false;
-#pragma unroll 1
+      // fully unrolled: a run-time (jj, f) index into x would move the whole cell to scratch
+#pragma unroll
       for (int jj = 0; jj < NSP; ++jj)
-#pragma unroll 1
+#pragma unroll
         for (int f = 0; f < 4; ++f) {
           double w[4];
 #pragma unroll

@@ -1,0 +1,124 @@
+"""The measured workloads besides the 2D headline stage, as functions that bench.py calls and as a
+command line that runs one of them alone (for a kernel profile of that workload):
+
+    python3 -m frhip.workloads ensemble       512 seeded 1D Euler Riemann problems, SSP-RK3
+    python3 -m frhip.workloads bgk            2D BGK 128^2 p = 2 x 24^2 velocities, one member
+    python3 -m frhip.workloads bgk64          the same, 64 members with log-uniform tau
+    python3 -m frhip.workloads shock_vortex --mode learned --steps 600
+                                              the shock-vortex host loop on 512 x 256, p = 3
+
+Each prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import ics, sensor
+from .bgk import BGK2D, maxwellian_state, velocity_grid
+from .fr1d import Ensemble1D
+
+
+def riemann_ensemble(steps=10, nprob=512, seed=2024):
+    """nprob seeded 1D Euler Riemann problems (64..2048 cells, p in {2,3,4}), Rusanov, SSP-RK3;
+    one batched launch per stage (autotuned mapping, hipGraph) vs one launch per member."""
+    import torch
+
+    ncell, deg, dx, members = ics.riemann_ensemble(nprob, seed=seed)
+    e = Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
+    flat = e.pack(members)
+    lam = max(float(np.max(np.abs(m[..., 1] / m[..., 0]) + np.sqrt(1.4 * 0.4 * (m[..., 2] - 0.5 * m[..., 1] ** 2 / m[..., 0]) / m[..., 0]))) for m in members)
+    dt = 0.05 * float(np.min(dx)) / lam
+    dof = int(sum(int(n) * (int(p) + 1) * 3 for n, p in zip(ncell, deg)))
+    e.set_state(flat)
+    choice = e.autotune(dt)
+    res = {}
+    for name, fn in (("looped", lambda: e.step(dt, steps, looped=True)),
+                     ("batched", lambda: e.step(dt, steps)),
+                     ("batched_graph", lambda: e.step_graph(dt, steps, 1))):
+        e.set_state(flat)
+        fn()  # warm-up (and graph build for the graph variant is inside the timed call)
+        e.set_state(flat)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        res[name] = dof * 3 * steps / (time.perf_counter() - t0)
+    out = e.get_state()
+    finite = sum(bool(np.all(np.isfinite(e.member(out, b)))) for b in range(len(members)))
+    return {"workload": f"{nprob} 1D Euler Riemann problems, 64..2048 cells, p in {{2,3,4}}, Rusanov, SSP-RK3",
+            "dof": dof, "steps": steps, "unit": "DOF·RK-stage/s", "looped": res["looped"],
+            "batched": res["batched"], "batched_graph": res["batched_graph"],
+            "speedup_vs_looped": max(res["batched"], res["batched_graph"]) / res["looped"],
+            "autotune_choice": choice, "members_finite": finite,
+            "note": "piecewise-constant jumps, plain unlimited scheme: members whose p = 4 jump overshoots "
+                    "to negative pressure go non-finite (the throughput is unaffected)"}
+
+
+def bgk(steps=3, members=1, seed=5):
+    """2D BGK, 128^2 periodic quads p = 2 x 24^2 Gauss-Hermite velocities; one member with
+    Kn = tau = 1e-2, or `members` members with tau log-uniform in [1e-3, 1e-1].  Algorithmic bytes
+    per value and stage: 8 (moments pass) + 21.33 (transport: 16 in stage 1, 24 in stages 2-3)."""
+    import torch
+
+    u, wu = velocity_grid(24)
+    dt = 0.05 * (1.0 / 128) / float(np.max(np.abs(u)))
+    taus = np.array([1e-2]) if members == 1 else np.exp(np.random.default_rng(seed).uniform(np.log(1e-3), np.log(1e-1), members))
+    g = BGK2D(128, 128, 2, u, wu, u, wu, taus)
+    one = torch.from_numpy(maxwellian_state(128, 128, 2, u, u)).cuda()
+    g.set_state(one.repeat(len(taus)))
+    del one
+    g.step(dt, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.step(dt, steps)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    vals = g.len
+    g.close()
+    return {"members": len(taus), "values": int(vals), "value": vals * 3 * steps / el,
+            "alg_GBs": vals * 3 * steps * (8 + 64 / 3) / el / 1e9, "steps": steps}
+
+
+def shock_vortex_loop(mode="modal", steps=600):
+    """`steps` iterations of the shock-vortex host loop (example/shock-vortex.jl:296-326) on the
+    512 x 256, p = 3 case: positivity, SSP-RK3, sensor + filter, ghost refresh."""
+    import torch
+
+    c = sensor.SHOCK_VORTEX
+    ic = {k: c[k] for k in ("shock_mach", "shock_x", "vortex_mach", "vortex_r", "vortex_x", "vortex_y")}
+    ps, u0, s = sensor.shock_vortex_solver(c["nx"], c["ny"], **ic)
+    if mode == "learned":
+        sensor.attach(s, sensor.SensorMLP.load())
+    dt, _ = ics.cfl_steps(u0, c["gamma"], ps.dx0, c["cfl"], 0.4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sensor.run_shock_vortex(s, dt, steps, mode)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    s.close()
+    return {"mode": mode, "steps": steps, "ms_per_step": 1e3 * el / steps}
+
+
+def main():
+    ap = argparse.ArgumentParser(description="run one measured workload and print one JSON line")
+    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "shock_vortex"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--mode", default="modal", choices=["modal", "learned", "off"])
+    a = ap.parse_args()
+    if a.workload == "ensemble":
+        r = riemann_ensemble(a.steps or 10)
+    elif a.workload == "bgk":
+        r = bgk(a.steps or 3)
+    elif a.workload == "bgk64":
+        r = bgk(a.steps or 1, members=64)
+    else:
+        r = shock_vortex_loop(a.mode, a.steps or 600)
+    print(json.dumps(r), flush=True)
+
+
+if __name__ == "__main__":
+    main()
