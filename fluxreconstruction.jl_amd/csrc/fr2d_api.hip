@@ -385,6 +385,44 @@ int fr2d_sensor_eval(fr2d_ctx *c, int mode, double S0, double kappa, float *d_ou
   return fr2d_launch_sensor_filter(*c, c->d_u, mode, 1.0, S0, kappa, nullptr, d_out, st);
 }
 
+int fr2d_run_host_loop(fr2d_ctx *c, int scheme, double dt, int nsteps, int mode, double lambda, double S0,
+                       double kappa, int rule, int *n_filtered) {
+  FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_run_host_loop: no state");
+  FR_REQUIRE(nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_run_host_loop: bad dt / nsteps");
+  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_run_host_loop: unknown scheme");
+  FR_REQUIRE(mode >= FR_SENSOR_OFF && mode <= FR_SENSOR_LEARNED, FR_E_INVALID, "fr2d_run_host_loop: unknown mode");
+  FR_REQUIRE(lambda >= 0.0 && kappa > 0.0, FR_E_INVALID, "fr2d_run_host_loop: lambda >= 0 and kappa > 0 required");
+  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_INFLOW_LEFT, FR_E_INVALID, "fr2d_run_host_loop: unknown rule");
+  hipStream_t st = stream_of(*c);
+  if (mode == FR_SENSOR_LEARNED && !c->mlp_loaded) mode = FR_SENSOR_MODAL;  // plain sensor is the fallback
+  // d_count[0]: filtered cells, d_count[1]: limiter failures, both summed over the loop
+  FR_HIP(hipMemsetAsync(c->d_count, 0, 2 * sizeof(int), st));
+  int rc = fr2d_launch_positivity(*c, c->d_u, c->d_count + 1, st);
+  for (int n = 0; n < nsteps && rc == FR_OK; ++n) {
+    if ((rc = prepare_step(*c, scheme, st))) break;
+    if ((rc = enqueue_steps(*c, scheme, dt, 1, st))) break;
+    if ((rc = finish_step(*c, st))) break;
+    if (mode == FR_SENSOR_LEARNED && (rc = fr2d_launch_mlp_sensor(*c, c->d_u, st))) break;
+    if (n + 1 < nsteps) {
+      rc = fr2d_launch_filter_refresh_limit(*c, c->d_u, mode, lambda, S0, kappa, rule, c->d_count, st);
+    } else {
+      if (mode != FR_SENSOR_OFF &&
+          (rc = fr2d_launch_sensor_filter(*c, c->d_u, mode, lambda, S0, kappa, c->d_count, nullptr, st)))
+        break;
+      rc = fr2d_launch_fill_ghost(*c, c->d_u, rule, st);
+    }
+  }
+  if (rc) return rc;
+  int h[2] = {0, 0};
+  FR_HIP(hipMemcpyAsync(h, c->d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  FR_HIP(hipStreamSynchronize(st));
+  if (n_filtered) *n_filtered = h[0];
+  if (h[1] > 0)
+    return fail(FR_E_LIMITER, "fr2d_run_host_loop: " + std::to_string(h[1]) +
+                                  " limiter failure(s) (element mean density not positive) over the loop");
+  return FR_OK;
+}
+
 int fr2d_positivity(fr2d_ctx *c) {
   FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_positivity: no state");
   hipStream_t st = stream_of(*c);

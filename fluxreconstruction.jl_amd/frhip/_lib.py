@@ -114,6 +114,8 @@ SIGNATURES = {
     "fr2d_check_state": (C.c_int, [_vp]),
     "fr2d_primitives": (C.c_int, [_vp, _dp]),
     "fr2d_set_geometry": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "fr2d_run_host_loop": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                     C.c_int, _ip]),
     "fr2d_primitives_dev": (C.c_int, [_vp, _vp]),
 }
 

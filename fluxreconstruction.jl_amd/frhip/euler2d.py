@@ -175,6 +175,18 @@ class FREuler2D:
                                                    _lib.GHOST[rule], C.byref(n)))
         return n.value
 
+    def run_host_loop(self, dt: float, nsteps: int, mode: str = "modal", lam: float = 5e-4,
+                      S0: float | None = None, kappa: float = 9.0, rule: str = "inflow_left",
+                      scheme: str = "ssprk3") -> int:
+        """nsteps iterations of the shock-vortex.jl:296-326 host loop enqueued natively with one
+        synchronisation at the end (fr2d_run_host_loop); returns #filtered cells over the loop."""
+        if S0 is None:
+            S0 = -3.0 * np.log10(self.ps.deg)
+        n = C.c_int(0)
+        check(_lib.lib().fr2d_run_host_loop(self.h, _lib.RK[scheme], float(dt), int(nsteps), _lib.SENSOR[mode],
+                                            float(lam), float(S0), float(kappa), _lib.GHOST[rule], C.byref(n)))
+        return n.value
+
     def positivity(self):
         """Positivity limiter of src/dissipation.jl:125-206 on every interior cell."""
         check(_lib.lib().fr2d_positivity(self.h))

@@ -198,14 +198,20 @@ def shock_vortex_solver(nx, ny, deg=3, gamma=1.4, flux="hll", **ic):
 
 
 def run_shock_vortex(s: FREuler2D, dt, nsteps, mode="modal", lam=SHOCK_VORTEX["lam"], kappa=SHOCK_VORTEX["kappa"], callback=None,
-                     every=0, fused=True):
+                     every=0, fused=True, native=False):
     """Host loop of example/shock-vortex.jl:296-326 on the device state: positivity limiter on
     every cell (:298-303), SSP-RK3 step, sensor + filter (mode "modal" / "learned" / "off",
     :306-321), ghost refresh of the bottom, top and right sides with the inflow column kept
     (:322-325).  fused: the filter, the ghost refresh and the next iteration's limiter run as one
     pass over the state (fr2d_filter_refresh_limit, the same arithmetic); the last iteration ends
-    with the filter and the ghost refresh."""
+    with the filter and the ghost refresh.  native: the whole fused loop is enqueued by the
+    library (fr2d_run_host_loop, no callback) with one synchronisation at the end."""
     S0 = -3.0 * np.log10(s.ps.deg)
+    if native:
+        if callback is not None or not fused:
+            raise ValueError("run_shock_vortex: the native loop is the fused loop without callbacks")
+        s.run_host_loop(dt, nsteps, mode, lam, S0, kappa, "inflow_left")
+        return
     s.positivity()
     for n in range(nsteps):
         s.step(dt, 1, "ssprk3")
@@ -328,7 +334,7 @@ def evaluate_shock_vortex(mlp: SensorMLP | None, nx=SHOCK_VORTEX["nx"], ny=SHOCK
         dt, n = ics.cfl_steps(u0, SHOCK_VORTEX["gamma"], ps.dx0, SHOCK_VORTEX["cfl"], t_end)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        run_shock_vortex(s, dt, n, mode)
+        run_shock_vortex(s, dt, n, mode, native=True)
         torch.cuda.synchronize()
         el = time.perf_counter() - t0
         uf = s.get_state()

@@ -96,7 +96,7 @@ def shock_vortex_loop(mode="modal", steps=600):
     dt, _ = ics.cfl_steps(u0, c["gamma"], ps.dx0, c["cfl"], 0.4)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    sensor.run_shock_vortex(s, dt, steps, mode)
+    sensor.run_shock_vortex(s, dt, steps, mode, native=True)
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     s.close()

@@ -155,6 +155,15 @@ int fr2d_positivity(fr2d_ctx *ctx);
  * FR_E_LIMITER like fr2d_positivity. */
 int fr2d_filter_refresh_limit(fr2d_ctx *ctx, int mode, double lambda, double S0, double kappa, int rule,
                               int *n_filtered);
+/* nsteps iterations of the host loop of example/shock-vortex.jl:296-326 on the device state,
+ * enqueued without host round trips: the positivity limiter, then per iteration one step of
+ * `scheme`, the sensor + filter (`mode`), the ghost refresh `rule` and (but for the last
+ * iteration) the next positivity limiter, the last three fused as in fr2d_filter_refresh_limit.
+ * Equal to those calls in sequence; limiter failures are summed on the device and reported once
+ * at the end (FR_E_LIMITER) instead of at the iteration that met them.  n_filtered (optional):
+ * filtered cells summed over the loop. */
+int fr2d_run_host_loop(fr2d_ctx *ctx, int scheme, double dt, int nsteps, int mode, double lambda, double S0,
+                       double kappa, int rule, int *n_filtered);
 /* Scan the device state for NaN/Inf (FR_E_NONFINITE) or rho <= 0 / p <= 0 (FR_E_NEGATIVE_STATE). */
 int fr2d_check_state(fr2d_ctx *ctx);
 /* Uniform parallelogram cells (dev/parallelogram.jl:78-162): iJ is the 2x2 inverse Jacobian of

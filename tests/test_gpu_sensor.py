@@ -109,7 +109,8 @@ def test_solver_features_and_learned_filter():
 @pytest.mark.parametrize("mode", ["off", "modal", "learned"])
 def test_fused_filter_refresh_limit_equals_host_loop(mode):
     """fr2d_filter_refresh_limit (filter, inflow-keeping ghost refresh and the next limiter in one
-    pass) reproduces the unfused host loop of example/shock-vortex.jl:296-326 bitwise."""
+    pass) and the natively enqueued loop fr2d_run_host_loop reproduce the unfused host loop of
+    example/shock-vortex.jl:296-326 bitwise."""
     import os
 
     ps = frhip.FRPSpace2D(0.0, 2.0, 48, 0.0, 1.0, 24, 3, 1, 1)
@@ -121,13 +122,14 @@ def test_fused_filter_refresh_limit_equals_host_loop(mode):
             pytest.skip("no trained sensor weights in the package")
         mlp = S.SensorMLP.load()
     out = []
-    for fused in (False, True):
+    for fused, native in ((False, False), (True, False), (True, True)):
         s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
         s.set_state(u0)
         if mlp is not None:
             S.attach(s, mlp)
-        S.run_shock_vortex(s, dt, 12, mode, fused=fused)
+        S.run_shock_vortex(s, dt, 12, mode, fused=fused, native=native)
         out.append(s.get_state())
         s.close()
     assert np.all(np.isfinite(out[0]))
     assert np.array_equal(out[0], out[1])
+    assert np.array_equal(out[0], out[2])  # fr2d_run_host_loop
