@@ -53,6 +53,20 @@ def test_residual_parity_shapes(nx, ny):
     assert rel_inf(s.rhs(u), orc.rhs(u)) <= RES_TOL
 
 
+@pytest.mark.parametrize("bc", ["ghost", "periodic"])
+def test_wide_grid_strip_order(bc):
+    """1100 x 6: 69 tiles across, so the workgroups take column strips of 16 tiles with a last
+    strip of 5, and the tile rows are partial; residual and SSP-RK3 steps against the oracle."""
+    ps, u, s, orc = setup(1100, 6, 3, "rusanov", bc, L=10.0, seed=5)
+    assert rel_inf(s.rhs(u), orc.rhs(u)) <= RES_TOL
+    dt, _ = ics.cfl_steps(u, 1.4, ps.dx0, 0.05, 1.0)
+    s.set_state(u)
+    s.step(dt, 3)
+    uc = u.copy(order="F")
+    orc.ssprk3(uc, dt, 3)
+    assert rel_l2(s.get_state(), uc) <= STATE_TOL
+
+
 def test_residual_garbage_du_overwritten():
     ps, u, s, orc = setup(20, 9, 3, "rusanov", "periodic")
     du = np.full(s.shape, np.nan, order="F")
