@@ -282,7 +282,7 @@ __device__ __forceinline__ void diss1d_cell(double *__restrict__ u, long long n,
   double tmin = 1.0e300;
 #pragma unroll
   for (int f = 0; f < 2; ++f)
-    if (prim_lambda(bv[f], g) < eps) {
+    if (prim_lambda_below(bv[f], g, eps)) {
       have = true;
       tmin = fmin(tmin, tj_root(bv[f], um4, g, eps));
     }
@@ -290,7 +290,7 @@ __device__ __forceinline__ void diss1d_cell(double *__restrict__ u, long long n,
   for (int p = 0; p < NSP; ++p) {
     x[p][0] = t1 * (x[p][0] - um[0]) + um[0];
     const double w4[4] = {x[p][0], x[p][1], 0.0, x[p][2]};
-    if (prim_lambda(w4, g) < eps) {
+    if (prim_lambda_below(w4, g, eps)) {
       have = true;
       tmin = fmin(tmin, tj_root(w4, um4, g, eps));
     }

@@ -150,7 +150,7 @@ __global__ __launch_bounds__(256) void fr2d_positivity_kernel(double *u, int nx,
         }
         w[m] = s;
       }
-      if (prim_lambda(w, g) < eps) {
+      if (prim_lambda_below(w, g, eps)) {
         have = true;
         tmin = fmin(tmin, tj_root(w, um, g, eps));
       }
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(256) void fr2d_positivity_kernel(double *u, int nx,
     const double r = t1 * (u[e + ldp * q] - um[0]) + um[0];
     u[e + ldp * q] = r;
     double w[4] = {r, u[e + ldp * (q + NP)], u[e + ldp * (q + 2 * NP)], u[e + ldp * (q + 3 * NP)]};
-    if (prim_lambda(w, g) < eps) {
+    if (prim_lambda_below(w, g, eps)) {
       have = true;
       tmin = fmin(tmin, tj_root(w, um, g, eps));
     }
@@ -195,17 +195,42 @@ __device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int
 // (example/shock-vortex.jl:306-325, then :298-303): sensor + filter of every cell, the ghost
 // refresh `rule` written from the filtered values, and the positivity limiter of every interior
 // cell.  Each cell is read once; a ghost cell that the rule overwrites is written only by the
-// thread of its source cell, so no cell has two writers.  count[0]: filtered cells whose values
+// threads of its source cell, so no cell has two writers.  count[0]: filtered cells whose values
 // are kept (interior and kept ghosts), count[1]: cells whose density assert failed (left as
-// filtered).  The arithmetic is that of fr2d_sensor_filter_kernel and fr2d_positivity_kernel.
+// filtered).
+//
+// Four lanes per cell: lane r of a quad holds variable r of the cell at every solution point, so
+// the modal transform and the filter of the four variables run side by side.  The density
+// lane's sensor value is broadcast over the quad; the limiter's test points are split over the
+// quad (lane r: the points of face r and the nodes q = r (mod 4)), their four variables gathered
+// through LDS, and the corrector minimum reduced over the quad.  Every value is formed with the
+// operations, in the order, of fr2d_sensor_filter_kernel and fr2d_positivity_kernel, so the pass
+// reproduces the unfused sequence bitwise.
+// (256, 4): four wavefronts per SIMD; at nsp = 4 the compiler then keeps the kernel in 128
+// VGPRs with a few spilled values, measured 3-4 % faster than 142 VGPRs at three wavefronts
 template <int NSP>
-__global__ __launch_bounds__(256) void fr2d_filter_refresh_limit_kernel(
+__global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
     double *u, int nx, int ny, long long ldp, const double *__restrict__ V, const double *__restrict__ iV,
     const double *__restrict__ wts, const double *__restrict__ ll, const double *__restrict__ lr, int mode,
     double lambda, double S0, double kappa, const float *__restrict__ strength, int rule, double g, int *count) {
   constexpr int NP = NSP * NSP;
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (e >= ldp) return;
+  constexpr int NS = NP > 4 * NSP ? NP : 4 * NSP;  // exchange slots per cell and variable
+  constexpr int SS = 64 * 4 + 4;                   // doubles per slot: 64 cells x 4 variables (+ pad)
+  __shared__ double sx[NS * SS];
+  __shared__ double smean[64 * 4];
+  // the rows of iV, then the rows of V: every lane reads the same entry at a time (LDS broadcast),
+  // so the matrices occupy neither scalar nor vector registers
+  __shared__ double smat[2 * NP * NP];
+  for (int t = threadIdx.x; t < NP * NP; t += 256) {
+    const int a = t / NP, b = t % NP;
+    smat[t] = iV[a + NP * b];
+    smat[NP * NP + t] = V[a + NP * b];
+  }
+  __syncthreads();
+  const int r = threadIdx.x & 3;   // variable held by this lane
+  const int c = threadIdx.x >> 2;  // cell slot of the workgroup
+  const long long e = blockIdx.x * 64LL + c;
+  if (e >= ldp) return;  // whole quads leave together
   const int nxg = nx + 2;
   const int i = (int)(e % nxg), j = (int)(e / nxg);
   const bool ghost = i == 0 || i == nx + 1 || j == 0 || j == ny + 1;
@@ -213,60 +238,60 @@ __global__ __launch_bounds__(256) void fr2d_filter_refresh_limit_kernel(
     int si, sj;
     bool keep;
     ghost_source(i, j, nx, ny, rule, si, sj, keep);
-    if (!keep) return;  // refreshed by the thread of its source cell
+    if (!keep) return;  // refreshed by the threads of its source cell
   }
-  double x[NP][4];
+  auto X = [&](int q, int m) -> double & { return sx[q * SS + c * 4 + m]; };
+  double *const ur = u + e + ldp * (NP * r);
+  double x[NP];
 #pragma unroll
-  for (int m = 0; m < 4; ++m)
-#pragma unroll
-    for (int q = 0; q < NP; ++q) x[q][m] = u[e + ldp * (q + NP * m)];
+  for (int q = 0; q < NP; ++q) x[q] = ur[ldp * q];
 
-  // sensor + filter
-  bool changed = false;
+  // sensor + filter.  The modal coefficients of the density lane give the switch; they are
+  // formed again for a flagged cell, so that no coefficient is live across the switch's sine.
+  bool changed = false, flagged = false;
   if (mode != FR_SENSOR_OFF) {
-    double mod[NP];
+    auto modal = [&](double *mod) {
 #pragma unroll
-    for (int a = 0; a < NP; ++a) {
-      double s = 0.0;
+      for (int a = 0; a < NP; ++a) {
+        double s = 0.0;
 #pragma unroll
-      for (int b = 0; b < NP; ++b) s += iV[a + NP * b] * x[b][0];
-      mod[a] = s;
-    }
+        for (int b = 0; b < NP; ++b) s += smat[a * NP + b] * x[b];
+        mod[a] = s;
+      }
+    };
     double lam = 0.0;
     if (mode == FR_SENSOR_MODAL) {
-      double ss = 0.0;
+      double su;
+      {
+        double mod[NP];
+        modal(mod);
+        double ss = 0.0;
 #pragma unroll
-      for (int a = 0; a < NP; ++a) ss += mod[a] * mod[a];
-      const double su = mod[NP - 1] * mod[NP - 1] / ss;
+        for (int a = 0; a < NP; ++a) ss += mod[a] * mod[a];
+        su = __shfl(mod[NP - 1] * mod[NP - 1] / ss, 0, 4);  // the density lane's value
+      }
       lam = shock_switch(log10(su), S0, kappa) ? lambda : 0.0;
     } else {
       lam = lambda * (double)strength[e];
     }
     if (lam > 0.0) {
-      changed = true;
-      atomicAdd(count, 1);
+      changed = flagged = true;
+      double mod[NP];
+      modal(mod);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        if (m > 0) {
+      for (int a = 1; a < NP; ++a) mod[a] /= (1.0 + lam * (double)((a + 1) * (a + 1)) * (double)(a * a));
 #pragma unroll
-          for (int a = 0; a < NP; ++a) {
-            double s = 0.0;
+      for (int q = 0; q < NP; ++q) {
+        double s = 0.0;
 #pragma unroll
-            for (int b = 0; b < NP; ++b) s += iV[a + NP * b] * x[b][m];
-            mod[a] = s;
-          }
-        }
-#pragma unroll
-        for (int a = 1; a < NP; ++a) mod[a] /= (1.0 + lam * (double)((a + 1) * (a + 1)) * (double)(a * a));
-#pragma unroll
-        for (int q = 0; q < NP; ++q) {
-          double s = 0.0;
-#pragma unroll
-          for (int b = 0; b < NP; ++b) s += V[q + NP * b] * mod[b];
-          x[q][m] = s;
-        }
+        for (int b = 0; b < NP; ++b) s += smat[NP * NP + q * NP + b] * mod[b];
+        x[q] = s;
       }
     }
+  }
+  {  // one count update per wavefront
+    const unsigned long long fb = __ballot(flagged && r == 0);
+    if (fb && (int)(threadIdx.x & 63) == __ffsll((long long)fb) - 1) atomicAdd(count, __popcll(fb));
   }
 
   // ghost refresh: copy the filtered cell into every ghost cell whose source it is
@@ -278,40 +303,43 @@ __global__ __launch_bounds__(256) void fr2d_filter_refresh_limit_kernel(
       bool dup = false;
       for (int b = 0; b < a; ++b) dup = dup || ci[b] == gi;
       if (dup) continue;
-      for (int c = 0; c < 5; ++c) {
-        const int gj = cj[c];
+      for (int cc = 0; cc < 5; ++cc) {
+        const int gj = cj[cc];
         if (gj < 0 || gj > ny + 1) continue;
         bool dupj = false;
-        for (int d = 0; d < c; ++d) dupj = dupj || cj[d] == gj;
+        for (int d = 0; d < cc; ++d) dupj = dupj || cj[d] == gj;
         if (dupj || (gi == i && gj == j)) continue;
         if (!(gi == 0 || gi == nx + 1 || gj == 0 || gj == ny + 1)) continue;
         int si, sj;
         bool keep;
         ghost_source(gi, gj, nx, ny, rule, si, sj, keep);
         if (keep || si != i || sj != j) continue;
-        const long long ge = (long long)gi + (long long)nxg * gj;
+        double *const gr = u + (long long)gi + (long long)nxg * gj + ldp * (NP * r);
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-          for (int q = 0; q < NP; ++q) u[ge + ldp * (q + NP * m)] = x[q][m];
+        for (int q = 0; q < NP; ++q) gr[ldp * q] = x[q];
       }
     }
   }
 
   // positivity limiter of src/dissipation.jl:125-206 on the interior cells (weights wp/4)
+  bool failed = false;
   if (!ghost) {
+    double um_own = 0.0;
+#pragma unroll
+    for (int l = 0; l < NSP; ++l)
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) um_own += x[k + NSP * l] * wts[k + NSP * l];
+    smean[c * 4 + r] = um_own;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) X(q, r) = x[q];
+    wave_lds_sync();
     double um[4];
 #pragma unroll
-    for (int m = 0; m < 4; ++m) {
-      double s = 0.0;
-#pragma unroll
-      for (int l = 0; l < NSP; ++l)
-#pragma unroll
-        for (int k = 0; k < NSP; ++k) s += x[k + NSP * l][m] * wts[k + NSP * l];
-      um[m] = s;
-    }
+    for (int m = 0; m < 4; ++m) um[m] = smean[c * 4 + m];
     const double p_mean = 0.5 * um[0] * (1.0 / prim_lambda(um, g));
     const double eps = fmin(1e-13, fmin(um[0], p_mean));
+    // density minimum over the boundary points (faces 1 bottom, 2 right, 3 top, 4 left) and the
+    // nodes, taken by the density lane from its registers and broadcast over the quad
     double rmin = INFINITY;
 #pragma unroll
     for (int jj = 0; jj < NSP; ++jj)
@@ -320,66 +348,83 @@ __global__ __launch_bounds__(256) void fr2d_filter_refresh_limit_kernel(
         double s = 0.0;
 #pragma unroll
         for (int q = 0; q < NSP; ++q) {
-          const double w = (f == 0 || f == 3) ? ll[q] : lr[q];
-          s += ((f == 0 || f == 2) ? x[jj + NSP * q][0] : x[q + NSP * jj][0]) * w;
+          const double wq = (f == 0 || f == 3) ? ll[q] : lr[q];
+          s += x[(f == 0 || f == 2) ? jj + NSP * q : q + NSP * jj] * wq;
         }
         rmin = fmin(rmin, s);
       }
 #pragma unroll
-    for (int q = 0; q < NP; ++q) rmin = fmin(rmin, x[q][0]);
+    for (int q = 0; q < NP; ++q) rmin = fmin(rmin, x[q]);
+    rmin = __shfl(rmin, 0, 4);
     const double t1 = fmin((um[0] - eps) / (um[0] - rmin + 1e-8), 1.0);
     if (!(t1 > 0.0 && t1 <= 1.0)) {
-      atomicAdd(count + 1, 1);
+      failed = true;
     } else {
       double tmin = INFINITY;
       bool have = false;
-      // fully unrolled: a run-time (jj, f) index into x would move the whole cell to scratch
+      // the density-fixed nodes q = r (mod 4)
 #pragma unroll
-      for (int jj = 0; jj < NSP; ++jj)
-#pragma unroll
-        for (int f = 0; f < 4; ++f) {
-          double w[4];
-#pragma unroll
-          for (int m = 0; m < 4; ++m) {
-            double s = 0.0;
-#pragma unroll
-            for (int q = 0; q < NSP; ++q) {
-              const double wq = (f == 0 || f == 3) ? ll[q] : lr[q];
-              s += ((f == 0 || f == 2) ? x[jj + NSP * q][m] : x[q + NSP * jj][m]) * wq;
-            }
-            w[m] = s;
-          }
-          if (prim_lambda(w, g) < eps) {
+      for (int t = 0; t < (NP + 3) / 4; ++t) {
+        const int q = 4 * t + r;
+        if (q < NP) {
+          double w[4] = {t1 * (X(q, 0) - um[0]) + um[0], X(q, 1), X(q, 2), X(q, 3)};
+          if (prim_lambda_below(w, g, eps)) {
             have = true;
             tmin = fmin(tmin, tj_root(w, um, g, eps));
           }
         }
-      if (t1 < 1.0) changed = true;
+      }
+      wave_lds_sync();  // the nodes are read: the slots take this lane's variable at every
+                        // boundary point (x is still the state before the density fix)
 #pragma unroll
-      for (int q = 0; q < NP; ++q) {
-        const double r = t1 * (x[q][0] - um[0]) + um[0];
-        x[q][0] = r;
-        double w[4] = {r, x[q][1], x[q][2], x[q][3]};
-        if (prim_lambda(w, g) < eps) {
+      for (int jj = 0; jj < NSP; ++jj)
+#pragma unroll
+        for (int f = 0; f < 4; ++f) {
+          double s = 0.0;
+#pragma unroll
+          for (int q = 0; q < NSP; ++q) {
+            const double wq = (f == 0 || f == 3) ? ll[q] : lr[q];
+            s += x[(f == 0 || f == 2) ? jj + NSP * q : q + NSP * jj] * wq;
+          }
+          X(4 * jj + f, r) = s;
+        }
+      wave_lds_sync();
+      // the boundary points of face r
+#pragma unroll
+      for (int jj = 0; jj < NSP; ++jj) {
+        double w[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) w[m] = X(4 * jj + r, m);
+        if (prim_lambda_below(w, g, eps)) {
           have = true;
           tmin = fmin(tmin, tj_root(w, um, g, eps));
         }
       }
-      if (have) {
+      tmin = fmin(tmin, __shfl_xor(tmin, 1, 4));
+      tmin = fmin(tmin, __shfl_xor(tmin, 2, 4));
+      int hv = have ? 1 : 0;
+      hv |= __shfl_xor(hv, 1, 4);
+      hv |= __shfl_xor(hv, 2, 4);
+      if (t1 < 1.0) changed = true;
+      if (r == 0) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) x[q] = t1 * (x[q] - um[0]) + um[0];
+      }
+      if (hv) {
         changed = true;
         const double t2 = fmin(tmin, 1.0);
 #pragma unroll
-        for (int m = 0; m < 4; ++m)
-#pragma unroll
-          for (int q = 0; q < NP; ++q) x[q][m] = t2 * (x[q][m] - um[m]) + um[m];
+        for (int q = 0; q < NP; ++q) x[q] = t2 * (x[q] - um_own) + um_own;
       }
     }
   }
+  {
+    const unsigned long long fb = __ballot(failed && r == 0);
+    if (fb && (int)(threadIdx.x & 63) == __ffsll((long long)fb) - 1) atomicAdd(count + 1, __popcll(fb));
+  }
   if (changed) {
 #pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int q = 0; q < NP; ++q) u[e + ldp * (q + NP * m)] = x[q][m];
+    for (int q = 0; q < NP; ++q) ur[ldp * q] = x[q];
   }
 }
 
@@ -495,7 +540,8 @@ static int frl_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, doubl
                    int *cnt, hipStream_t st) {
   const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
   DissOps o = fr2d_diss_ops(c);
-  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((ldp + 255) / 256)), dim3(256), 0, st,
+  // four lanes per cell, 64 cells per workgroup
+  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((ldp + 63) / 64)), dim3(256), 0, st,
                      d_u, c.nx, c.ny, ldp, o.V, o.iV, o.wts, o.ll, o.lr, mode, lambda, S0, kappa,
                      (const float *)c.d_strength, rule, c.gamma, cnt);
   FR_LAUNCHED("fr2d_filter_refresh_limit_kernel");

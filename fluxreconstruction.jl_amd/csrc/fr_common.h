@@ -241,6 +241,24 @@ __device__ __forceinline__ double prim_lambda(const double *w, double g) {
   return 0.5 * w[0] / (g - 1.0) / (w[3] - 0.5 * (w[1] * w[1] + w[2] * w[2]) / w[0]);
 }
 
+// prim_lambda(w, g) < eps with the same outcome as the three-division form, at the cost of one
+// refined reciprocal on all but borderline states.  With w0 > 0 the approximate
+// d = E - |m|^2/(2 rho) is within 2^-49 (|E| + |m|^2/(2 rho)) of the exactly rounded one, so
+// d > 2^-24 (|E| + |m|^2/(2 rho)) gives d > 0 and fixes d to a relative 2^-25.  Then
+// lambda = (rho/(2(g-1))) / d > eps for certain when d eps < rho/(2(g-1)) (1 - 2^-20), a
+// margin far above every rounding error of the two sides.  Every other state (non-positive or
+// non-finite values, d near 0, lambda near eps) takes the exact form.
+__device__ __forceinline__ bool prim_lambda_below(const double *w, double g, double eps) {
+  if (w[0] > 0.0) {
+    const double ri = fr_rcp(w[0]);
+    const double k = 0.5 * (w[1] * w[1] + w[2] * w[2]) * ri;
+    const double d = w[3] - k;
+    const double a = 0.5 * w[0] * fr_rcp(g - 1.0);
+    if (d > 0x1p-24 * (fabs(w[3]) + k) && d * eps < a * (1.0 - 0x1p-20)) return false;
+  }
+  return prim_lambda(w, g) < eps;
+}
+
 // y-direction interface flux: rotate into the face-normal frame n = (0, 1), solve, rotate back
 // (KitBase local_frame / global_frame with cos = 0, sin = 1; example/shock-vortex.jl:84-87).
 // Riemann flux across a face with unit normal n = (c, s): the states go to the face frame

@@ -133,3 +133,40 @@ def test_fused_filter_refresh_limit_equals_host_loop(mode):
     assert np.all(np.isfinite(out[0]))
     assert np.array_equal(out[0], out[1])
     assert np.array_equal(out[0], out[2])  # fr2d_run_host_loop
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("mode", ["off", "modal"])
+@pytest.mark.parametrize("rule", ["periodic", "outflow", "inflow_left"])
+def test_fused_pass_with_active_limiter(deg, mode, rule):
+    """The fused pass on a state whose limiter corrects cells (density and energy pushed down at a
+    few points) reproduces the unfused sensor_filter -> fill_ghost -> positivity sequence bitwise,
+    for every degree and ghost rule."""
+    nx, ny = 21, 13
+    ps = frhip.FRPSpace2D(0.0, 1.0, nx, 0.0, 1.0, ny, deg, 1, 1)
+    u = ics.isentropic_vortex(ps, 1.4, strength=5.0, center=(0.5, 0.5))
+    u = ics.perturbed(u, seed=deg, amplitude=0.02)
+    rng = np.random.default_rng(7 + deg)
+    n = deg + 1
+    for _ in range(8):
+        i, j = rng.integers(1, nx + 1), rng.integers(1, ny + 1)
+        k, l = rng.integers(0, n), rng.integers(0, n)
+        u[i, j, k, l, 0] *= -0.2
+        u[i, j, k, l, 3] *= 0.05
+    out = {}
+    for path in ("fused", "unfused", "no_limiter"):
+        s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
+        s.set_state(u)
+        if path == "fused":
+            s.filter_refresh_limit(mode, 5e-4, kappa=9.0, rule=rule)
+        else:
+            if mode != "off":
+                s.sensor_filter(mode, 5e-4, kappa=9.0)
+            s.fill_ghost(rule)
+            if path == "unfused":
+                s.positivity()
+        out[path] = s.get_state()
+        s.close()
+    assert np.all(np.isfinite(out["fused"]))
+    assert not np.array_equal(out["unfused"], out["no_limiter"])  # the limiter corrected cells
+    assert np.array_equal(out["fused"], out["unfused"])
