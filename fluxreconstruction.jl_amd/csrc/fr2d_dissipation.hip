@@ -1,8 +1,9 @@
 // fr2d_dissipation.hip -- per-step dissipation kernels of the 2D solver (gfx950):
 // modal shock sensor + L2 modal filter (example/shock-vortex.jl:308-321, src/dissipation.jl:13-23),
 // positivity limiter (src/dissipation.jl:125-215), state checks and ghost-layer copies.
-// One thread per element; the element's planes are read straight from the SoA state (lanes =
-// consecutive elements, so every access of a wavefront is contiguous).
+// The filter and the limiter run four lanes per element (one per conserved variable); the sensor
+// evaluation and the state check one thread per element.  Element planes are read straight from
+// the SoA state (consecutive elements on consecutive lanes, so accesses stay contiguous).
 #include <algorithm>
 
 #include "fr2d_internal.h"
@@ -86,94 +87,6 @@ __global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int 
 }
 
 
-// Positivity limiter of src/dissipation.jl:125-206 on every interior cell, weights wp/4.
-template <int NSP>
-__global__ __launch_bounds__(256) void fr2d_positivity_kernel(double *u, int nx, int ny, long long ldp,
-                                                              const double *__restrict__ wts,
-                                                              const double *__restrict__ ll,
-                                                              const double *__restrict__ lr, double g,
-                                                              int *bad) {
-  constexpr int NP = NSP * NSP;
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= (long long)nx * ny) return;
-  const long long nxg = nx + 2;
-  const long long e = (t % nx + 1) + nxg * (t / nx + 1);
-  auto U = [&](int k, int l, int m) -> double & { return u[e + ldp * (k + NSP * (l + NSP * m))]; };
-  double um[4];
-#pragma unroll
-  for (int m = 0; m < 4; ++m) {
-    double s = 0.0;
-#pragma unroll
-    for (int l = 0; l < NSP; ++l)
-#pragma unroll
-      for (int k = 0; k < NSP; ++k) s += U(k, l, m) * wts[k + NSP * l];
-    um[m] = s;
-  }
-  const double p_mean = 0.5 * um[0] * (1.0 / prim_lambda(um, g));
-  const double eps = fmin(1e-13, fmin(um[0], p_mean));
-  // boundary states before any correction: face f (1 bottom, 2 right, 3 top, 4 left), point j
-  double rmin = INFINITY;
-#pragma unroll
-  for (int j = 0; j < NSP; ++j)
-#pragma unroll
-    for (int f = 0; f < 4; ++f) {
-      double s = 0.0;
-#pragma unroll
-      for (int q = 0; q < NSP; ++q) {
-        const double w = (f == 0 || f == 3) ? ll[q] : lr[q];
-        s += ((f == 0 || f == 2) ? U(j, q, 0) : U(q, j, 0)) * w;
-      }
-      rmin = fmin(rmin, s);
-    }
-#pragma unroll
-  for (int q = 0; q < NP; ++q) rmin = fmin(rmin, u[e + ldp * q]);
-  const double t1 = fmin((um[0] - eps) / (um[0] - rmin + 1e-8), 1.0);
-  if (!(t1 > 0.0 && t1 <= 1.0)) {
-    atomicAdd(bad, 1);
-    return;
-  }
-  // pressure corrector on the (pre-fix) boundary states, then on the density-fixed points
-  double tmin = INFINITY;
-  bool have = false;
-#pragma unroll 1
-  for (int j = 0; j < NSP; ++j)
-#pragma unroll 1
-    for (int f = 0; f < 4; ++f) {
-      double w[4];
-#pragma unroll
-      for (int m = 0; m < 4; ++m) {
-        double s = 0.0;
-#pragma unroll
-        for (int q = 0; q < NSP; ++q) {
-          const double wq = (f == 0 || f == 3) ? ll[q] : lr[q];
-          s += ((f == 0 || f == 2) ? U(j, q, m) : U(q, j, m)) * wq;
-        }
-        w[m] = s;
-      }
-      if (prim_lambda_below(w, g, eps)) {
-        have = true;
-        tmin = fmin(tmin, tj_root(w, um, g, eps));
-      }
-    }
-#pragma unroll
-  for (int q = 0; q < NP; ++q) {
-    const double r = t1 * (u[e + ldp * q] - um[0]) + um[0];
-    u[e + ldp * q] = r;
-    double w[4] = {r, u[e + ldp * (q + NP)], u[e + ldp * (q + 2 * NP)], u[e + ldp * (q + 3 * NP)]};
-    if (prim_lambda_below(w, g, eps)) {
-      have = true;
-      tmin = fmin(tmin, tj_root(w, um, g, eps));
-    }
-  }
-  if (have) {
-    const double t2 = fmin(tmin, 1.0);
-#pragma unroll
-    for (int m = 0; m < 4; ++m)
-#pragma unroll
-      for (int q = 0; q < NP; ++q) u[e + ldp * (q + NP * m)] = t2 * (u[e + ldp * (q + NP * m)] - um[m]) + um[m];
-  }
-}
-
 // Source cell of ghost cell (gi, gj) under a ghost refresh rule (as fr2d_fill_ghost_kernel);
 // keep = the rule leaves the cell as it is (the inflow column of FR_GHOST_INFLOW_LEFT).
 __device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int rule, int &si, int &sj, bool &keep) {
@@ -204,15 +117,20 @@ __device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int
 // lane's sensor value is broadcast over the quad; the limiter's test points are split over the
 // quad (lane r: the points of face r and the nodes q = r (mod 4)), their four variables gathered
 // through LDS, and the corrector minimum reduced over the quad.  Every value is formed with the
-// operations, in the order, of fr2d_sensor_filter_kernel and fr2d_positivity_kernel, so the pass
-// reproduces the unfused sequence bitwise.
+// operations, in the order, of one thread per cell running src/dissipation.jl:13-23 and
+// :125-215, so the result does not depend on how the cells or the points are split.
+//
+// The same kernel serves the separate entry points: rule -1 skips the ghost refresh (every cell
+// is then filtered where it lies, ghosts included, as shock-vortex.jl:308-321), limit 0 skips the
+// limiter, mode FR_SENSOR_OFF the filter.
 // (256, 4): four wavefronts per SIMD; at nsp = 4 the compiler then keeps the kernel in 128
 // VGPRs with a few spilled values, measured 3-4 % faster than 142 VGPRs at three wavefronts
 template <int NSP>
 __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
     double *u, int nx, int ny, long long ldp, const double *__restrict__ V, const double *__restrict__ iV,
     const double *__restrict__ wts, const double *__restrict__ ll, const double *__restrict__ lr, int mode,
-    double lambda, double S0, double kappa, const float *__restrict__ strength, int rule, double g, int *count) {
+    double lambda, double S0, double kappa, const float *__restrict__ strength, int rule, int limit, double g,
+    int *nfilt, int *nfail) {
   constexpr int NP = NSP * NSP;
   constexpr int NS = NP > 4 * NSP ? NP : 4 * NSP;  // exchange slots per cell and variable
   constexpr int SS = 64 * 4 + 4;                   // doubles per slot: 64 cells x 4 variables (+ pad)
@@ -235,10 +153,16 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
   const int i = (int)(e % nxg), j = (int)(e / nxg);
   const bool ghost = i == 0 || i == nx + 1 || j == 0 || j == ny + 1;
   if (ghost) {
-    int si, sj;
-    bool keep;
-    ghost_source(i, j, nx, ny, rule, si, sj, keep);
-    if (!keep) return;  // refreshed by the threads of its source cell
+    if (rule < 0) {
+      if (mode == FR_SENSOR_OFF) return;  // neither filtered nor limited
+    } else {
+      // a kept ghost stays: it may be the source of other ghosts (the corners of
+      // FR_GHOST_INFLOW_LEFT copy the inflow column)
+      int si, sj;
+      bool keep;
+      ghost_source(i, j, nx, ny, rule, si, sj, keep);
+      if (!keep) return;  // refreshed by the threads of its source cell
+    }
   }
   auto X = [&](int q, int m) -> double & { return sx[q * SS + c * 4 + m]; };
   double *const ur = u + e + ldp * (NP * r);
@@ -291,11 +215,11 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
   }
   {  // one count update per wavefront
     const unsigned long long fb = __ballot(flagged && r == 0);
-    if (fb && (int)(threadIdx.x & 63) == __ffsll((long long)fb) - 1) atomicAdd(count, __popcll(fb));
+    if (fb && (int)(threadIdx.x & 63) == __ffsll((long long)fb) - 1) atomicAdd(nfilt, __popcll(fb));
   }
 
   // ghost refresh: copy the filtered cell into every ghost cell whose source it is
-  if (!(i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1)) {
+  if (rule >= 0 && !(i >= 2 && i <= nx - 1 && j >= 2 && j <= ny - 1)) {
     const int ci[5] = {i - 1, i, i + 1, 0, nx + 1}, cj[5] = {j - 1, j, j + 1, 0, ny + 1};
     for (int a = 0; a < 5; ++a) {
       const int gi = ci[a];
@@ -323,7 +247,7 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
 
   // positivity limiter of src/dissipation.jl:125-206 on the interior cells (weights wp/4)
   bool failed = false;
-  if (!ghost) {
+  if (limit && !ghost) {
     double um_own = 0.0;
 #pragma unroll
     for (int l = 0; l < NSP; ++l)
@@ -420,7 +344,7 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
   }
   {
     const unsigned long long fb = __ballot(failed && r == 0);
-    if (fb && (int)(threadIdx.x & 63) == __ffsll((long long)fb) - 1) atomicAdd(count + 1, __popcll(fb));
+    if (fb && (int)(threadIdx.x & 63) == __ffsll((long long)fb) - 1) atomicAdd(nfail, __popcll(fb));
   }
   if (changed) {
 #pragma unroll
@@ -491,72 +415,64 @@ int fr2d_launch_copy_ghost(const Fr2dCtx &c, const double *src, double *dst, hip
 }
 
 
+// One launch of the four-lanes-per-cell pass: the filter (mode), the ghost refresh (rule, -1 for
+// none: every cell is then processed where it lies) and the limiter (limit) as selected.
 template <int NSP>
-static int sensor_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int *cnt,
-                      float *eval_out, hipStream_t st) {
+static int quad_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int rule,
+                    int limit, int *nfilt, int *nfail, hipStream_t st) {
+  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
+  DissOps o = fr2d_diss_ops(c);
+  // four lanes per cell, 64 cells per workgroup
+  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((ldp + 63) / 64)), dim3(256), 0, st,
+                     d_u, c.nx, c.ny, ldp, o.V, o.iV, o.wts, o.ll, o.lr, mode, lambda, S0, kappa,
+                     (const float *)c.d_strength, rule, limit, c.gamma, nfilt, nfail);
+  FR_LAUNCHED("fr2d_filter_refresh_limit_kernel");
+  return FR_OK;
+}
+
+static int launch_quad(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int rule,
+                       int limit, int *nfilt, int *nfail, hipStream_t st) {
+  switch (c.nsp) {
+    case 2: return quad_nsp<2>(c, d_u, mode, lambda, S0, kappa, rule, limit, nfilt, nfail, st);
+    case 3: return quad_nsp<3>(c, d_u, mode, lambda, S0, kappa, rule, limit, nfilt, nfail, st);
+    case 4: return quad_nsp<4>(c, d_u, mode, lambda, S0, kappa, rule, limit, nfilt, nfail, st);
+    case 5: return quad_nsp<5>(c, d_u, mode, lambda, S0, kappa, rule, limit, nfilt, nfail, st);
+    default: return fail(FR_E_UNSUPPORTED, "dissipation: degree out of range");
+  }
+}
+
+// per-cell strength evaluation without touching the state (the sensor evaluation entry point)
+template <int NSP>
+static int sensor_eval_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
+                           float *eval_out, hipStream_t st) {
   const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
   const int ncell = (int)ldp;
   DissOps o = fr2d_diss_ops(c);
   hipLaunchKernelGGL(fr2d_sensor_filter_kernel<NSP>, dim3((ncell + 255) / 256), dim3(256), 0, st, d_u, ncell, ldp,
-                     o.V, o.iV, mode, lambda, S0, kappa, (const float *)c.d_strength, cnt, eval_out);
+                     o.V, o.iV, mode, lambda, S0, kappa, (const float *)c.d_strength, nullptr, eval_out);
   FR_LAUNCHED("fr2d_sensor_filter_kernel");
   return FR_OK;
 }
 
 int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
                               int *d_count, float *eval_out, hipStream_t st) {
+  if (!eval_out) return launch_quad(c, d_u, mode, lambda, S0, kappa, -1, 0, d_count, nullptr, st);
   switch (c.nsp) {
-    case 2: return sensor_nsp<2>(c, d_u, mode, lambda, S0, kappa, d_count, eval_out, st);
-    case 3: return sensor_nsp<3>(c, d_u, mode, lambda, S0, kappa, d_count, eval_out, st);
-    case 4: return sensor_nsp<4>(c, d_u, mode, lambda, S0, kappa, d_count, eval_out, st);
-    case 5: return sensor_nsp<5>(c, d_u, mode, lambda, S0, kappa, d_count, eval_out, st);
+    case 2: return sensor_eval_nsp<2>(c, d_u, mode, lambda, S0, kappa, eval_out, st);
+    case 3: return sensor_eval_nsp<3>(c, d_u, mode, lambda, S0, kappa, eval_out, st);
+    case 4: return sensor_eval_nsp<4>(c, d_u, mode, lambda, S0, kappa, eval_out, st);
+    case 5: return sensor_eval_nsp<5>(c, d_u, mode, lambda, S0, kappa, eval_out, st);
     default: return fail(FR_E_UNSUPPORTED, "sensor: degree out of range");
   }
 }
 
-template <int NSP>
-static int positivity_nsp(const Fr2dCtx &c, double *d_u, int *bad, hipStream_t st) {
-  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
-  const long long n = (long long)c.nx * c.ny;
-  DissOps o = fr2d_diss_ops(c);
-  hipLaunchKernelGGL(fr2d_positivity_kernel<NSP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_u, c.nx,
-                     c.ny, ldp, o.wts, o.ll, o.lr, c.gamma, bad);
-  FR_LAUNCHED("fr2d_positivity_kernel");
-  return FR_OK;
-}
-
 int fr2d_launch_positivity(const Fr2dCtx &c, double *d_u, int *d_count, hipStream_t st) {
-  switch (c.nsp) {
-    case 2: return positivity_nsp<2>(c, d_u, d_count, st);
-    case 3: return positivity_nsp<3>(c, d_u, d_count, st);
-    case 4: return positivity_nsp<4>(c, d_u, d_count, st);
-    case 5: return positivity_nsp<5>(c, d_u, d_count, st);
-    default: return fail(FR_E_UNSUPPORTED, "positivity: degree out of range");
-  }
-}
-
-template <int NSP>
-static int frl_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int rule,
-                   int *cnt, hipStream_t st) {
-  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
-  DissOps o = fr2d_diss_ops(c);
-  // four lanes per cell, 64 cells per workgroup
-  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((ldp + 63) / 64)), dim3(256), 0, st,
-                     d_u, c.nx, c.ny, ldp, o.V, o.iV, o.wts, o.ll, o.lr, mode, lambda, S0, kappa,
-                     (const float *)c.d_strength, rule, c.gamma, cnt);
-  FR_LAUNCHED("fr2d_filter_refresh_limit_kernel");
-  return FR_OK;
+  return launch_quad(c, d_u, FR_SENSOR_OFF, 0.0, 0.0, 1.0, -1, 1, nullptr, d_count, st);
 }
 
 int fr2d_launch_filter_refresh_limit(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0,
                                      double kappa, int rule, int *d_count, hipStream_t st) {
-  switch (c.nsp) {
-    case 2: return frl_nsp<2>(c, d_u, mode, lambda, S0, kappa, rule, d_count, st);
-    case 3: return frl_nsp<3>(c, d_u, mode, lambda, S0, kappa, rule, d_count, st);
-    case 4: return frl_nsp<4>(c, d_u, mode, lambda, S0, kappa, rule, d_count, st);
-    case 5: return frl_nsp<5>(c, d_u, mode, lambda, S0, kappa, rule, d_count, st);
-    default: return fail(FR_E_UNSUPPORTED, "filter_refresh_limit: degree out of range");
-  }
+  return launch_quad(c, d_u, mode, lambda, S0, kappa, rule, 1, d_count, d_count + 1, st);
 }
 
 template <int NSP>

@@ -157,12 +157,13 @@ def test_graph_replay_matches_eager():
     assert np.array_equal(s.get_state(), s2.get_state())
 
 
-def test_sensor_filter_parity():
-    ps = frhip.FRPSpace2D(0.0, 2.0, 40, 0.0, 1.0, 20, 3, 1, 1)
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+def test_sensor_filter_parity(deg):
+    ps = frhip.FRPSpace2D(0.0, 2.0, 40, 0.0, 1.0, 20, deg, 1, 1)
     u = ics.shock_vortex(ps, 1.4)
     u = ics.perturbed(u, seed=5, amplitude=0.02)
     s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
-    orc = O.Fr2d(40, 20, 3, 2.0 / 40, 1.0 / 20, 1.4, "hll", "ghost")
+    orc = O.Fr2d(40, 20, deg, 2.0 / 40, 1.0 / 20, 1.4, "hll", "ghost")
     s.set_state(u)
     n_gpu = s.sensor_filter("modal", 5e-4, kappa=9.0)
     uc = u.copy(order="F")
@@ -171,24 +172,49 @@ def test_sensor_filter_parity():
     assert rel_inf(s.get_state(), uc) <= 1e-13
 
 
-def test_positivity_parity():
-    ps = frhip.FRPSpace2D(0.0, 1.0, 12, 0.0, 1.0, 10, 3, 1, 1)
+@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+def test_positivity_parity(deg):
+    ps = frhip.FRPSpace2D(0.0, 1.0, 12, 0.0, 1.0, 10, deg, 1, 1)
     u = ics.isentropic_vortex(ps, 1.4, strength=5.0, center=(0.5, 0.5))
     rng = np.random.default_rng(3)
+    n = deg + 1
     # push a few solution points below zero density / pressure
     for _ in range(6):
-        i, j, k, l = rng.integers(1, 11), rng.integers(1, 9), rng.integers(0, 4), rng.integers(0, 4)
+        i, j, k, l = rng.integers(1, 11), rng.integers(1, 9), rng.integers(0, n), rng.integers(0, n)
         u[i, j, k, l, 0] *= -0.2
         u[i, j, k, l, 3] *= 0.05
     s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
-    orc = O.Fr2d(12, 10, 3, 1 / 12, 1 / 10, 1.4, "hll", "ghost")
+    orc = O.Fr2d(12, 10, deg, 1 / 12, 1 / 10, 1.4, "hll", "ghost")
     s.set_state(u)
     s.positivity()
     uc = u.copy(order="F")
     bad = orc.positivity(uc)
     assert bad == 0
+    assert not np.array_equal(uc, u)  # the limiter corrected cells
     assert rel_inf(s.get_state(), uc) <= 1e-12
     s.check_state()
+
+
+def test_limiter_failure_is_reported():
+    """An element whose mean density is not positive fails the limiter's assert
+    (src/dissipation.jl:143-145): the plain and the fused limiter report FR_E_LIMITER and leave
+    that element as it was, like the oracle's count of failed elements."""
+    ps = frhip.FRPSpace2D(0.0, 1.0, 12, 0.0, 1.0, 10, 3, 1, 1)
+    u = ics.isentropic_vortex(ps, 1.4, strength=5.0, center=(0.5, 0.5))
+    u[4, 5, :, :, 0] = -0.5
+    orc = O.Fr2d(12, 10, 3, 1 / 12, 1 / 10, 1.4, "hll", "ghost")
+    assert orc.positivity(u.copy(order="F")) == 1
+    for fused in (False, True):
+        s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
+        s.set_state(u)
+        with pytest.raises(frhip.FRError) as e:
+            if fused:
+                s.filter_refresh_limit("off", rule="outflow")
+            else:
+                s.positivity()
+        assert e.value.code == 7
+        assert np.array_equal(s.get_state()[4, 5], u[4, 5])
+        s.close()
 
 
 def test_check_state_flags_negative_and_nan():
