@@ -41,9 +41,10 @@ static int ensure_stage_buffers(Fr2dCtx &c, bool acc) {
   return FR_OK;
 }
 
-// Stage launches of one explicit step on the device state (see fr.h: fr2d_step).  Stage buffers
-// carry the same ghost layer as the state (copied before the step loop), and no stage writes a
-// ghost cell, so FR_BC_GHOST keeps the ghost layer fixed during a step as the reference does.
+// Stage launches of one explicit step on the device state (see fr.h: fr2d_step).  The first stage
+// of every step copies the state's ghost layer into both stage buffers (from its halo loads), and
+// no stage writes a ghost cell of its result, so FR_BC_GHOST keeps the ghost layer fixed during a
+// step as the reference does (du = 0 there).
 static int enqueue_steps(Fr2dCtx &c, int scheme, double dt, int nsteps, hipStream_t st) {
   double *U = c.d_u, *S1 = c.d_s1, *S2 = c.d_s2;
   for (int n = 0; n < nsteps; ++n) {
@@ -51,26 +52,25 @@ static int enqueue_steps(Fr2dCtx &c, int scheme, double dt, int nsteps, hipStrea
     switch (scheme) {
       case FR_RK_SSPRK3:
         // Shu-Osher: u1 = u + dt L(u); u2 = 3/4 u + 1/4 u1 + dt/4 L(u1); u = 1/3 u + 2/3 u2 + 2dt/3 L(u2)
-        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st, S1, S2))) return rc;
         if ((rc = fr2d_launch_stage(c, U, S1, S2, 0.75, 0.25, 0.25 * dt, nullptr, nullptr, 0.0, st))) return rc;
         if ((rc = fr2d_launch_stage(c, U, S2, U, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, nullptr, nullptr, 0.0, st)))
           return rc;
         break;
       case FR_RK_RK4: {
         double *A = c.d_acc;
-        if ((rc = fr2d_launch_stage(c, U, U, S2, 1.0, 0.0, 0.5 * dt, U, A, dt / 6.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, U, U, S2, 1.0, 0.0, 0.5 * dt, U, A, dt / 6.0, st, S1, S2))) return rc;
         if ((rc = fr2d_launch_stage(c, U, S2, S1, 1.0, 0.0, 0.5 * dt, A, A, dt / 3.0, st))) return rc;
         if ((rc = fr2d_launch_stage(c, U, S1, S2, 1.0, 0.0, dt, A, A, dt / 3.0, st))) return rc;
         if ((rc = fr2d_launch_stage(c, A, S2, U, 1.0, 0.0, dt / 6.0, nullptr, nullptr, 0.0, st))) return rc;
         break;
       }
       case FR_RK_MIDPOINT:
-        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0, st))) return rc;
+        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0, st, S1, S2))) return rc;
         if ((rc = fr2d_launch_stage(c, U, S1, U, 1.0, 0.0, dt, nullptr, nullptr, 0.0, st))) return rc;
         break;
       case FR_RK_EULER:
-        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st))) return rc;
-        if ((rc = fr2d_launch_copy_ghost(c, U, S1, st))) return rc;  // keep the pair identical
+        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st, S1, S2))) return rc;
         FR_HIP(hipMemcpyAsync(U, S1, c.len * sizeof(double), hipMemcpyDeviceToDevice, st));
         break;
       default:
@@ -82,11 +82,9 @@ static int enqueue_steps(Fr2dCtx &c, int scheme, double dt, int nsteps, hipStrea
 
 static int prepare_step(Fr2dCtx &c, int scheme, hipStream_t st) {
   FR_REQUIRE(c.d_u, FR_E_INVALID, "fr2d_step: no state (call fr2d_set_state first)");
-  int rc = ensure_stage_buffers(c, scheme == FR_RK_RK4);
-  if (rc) return rc;
-  if ((rc = fr2d_launch_copy_ghost(c, c.d_u, c.d_s1, st))) return rc;
-  if ((rc = fr2d_launch_copy_ghost(c, c.d_u, c.d_s2, st))) return rc;
-  return FR_OK;
+  // the stage buffers take the state's ghost layer inside the first stage of every step
+  (void)st;
+  return ensure_stage_buffers(c, scheme == FR_RK_RK4);
 }
 
 static int finish_step(Fr2dCtx &c, hipStream_t st) {

@@ -375,44 +375,8 @@ __global__ __launch_bounds__(256) void fr2d_check_kernel(const double *u, int nx
   if (negative) atomicAdd(count + 1, negative);
 }
 
-// copy the ghost ring of every plane from src to dst
-__global__ void fr2d_copy_ghost_kernel(const double *src, double *dst, int nx, int ny, long long ldp, int planes) {
-  const int nring = 2 * (nx + 2) + 2 * ny;
-  const long long total = (long long)nring * planes;
-  const long long nxg = nx + 2;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
-       t += (long long)gridDim.x * blockDim.x) {
-    const int g = (int)(t % nring);
-    const long long pl = t / nring;
-    int i, j;
-    if (g < nx + 2) {
-      i = g;
-      j = 0;
-    } else if (g < 2 * (nx + 2)) {
-      i = g - (nx + 2);
-      j = ny + 1;
-    } else if (g < 2 * (nx + 2) + ny) {
-      i = 0;
-      j = g - 2 * (nx + 2) + 1;
-    } else {
-      i = nx + 1;
-      j = g - 2 * (nx + 2) - ny + 1;
-    }
-    const long long o = (long long)i + nxg * j + ldp * pl;
-    dst[o] = src[o];
-  }
-}
 
 // ------------------------------------------------------------------ launchers
-int fr2d_launch_copy_ghost(const Fr2dCtx &c, const double *src, double *dst, hipStream_t st) {
-  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
-  const int planes = c.nsp * c.nsp * 4;
-  const long long total = (long long)(2 * (c.nx + 2) + 2 * c.ny) * planes;
-  const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
-  hipLaunchKernelGGL(fr2d_copy_ghost_kernel, dim3(blocks), dim3(256), 0, st, src, dst, c.nx, c.ny, ldp, planes);
-  FR_LAUNCHED("fr2d_copy_ghost_kernel");
-  return FR_OK;
-}
 
 
 // One launch of the four-lanes-per-cell pass: the filter (mode), the ghost refresh (rule, -1 for

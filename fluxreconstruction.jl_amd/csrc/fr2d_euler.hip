@@ -63,6 +63,9 @@ struct StageArgs {
   const double *acc_in;
   double *acc_out;   // optional second output acc_out = acc_in + b2 * L(u)
   double a0, a1, b, b2;
+  // optional copies of u's ghost layer (non-periodic grids): the stage buffers that later stages
+  // read take the ghost cells from the halo loads of the first stage
+  double *gc0, *gc1;
 };
 
 // Tile order: column strips of `sw` tiles, row by row inside a strip (then cut into one
@@ -78,8 +81,16 @@ __device__ __forceinline__ void strip_tile(int t, int ntx, int nty, int sw, int 
   bx = s * sw + r % ws;
 }
 
+// Four workgroups per CU at nsp = 4: HLL and Rusanov fit 128 VGPRs (HLL with at most a few
+// spilled values); Roe and the RK4 accumulator form would spill heavily and keep the default.
+template <int NSP, int FLUX, int EPI>
+constexpr int fr2d_min_blocks() {
+  return (NSP == 4 && FLUX != FR_FLUX_ROE && EPI != EPI_ACC) ? 4 : 1;
+}
+
 template <int NSP, int FLUX, int EPI, int ADDR, int GEO>
-__global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const StageArgs sa) {
+__global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void fr2d_kernel(const Geo2D geo,
+                                                                                         const StageArgs sa) {
   // operators are compile-time constants (Gauss-Legendre points, Radau correction), so they
   // occupy no scalar registers for the length of the kernel
   constexpr StdOps2D<NSP> ops{};
@@ -168,6 +179,17 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
         sa.out[ci + (long long)nxg * cj + ldp * o] = 0.0;
       }
     }
+  } else {
+    if (blockIdx.x == 0 && sa.gc0) {  // the four ghost corners, which no halo task loads
+      constexpr int per = NSP * NSP * 4;
+      for (int v = tid; v < 4 * per; v += NSP * 64) {
+        const int c = v / per, o = v % per;
+        const long long ci = (c & 1) ? geo.nx + 1 : 0, cj = (c & 2) ? geo.ny + 1 : 0;
+        const long long at = ci + (long long)nxg * cj + ldp * o;
+        sa.gc0[at] = u[at];
+        sa.gc1[at] = u[at];
+      }
+    }
   }
 
   // ---------------------------------------------------------------- P0: own row + halo lines
@@ -237,16 +259,25 @@ __global__ __launch_bounds__(NSP * 64) void fr2d_kernel(const Geo2D geo, const S
     double w[NSP];
 #pragma unroll
     for (int q = 0; q < NSP; ++q) w[q] = lo ? ops.lr[q] : ops.ll[q];
+    // the ghost element's line is copied to gc0, gc1 when asked (then hi, hj = si, sj)
+    const bool ghost = hi == 0 || hi == geo.nx + 1 || hj == 0 || hj == geo.ny + 1;
+    const bool gcopy = EPI != EPI_RESIDUAL && ghost && sa.gc0 != nullptr;
 #pragma unroll
     for (int m = 0; m < 4; ++m) {
       double s = 0.0;
 #pragma unroll
-      for (int q = 0; q < NSP; ++q) s += LDL(u, pb + ps * q, m, hbo) * w[q];
+      for (int q = 0; q < NSP; ++q) {
+        const double v = LDL(u, pb + ps * q, m, hbo);
+        s += v * w[q];
+        if (gcopy) {
+          STL(sa.gc0, pb + ps * q, m, hbo, v);
+          STL(sa.gc1, pb + ps * q, m, hbo, v);
+        }
+      }
       htr[m] = s;
     }
     if constexpr (EPI == EPI_RESIDUAL) {
       // du vanishes on the ghost layer (example/shock-vortex.jl:27)
-      const bool ghost = hi == 0 || hi == geo.nx + 1 || hj == 0 || hj == geo.ny + 1;
       if (ghost) {
         const unsigned gbo = 8u * ((unsigned)hi + nxg * (unsigned)hj);
 #pragma unroll
@@ -643,8 +674,13 @@ int fr2d_launch_residual(const Fr2dCtx &c, const double *d_u, double *d_du, hipS
 }
 
 int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, double *d_out, double a0,
-                      double a1, double b, const double *acc_in, double *acc_out, double b2, hipStream_t st) {
+                      double a1, double b, const double *acc_in, double *acc_out, double b2, hipStream_t st,
+                      double *ghost_copy0, double *ghost_copy1) {
   StageArgs sa{};
+  if (c.bc != FR_BC_PERIODIC) {  // periodic grids never read the ghost layer
+    sa.gc0 = ghost_copy0;
+    sa.gc1 = ghost_copy1 ? ghost_copy1 : ghost_copy0;
+  }
   sa.u = d_us;
   sa.out = d_out;
   sa.u0 = d_u0;

@@ -44,10 +44,11 @@ struct Fr2dCtx {
 inline hipStream_t fr2d_stream(const Fr2dCtx &c) { return c.caller_stream ? c.stream : c.own_stream; }
 
 int fr2d_launch_residual(const Fr2dCtx &c, const double *d_u, double *d_du, hipStream_t st);
+// ghost_copy0/1 (optional, ignored on periodic grids): buffers that receive d_us's ghost layer
 int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, double *d_out, double a0,
-                      double a1, double b, const double *acc_in, double *acc_out, double b2, hipStream_t st);
+                      double a1, double b, const double *acc_in, double *acc_out, double b2, hipStream_t st,
+                      double *ghost_copy0 = nullptr, double *ghost_copy1 = nullptr);
 int fr2d_launch_fill_ghost(const Fr2dCtx &c, double *d_u, int rule, hipStream_t st);
-int fr2d_launch_copy_ghost(const Fr2dCtx &c, const double *src, double *dst, hipStream_t st);
 // eval_out != nullptr: write the per-cell strength multiplier (modal: 0/1, learned: MLP output)
 // instead of filtering
 int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
