@@ -186,6 +186,22 @@ def main():
         torch.cuda.synchronize()
     res_s = e0.elapsed_time(e1) * 1e-3 / nres
 
+    # practical streaming rate of this working set on this GPU: c = a + b over three state-sized
+    # arrays (24 B per value, torch's elementwise kernel), the traffic pattern of stages 2-3
+    sa, sb = torch.rand_like(ud), torch.rand_like(ud)
+    sc = torch.empty_like(ud)
+    with torch.cuda.stream(stream):
+        for _ in range(5):
+            torch.add(sa, sb, out=sc)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for _ in range(nres):
+            torch.add(sa, sb, out=sc)
+        e1.record(stream)
+        torch.cuda.synchronize()
+    stream_gbs = 24.0 * ud.numel() / (e0.elapsed_time(e1) * 1e-3 / nres) / 1e9
+    del sa, sb, sc
+
     if args.copy_calibration:
         src = torch.ones(8 * 1024 * 1024, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
@@ -228,6 +244,8 @@ def main():
             "kernel": "fr2d_kernel<4,RUSANOV,STAGE>",
             "avg_launch_us": per_launch_s * 1e6,
             "alg_bytes_per_launch": alg_bytes,
+            "stream_add_GBs": stream_gbs,
+            "frac_of_stream_add": achieved / stream_gbs,
         },
         "cpu_baseline": cpu,
         "residual_only": {
