@@ -92,6 +92,13 @@ def bench_bgk(steps=3, ens_steps=1):
             "single": workloads.bgk(steps), "ensemble64": workloads.bgk(ens_steps, members=64)}
 
 
+def bench_grids(sizes=(1024, 2048), steps=60):
+    """The same vortex stage on larger periodic grids (frhip.workloads.vortex_grid)."""
+    from frhip import workloads
+
+    return [workloads.vortex_grid(n, steps) for n in sizes]
+
+
 def bench_sensor():
     """Shock-vortex 512x256 p = 3 to t = 0.4 with the modal sensor, the learned sensor (trained
     weights frhip/data/sensor_mlp.npz) and no sensor; density L1 vs a 2x refined modal run."""
@@ -260,7 +267,8 @@ def main():
     if cpu:
         out["gpu_over_cpu"] = value / cpu["value"]
     if not args.no_extras:
-        for key, fn in (("ensemble_riemann512", bench_ensemble), ("bgk_128sq_p2_v24", bench_bgk), ("sensor_shock_vortex", bench_sensor)):
+        for key, fn in (("vortex_larger_grids", bench_grids), ("ensemble_riemann512", bench_ensemble),
+                        ("bgk_128sq_p2_v24", bench_bgk), ("sensor_shock_vortex", bench_sensor)):
             try:
                 out[key] = fn()
             except Exception as exc:  # an extra leg must not hide the headline line

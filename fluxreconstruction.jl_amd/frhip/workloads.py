@@ -6,6 +6,8 @@ command line that runs one of them alone (for a kernel profile of that workload)
     python3 -m frhip.workloads bgk64          the same, 64 members with log-uniform tau
     python3 -m frhip.workloads shock_vortex --mode learned --steps 600
                                               the shock-vortex host loop on 512 x 256, p = 3
+    python3 -m frhip.workloads vortex_grid --n 2048
+                                              the headline vortex stage on an n x n grid
 
 Each prints one JSON line.
 """
@@ -19,7 +21,9 @@ import numpy as np
 
 from . import ics, sensor
 from .bgk import BGK2D, maxwellian_state, velocity_grid
+from .euler2d import FREuler2D
 from .fr1d import Ensemble1D
+from .space import FRPSpace2D
 
 
 def riemann_ensemble(steps=10, nprob=512, seed=2024):
@@ -103,9 +107,44 @@ def shock_vortex_loop(mode="modal", steps=600):
     return {"mode": mode, "steps": steps, "ms_per_step": 1e3 * el / steps}
 
 
+def vortex_grid(n=1024, steps=60, flux="rusanov"):
+    """The isentropic-vortex stage of the headline case on an n x n periodic grid, p = 3 (same
+    [0,10]^2 domain, CFL 0.05): fused SSP-RK3 stage launches replayed from a graph, timed with HIP
+    events on the launch stream; algorithmic bytes 21.33 per DOF and stage."""
+    import torch
+
+    ps = FRPSpace2D(0.0, 10.0, n, 0.0, 10.0, n, 3, 1, 1)
+    u0 = ics.isentropic_vortex(ps, 1.4, strength=5.0, center=(5.0, 5.0))
+    dt, _ = ics.cfl_steps(u0, 1.4, ps.dx0, 0.05, 2.0)
+    s = FREuler2D(ps, 1.4, flux, "periodic")
+    st = torch.cuda.Stream()
+    s.use_torch_stream(st)
+    s.set_state(u0)
+    del u0
+    chunk = 10
+    with torch.cuda.stream(st):
+        s.step_graph(dt, chunk, 0)
+        s.step(dt, 3)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        s.step_graph(dt, chunk, max(1, steps // chunk))
+        e1.record(st)
+        torch.cuda.synchronize()
+    launches = 3 * chunk * max(1, steps // chunk)
+    us = e0.elapsed_time(e1) * 1e3 / launches
+    dof = n * n * 16 * 4
+    s.check_state()
+    s.close()
+    return {"grid": f"{n}x{n}", "dof": dof, "stage_us": us, "dof_stage_per_s": dof / (us * 1e-6),
+            "alg_GBs": 64.0 / 3.0 * dof / (us * 1e-6) / 1e9}
+
+
 def main():
     ap = argparse.ArgumentParser(description="run one measured workload and print one JSON line")
-    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "shock_vortex"])
+    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "shock_vortex", "vortex_grid"])
+    ap.add_argument("--n", type=int, default=1024, help="vortex_grid: cells per side")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--mode", default="modal", choices=["modal", "learned", "off"])
     a = ap.parse_args()
@@ -115,6 +154,8 @@ def main():
         r = bgk(a.steps or 3)
     elif a.workload == "bgk64":
         r = bgk(a.steps or 1, members=64)
+    elif a.workload == "vortex_grid":
+        r = vortex_grid(a.n, a.steps or 60)
     else:
         r = shock_vortex_loop(a.mode, a.steps or 600)
     print(json.dumps(r), flush=True)
