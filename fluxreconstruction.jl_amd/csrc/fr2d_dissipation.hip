@@ -108,9 +108,9 @@ __device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int
 // (example/shock-vortex.jl:306-325, then :298-303): sensor + filter of every cell, the ghost
 // refresh `rule` written from the filtered values, and the positivity limiter of every interior
 // cell.  Each cell is read once; a ghost cell that the rule overwrites is written only by the
-// threads of its source cell, so no cell has two writers.  count[0]: filtered cells whose values
-// are kept (interior and kept ghosts), count[1]: cells whose density assert failed (left as
-// filtered).
+// threads of its source cell, so no cell has two writers.  nfilt counts the filtered cells whose
+// values are kept (interior and kept ghosts), nfail the cells whose density assert failed (left
+// as filtered); each is touched only when its event occurs.
 //
 // Four lanes per cell: lane r of a quad holds variable r of the cell at every solution point, so
 // the modal transform and the filter of the four variables run side by side.  The density
@@ -123,8 +123,9 @@ __device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int
 // The same kernel serves the separate entry points: rule -1 skips the ghost refresh (every cell
 // is then filtered where it lies, ghosts included, as shock-vortex.jl:308-321), limit 0 skips the
 // limiter, mode FR_SENSOR_OFF the filter.
-// (256, 4): four wavefronts per SIMD; at nsp = 4 the compiler then keeps the kernel in 128
-// VGPRs with a few spilled values, measured 3-4 % faster than 142 VGPRs at three wavefronts
+//
+// Launch bound (256, 4): four wavefronts per SIMD; at nsp = 4 the compiler then keeps the kernel
+// in 128 VGPRs with a few spilled values, measured 3-4 % faster than 142 VGPRs at three.
 template <int NSP>
 __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
     double *u, int nx, int ny, long long ldp, const double *__restrict__ V, const double *__restrict__ iV,
