@@ -53,7 +53,9 @@ enum { FR_SENSOR_OFF = 0, FR_SENSOR_MODAL = 1, FR_SENSOR_LEARNED = 2 };
 
 const char *fr_last_error(void);
 const char *fr_version(void);
-/* number of visible HIP devices (0 on a machine without a GPU; never fails) */
+/* number of visible HIP devices (0 on a machine without a GPU; never fails).  fr_last_error,
+ * fr_version and fr_device_count have no reference counterpart: the status plumbing of a C ABI
+ * in place of Julia exceptions. */
 int fr_device_count(void);
 
 /* ------------------------------------------------------------------ operator data (host, native)
@@ -74,7 +76,7 @@ int fr_vandermonde_1d(int N, const double *r, int n, double *V);
 int fr_dvandermonde_1d(int N, const double *r, int n, double *Vr);
 /* quad tensor Vandermonde n x (N+1)^2, modes i-outer / j-inner (src/Transform/transform.jl:55-68) */
 int fr_vandermonde_quad(int N, const double *r, const double *s, int n, double *V);
-/* dense inverse (column-major) */
+/* dense inverse (column-major): LinearAlgebra.inv as used for iV = inv(V) (src/struct.jl:54,205) */
 int fr_inv(int n, const double *A, double *Ainv);
 /* bilinear quad Jacobian [xr xs; yr ys] at (r,s), vertices 4x2 (src/Geometry/geo_jacobi.jl:77-88) */
 int fr_rs_jacobi_quad(double r, double s, const double *vert, double *J);
@@ -103,7 +105,8 @@ typedef struct {
 int fr2d_create(fr2d_ctx **out, const fr2d_desc *desc);
 int fr2d_destroy(fr2d_ctx *ctx);
 /* Run on a caller-provided hipStream_t (NULL is HIP's null stream), or on the context's own
- * non-blocking stream again with FR_OWN_STREAM (the default after fr2d_create). */
+ * non-blocking stream again with FR_OWN_STREAM (the default after fr2d_create).  The stream and
+ * size queries have no reference counterpart (device plumbing of the C ABI). */
 #define FR_OWN_STREAM ((void *)~(uintptr_t)0)
 int fr2d_set_stream(fr2d_ctx *ctx, void *hip_stream);
 void *fr2d_get_stream(fr2d_ctx *ctx);
@@ -122,14 +125,16 @@ int fr2d_get_state(fr2d_ctx *ctx, double *u);        /* device -> host (synchron
 int fr2d_set_state_dev(fr2d_ctx *ctx, const double *d_u);
 int fr2d_get_state_dev(fr2d_ctx *ctx, double *d_u);
 double *fr2d_state_dev(fr2d_ctx *ctx);               /* device pointer of the state (or NULL) */
-/* nsteps explicit steps of size dt with scheme FR_RK_*; each stage is one fused residual+update
- * launch.  In FR_BC_GHOST mode the ghost layer is held fixed (L = 0 there). */
+/* nsteps explicit steps of size dt with scheme FR_RK_*, replacing step!(itg) of OrdinaryDiffEq
+ * (shock-vortex.jl:294,305; Midpoint there, SSP-RK3 / RK4 / Euler here too); each stage is one
+ * fused residual+update launch.  In FR_BC_GHOST mode the ghost layer is held fixed (L = 0 there). */
 int fr2d_step(fr2d_ctx *ctx, int scheme, double dt, int nsteps);
 /* the launches of nsteps steps captured into a hipGraph and replayed nrep times on the context's
  * stream (asynchronous, like fr2d_step); the context keeps the instantiated graph and reuses it
- * while scheme, dt and nsteps are unchanged */
+ * while scheme, dt and nsteps are unchanged.  No reference counterpart (a replay of fr2d_step). */
 int fr2d_step_graph(fr2d_ctx *ctx, int scheme, double dt, int nsteps, int nrep);
-/* refresh the ghost layer of the device state (FR_GHOST_*) */
+/* refresh the ghost layer of the device state (FR_GHOST_*): the ghost copies of the reference
+ * host loops (shock-vortex.jl:322-325, euler2d_wave.jl:125-135) */
 int fr2d_fill_ghost(fr2d_ctx *ctx, int rule);
 
 /* Per-step dissipation of example/shock-vortex.jl:296-321 on the device state:
@@ -140,7 +145,8 @@ int fr2d_fill_ghost(fr2d_ctx *ctx, int rule);
 int fr2d_sensor_filter(fr2d_ctx *ctx, int mode, double lambda, double S0, double kappa,
                        int *n_flagged);
 /* Per-cell strength multiplier of the sensor without filtering (d_out: float per cell of the
- * (nx+2)(ny+2) layout, device memory): modal 1 / 0 (shock_detector), learned max(MLP, 0). */
+ * (nx+2)(ny+2) layout, device memory): modal 1 / 0 (the shock_detector decision of
+ * shock-vortex.jl:308-317), learned max(MLP, 0). */
 int fr2d_sensor_eval(fr2d_ctx *ctx, int mode, double S0, double kappa, float *d_out);
 /* Positivity limiter of src/dissipation.jl:125-206 on every interior cell of the device state,
  * weights wp/4 (shock-vortex.jl:298-303).  Returns FR_E_LIMITER if an element fails the
@@ -164,7 +170,9 @@ int fr2d_filter_refresh_limit(fr2d_ctx *ctx, int mode, double lambda, double S0,
  * filtered cells summed over the loop. */
 int fr2d_run_host_loop(fr2d_ctx *ctx, int scheme, double dt, int nsteps, int mode, double lambda, double S0,
                        double kappa, int rule, int *n_filtered);
-/* Scan the device state for NaN/Inf (FR_E_NONFINITE) or rho <= 0 / p <= 0 (FR_E_NEGATIVE_STATE). */
+/* Scan the device state for NaN/Inf (FR_E_NONFINITE) or rho <= 0 / p <= 0 (FR_E_NEGATIVE_STATE).
+ * No reference counterpart: the reference only asserts inside positive_limiter
+ * (src/dissipation.jl:143-145). */
 int fr2d_check_state(fr2d_ctx *ctx);
 /* Uniform parallelogram cells (dev/parallelogram.jl:78-162): iJ is the 2x2 inverse Jacobian of
  * every cell, column-major like Julia's Matrix (iJ[0] = r_x, iJ[1] = s_x, iJ[2] = r_y, iJ[3] = s_y),
@@ -221,6 +229,8 @@ typedef struct {
   int bc;            /* FR_BC1_* (src/Equation/eq_euler.jl:68-98, eq_advection.jl:251-273) */
   int correction;    /* FR_CORR_* (src/struct.jl:63) */
 } fr1d_desc;
+/* Replaces FREulerProblem / FRAdvectionProblem (src/Equation/eq_euler.jl:1-27,
+ * eq_advection.jl:1-26): the scratch tuple p and the operator data, owned by the context. */
 int fr1d_create(fr1d_ctx **out, const fr1d_desc *desc);
 int fr1d_destroy(fr1d_ctx *ctx);
 size_t fr1d_state_len(const fr1d_ctx *ctx);
@@ -231,6 +241,8 @@ int fr1d_rhs_dev(fr1d_ctx *ctx, const double *d_u, double *d_du, double t);
 /* the stream of every later call on the context, with fr2d_set_stream's meaning (NULL is HIP's
  * null stream, FR_OWN_STREAM the context's own non-blocking stream, the default) */
 int fr1d_set_stream(fr1d_ctx *ctx, void *hip_stream);
+/* device-resident state and nsteps steps of FR_RK_*, replacing init / step!(itg) on the
+ * ODEProblem (example/euler_highlevel.jl:34,54) */
 int fr1d_set_state(fr1d_ctx *ctx, const double *u);
 int fr1d_get_state(fr1d_ctx *ctx, double *u);
 int fr1d_step(fr1d_ctx *ctx, int scheme, double dt, int nsteps);
@@ -247,7 +259,8 @@ int fr1d_positivity(fr1d_ctx *ctx, int *n_limited);
 /* ------------------------------------------------------------------ batched 1D Euler ensembles */
 typedef struct fr_ens_ctx fr_ens_ctx;
 /* nprob independent 1D Euler problems (layout of each member as fr1d); member b has ncell[b]
- * cells of degree deg[b] (1..4) and cell size dx[b]. */
+ * cells of degree deg[b] (1..4) and cell size dx[b].  No reference counterpart: the reference
+ * runs one FREulerProblem per script; an ensemble advances many in one launch per stage. */
 int fr_ens_create(fr_ens_ctx **out, int nprob, const int *ncell, const int *deg, const double *dx,
                   double gamma, int flux, int bc);
 int fr_ens_destroy(fr_ens_ctx *ctx);
@@ -293,7 +306,8 @@ int fr_bgk_get_state(fr_bgk_ctx *ctx, double *f);
 int fr_bgk_step(fr_bgk_ctx *ctx, double dt, int nsteps);
 /* conserved moments of the device state, w[member][c][point] (host) */
 int fr_bgk_moments(fr_bgk_ctx *ctx, double *w);
-/* Gauss-Hermite nodes / weights for the weight exp(-x^2) (host, native) */
+/* Gauss-Hermite nodes / weights for the weight exp(-x^2) (host, native): the velocity quadrature
+ * of the BGK state, in place of KitBase's VSpace1D of bgk_wave.jl:25 */
 int fr_gauss_hermite(int n, double *x, double *w);
 
 #ifdef __cplusplus
