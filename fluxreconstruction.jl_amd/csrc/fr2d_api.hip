@@ -26,18 +26,40 @@ DissOps fr2d_diss_ops(const Fr2dCtx &c) {
 
 static hipStream_t stream_of(const Fr2dCtx &c) { return fr2d_stream(c); }
 
+// one padded device array (Fr2dCtx: pitch, ldpi): raw is the allocation, the array starts
+// kPadFront doubles into it; the padding is zeroed once and never read
+static int alloc_padded(const Fr2dCtx &c, double *&raw, double *&arr) {
+  const size_t n = c.lenp + Fr2dCtx::kPadFront + 1;
+  FR_HIP(hipMalloc(&raw, n * sizeof(double)));
+  FR_HIP(hipMemsetAsync(raw, 0, n * sizeof(double), fr2d_stream(c)));
+  arr = raw + Fr2dCtx::kPadFront;
+  return FR_OK;
+}
+
 static int ensure_state(Fr2dCtx &c) {
-  if (!c.d_u) {
-    FR_HIP(hipMalloc(&c.d_u, c.len * sizeof(double)));
-    FR_HIP(hipMemsetAsync(c.d_u, 0, c.len * sizeof(double), stream_of(c)));
-  }
+  if (!c.d_u) return alloc_padded(c, c.raw_u, c.d_u);
   return FR_OK;
 }
 
 static int ensure_stage_buffers(Fr2dCtx &c, bool acc) {
-  if (!c.d_s1) FR_HIP(hipMalloc(&c.d_s1, c.len * sizeof(double)));
-  if (!c.d_s2) FR_HIP(hipMalloc(&c.d_s2, c.len * sizeof(double)));
-  if (acc && !c.d_acc) FR_HIP(hipMalloc(&c.d_acc, c.len * sizeof(double)));
+  int rc = FR_OK;
+  if (!c.d_s1 && (rc = alloc_padded(c, c.raw_s1, c.d_s1))) return rc;
+  if (!c.d_s2 && (rc = alloc_padded(c, c.raw_s2, c.d_s2))) return rc;
+  if (acc && !c.d_acc && (rc = alloc_padded(c, c.raw_acc, c.d_acc))) return rc;
+  return FR_OK;
+}
+
+// copies between a reference-layout array (rows of nx+2) and a padded one (rows of pitch)
+static int copy_to_padded(const Fr2dCtx &c, double *dst, const double *src, hipMemcpyKind kind, hipStream_t st) {
+  const size_t w = (size_t)(c.nx + 2) * sizeof(double);
+  FR_HIP(hipMemcpy2DAsync(dst, (size_t)c.pitch * sizeof(double), src, w, w, (size_t)(c.ny + 2) * c.nsp * c.nsp * 4,
+                          kind, st));
+  return FR_OK;
+}
+static int copy_from_padded(const Fr2dCtx &c, double *dst, const double *src, hipMemcpyKind kind, hipStream_t st) {
+  const size_t w = (size_t)(c.nx + 2) * sizeof(double);
+  FR_HIP(hipMemcpy2DAsync(dst, w, src, (size_t)c.pitch * sizeof(double), w, (size_t)(c.ny + 2) * c.nsp * c.nsp * 4,
+                          kind, st));
   return FR_OK;
 }
 
@@ -71,7 +93,7 @@ static int enqueue_steps(Fr2dCtx &c, int scheme, double dt, int nsteps, hipStrea
         break;
       case FR_RK_EULER:
         if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st, S1, S2))) return rc;
-        FR_HIP(hipMemcpyAsync(U, S1, c.len * sizeof(double), hipMemcpyDeviceToDevice, st));
+        FR_HIP(hipMemcpyAsync(U, S1, c.lenp * sizeof(double), hipMemcpyDeviceToDevice, st));
         break;
       default:
         return fail(FR_E_INVALID, "fr2d_step: unknown scheme");
@@ -109,8 +131,8 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
   FR_REQUIRE(d->gamma > 1.0 && std::isfinite(d->gamma), FR_E_INVALID, "fr2d_create: gamma must be > 1");
   FR_REQUIRE(d->iJ11 > 0.0 && d->iJ22 > 0.0, FR_E_INVALID, "fr2d_create: iJ11, iJ22 must be > 0");
   // the kernels address one (k, l, m) plane with a 32-bit byte offset per element
-  FR_REQUIRE((double)(d->nx + 2) * (d->ny + 2) * 8.0 < 4294967296.0, FR_E_INVALID,
-             "fr2d_create: grid too large ((nx+2)(ny+2) must be below 2^29)");
+  FR_REQUIRE((double)((d->nx + 2 + 15) / 16 * 16) * (d->ny + 2) * 8.0 < 4294967296.0, FR_E_INVALID,
+             "fr2d_create: grid too large (a padded plane, (nx+2 rounded up to 16)(ny+2), must be below 2^29)");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
     return fail(FR_E_HIP, "fr2d_create: no HIP device visible");
@@ -126,6 +148,9 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
   c->iJ11 = d->iJ11;
   c->iJ22 = d->iJ22;
   c->len = (size_t)(d->nx + 2) * (d->ny + 2) * c->nsp * c->nsp * 4;
+  c->pitch = (d->nx + 2 + 15) / 16 * 16;
+  c->ldpi = (long long)c->pitch * (d->ny + 2);
+  c->lenp = (size_t)c->ldpi * c->nsp * c->nsp * 4;
   if (!build_ops(d->deg, FR_CORR_RADAU, c->ops)) {
     delete c;
     return fail(FR_E_INVALID, "fr2d_create: operator construction failed");
@@ -180,7 +205,7 @@ int fr2d_destroy(fr2d_ctx *c) {
   if (!c) return FR_OK;
   hipStream_t st = stream_of(*c);
   if (st) (void)hipStreamSynchronize(st);
-  double *bufs[] = {c->d_u, c->d_s1, c->d_s2, c->d_acc, c->d_in, c->d_out, c->d_dops};
+  double *bufs[] = {c->raw_u, c->raw_s1, c->raw_s2, c->raw_acc, c->d_in, c->d_out, c->d_dops};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->d_count) (void)hipFree(c->d_count);
@@ -236,7 +261,7 @@ int fr2d_set_state(fr2d_ctx *c, const double *u) {
   int rc = ensure_state(*c);
   if (rc) return rc;
   hipStream_t st = stream_of(*c);
-  FR_HIP(hipMemcpyAsync(c->d_u, u, c->len * sizeof(double), hipMemcpyHostToDevice, st));
+  if ((rc = copy_to_padded(*c, c->d_u, u, hipMemcpyHostToDevice, st))) return rc;
   FR_HIP(hipStreamSynchronize(st));
   return FR_OK;
 }
@@ -245,7 +270,8 @@ int fr2d_get_state(fr2d_ctx *c, double *u) {
   FR_REQUIRE(c && u, FR_E_INVALID, "fr2d_get_state: null argument");
   FR_REQUIRE(c->d_u, FR_E_INVALID, "fr2d_get_state: no state");
   hipStream_t st = stream_of(*c);
-  FR_HIP(hipMemcpyAsync(u, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToHost, st));
+  int rc = copy_from_padded(*c, u, c->d_u, hipMemcpyDeviceToHost, st);
+  if (rc) return rc;
   FR_HIP(hipStreamSynchronize(st));
   return FR_OK;
 }
@@ -254,21 +280,26 @@ int fr2d_set_state_dev(fr2d_ctx *c, const double *d_u) {
   FR_REQUIRE(c && d_u, FR_E_INVALID, "fr2d_set_state_dev: null argument");
   int rc = ensure_state(*c);
   if (rc) return rc;
-  FR_HIP(hipMemcpyAsync(c->d_u, d_u, c->len * sizeof(double), hipMemcpyDeviceToDevice, stream_of(*c)));
-  return FR_OK;
+  return copy_to_padded(*c, c->d_u, d_u, hipMemcpyDeviceToDevice, stream_of(*c));
 }
 
 int fr2d_get_state_dev(fr2d_ctx *c, double *d_u) {
   FR_REQUIRE(c && d_u, FR_E_INVALID, "fr2d_get_state_dev: null argument");
   FR_REQUIRE(c->d_u, FR_E_INVALID, "fr2d_get_state_dev: no state");
-  FR_HIP(hipMemcpyAsync(d_u, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToDevice, stream_of(*c)));
-  return FR_OK;
+  return copy_from_padded(*c, d_u, c->d_u, hipMemcpyDeviceToDevice, stream_of(*c));
 }
 
 double *fr2d_state_dev(fr2d_ctx *c) {
   if (!c) return nullptr;
   if (ensure_state(*c)) return nullptr;
   return c->d_u;
+}
+
+int fr2d_state_pitch(const fr2d_ctx *c, int *pitch, long long *plane) {
+  FR_REQUIRE(c && pitch && plane, FR_E_INVALID, "fr2d_state_pitch: null argument");
+  *pitch = c->pitch;
+  *plane = c->ldpi;
+  return FR_OK;
 }
 
 int fr2d_step(fr2d_ctx *c, int scheme, double dt, int nsteps) {

@@ -26,18 +26,20 @@ __device__ __forceinline__ int shock_switch(double Se, double S0, double kappa) 
 // mode FR_SENSOR_MODAL: strength lambda on cells flagged by the Persson-Peraire switch;
 // FR_SENSOR_LEARNED: strength = lambda * s[e], s produced per cell by the MLP sensor kernels.
 template <int NSP>
-__global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int ncell, long long ldp,
+__global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int ncell, int nxg, int pitch,
+                                                                 long long ldp,
                                                                  const double *__restrict__ V,
                                                                  const double *__restrict__ iV, int mode,
                                                                  double lambda, double S0, double kappa,
                                                                  const float *__restrict__ strength, int *count,
                                                                  float *eval_out) {
   constexpr int NP = NSP * NSP;
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;  // cell, reference order
   if (e >= ncell) return;
+  const long long ep = e % nxg + (long long)pitch * (e / nxg);             // its place in a plane
   double nod[NP], mod[NP];
 #pragma unroll
-  for (int q = 0; q < NP; ++q) nod[q] = u[e + ldp * q];
+  for (int q = 0; q < NP; ++q) nod[q] = u[ep + ldp * q];
 #pragma unroll
   for (int a = 0; a < NP; ++a) {
     double s = 0.0;
@@ -64,7 +66,7 @@ __global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int 
   for (int m = 0; m < 4; ++m) {
     if (m > 0) {
 #pragma unroll
-      for (int q = 0; q < NP; ++q) nod[q] = u[e + ldp * (q + NP * m)];
+      for (int q = 0; q < NP; ++q) nod[q] = u[ep + ldp * (q + NP * m)];
 #pragma unroll
       for (int a = 0; a < NP; ++a) {
         double s = 0.0;
@@ -81,7 +83,7 @@ __global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int 
       double s = 0.0;
 #pragma unroll
       for (int b = 0; b < NP; ++b) s += V[q + NP * b] * mod[b];
-      u[e + ldp * (q + NP * m)] = s;
+      u[ep + ldp * (q + NP * m)] = s;
     }
   }
 }
@@ -128,10 +130,10 @@ __device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int
 // in 128 VGPRs with a few spilled values, measured 3-4 % faster than 142 VGPRs at three.
 template <int NSP>
 __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
-    double *u, int nx, int ny, long long ldp, const double *__restrict__ V, const double *__restrict__ iV,
-    const double *__restrict__ wts, const double *__restrict__ ll, const double *__restrict__ lr, int mode,
-    double lambda, double S0, double kappa, const float *__restrict__ strength, int rule, int limit, double g,
-    int *nfilt, int *nfail) {
+    double *u, int nx, int ny, int pitch, long long ldp, const double *__restrict__ V,
+    const double *__restrict__ iV, const double *__restrict__ wts, const double *__restrict__ ll,
+    const double *__restrict__ lr, int mode, double lambda, double S0, double kappa,
+    const float *__restrict__ strength, int rule, int limit, double g, int *nfilt, int *nfail) {
   constexpr int NP = NSP * NSP;
   constexpr int NS = NP > 4 * NSP ? NP : 4 * NSP;  // exchange slots per cell and variable
   constexpr int SS = 64 * 4 + 4;                   // doubles per slot: 64 cells x 4 variables (+ pad)
@@ -148,10 +150,11 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
   __syncthreads();
   const int r = threadIdx.x & 3;   // variable held by this lane
   const int c = threadIdx.x >> 2;  // cell slot of the workgroup
-  const long long e = blockIdx.x * 64LL + c;
-  if (e >= ldp) return;  // whole quads leave together
   const int nxg = nx + 2;
+  const long long e = blockIdx.x * 64LL + c;  // cell in the reference order (strength index)
+  if (e >= (long long)nxg * (ny + 2)) return;  // whole quads leave together
   const int i = (int)(e % nxg), j = (int)(e / nxg);
+  const long long ep = i + (long long)pitch * j;  // its place in a plane of the padded state
   const bool ghost = i == 0 || i == nx + 1 || j == 0 || j == ny + 1;
   if (ghost) {
     if (rule < 0) {
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
     }
   }
   auto X = [&](int q, int m) -> double & { return sx[q * SS + c * 4 + m]; };
-  double *const ur = u + e + ldp * (NP * r);
+  double *const ur = u + ep + ldp * (NP * r);
   double x[NP];
 #pragma unroll
   for (int q = 0; q < NP; ++q) x[q] = ur[ldp * q];
@@ -239,7 +242,7 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
         bool keep;
         ghost_source(gi, gj, nx, ny, rule, si, sj, keep);
         if (keep || si != i || sj != j) continue;
-        double *const gr = u + (long long)gi + (long long)nxg * gj + ldp * (NP * r);
+        double *const gr = u + (long long)gi + (long long)pitch * gj + ldp * (NP * r);
 #pragma unroll
         for (int q = 0; q < NP; ++q) gr[ldp * q] = x[q];
       }
@@ -355,12 +358,12 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
 
 // counts non-finite values (count[0]) and non-positive density / pressure points (count[1])
 template <int NSP>
-__global__ __launch_bounds__(256) void fr2d_check_kernel(const double *u, int nx, int ny, long long ldp, double g,
-                                                         int *count) {
+__global__ __launch_bounds__(256) void fr2d_check_kernel(const double *u, int nx, int ny, int pitch, long long ldp,
+                                                         double g, int *count) {
   constexpr int NP = NSP * NSP;
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (t >= (long long)nx * ny) return;
-  const long long e = (t % nx + 1) + (long long)(nx + 2) * (t / nx + 1);
+  const long long e = (t % nx + 1) + (long long)pitch * (t / nx + 1);
   int nonfinite = 0, negative = 0;
   for (int q = 0; q < NP; ++q) {
     double w[4];
@@ -385,11 +388,11 @@ __global__ __launch_bounds__(256) void fr2d_check_kernel(const double *u, int nx
 template <int NSP>
 static int quad_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int rule,
                     int limit, int *nfilt, int *nfail, hipStream_t st) {
-  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
+  const long long ncell = (long long)(c.nx + 2) * (c.ny + 2);
   DissOps o = fr2d_diss_ops(c);
   // four lanes per cell, 64 cells per workgroup
-  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((ldp + 63) / 64)), dim3(256), 0, st,
-                     d_u, c.nx, c.ny, ldp, o.V, o.iV, o.wts, o.ll, o.lr, mode, lambda, S0, kappa,
+  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, st,
+                     d_u, c.nx, c.ny, c.pitch, c.ldpi, o.V, o.iV, o.wts, o.ll, o.lr, mode, lambda, S0, kappa,
                      (const float *)c.d_strength, rule, limit, c.gamma, nfilt, nfail);
   FR_LAUNCHED("fr2d_filter_refresh_limit_kernel");
   return FR_OK;
@@ -410,10 +413,10 @@ static int launch_quad(const Fr2dCtx &c, double *d_u, int mode, double lambda, d
 template <int NSP>
 static int sensor_eval_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,
                            float *eval_out, hipStream_t st) {
-  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
-  const int ncell = (int)ldp;
+  const int ncell = (c.nx + 2) * (c.ny + 2);
   DissOps o = fr2d_diss_ops(c);
-  hipLaunchKernelGGL(fr2d_sensor_filter_kernel<NSP>, dim3((ncell + 255) / 256), dim3(256), 0, st, d_u, ncell, ldp,
+  hipLaunchKernelGGL(fr2d_sensor_filter_kernel<NSP>, dim3((ncell + 255) / 256), dim3(256), 0, st, d_u, ncell,
+                     c.nx + 2, c.pitch, c.ldpi,
                      o.V, o.iV, mode, lambda, S0, kappa, (const float *)c.d_strength, nullptr, eval_out);
   FR_LAUNCHED("fr2d_sensor_filter_kernel");
   return FR_OK;
@@ -442,10 +445,9 @@ int fr2d_launch_filter_refresh_limit(const Fr2dCtx &c, double *d_u, int mode, do
 
 template <int NSP>
 static int check_nsp(const Fr2dCtx &c, const double *d_u, int *cnt, hipStream_t st) {
-  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
   const long long n = (long long)c.nx * c.ny;
   hipLaunchKernelGGL(fr2d_check_kernel<NSP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, d_u, c.nx, c.ny,
-                     ldp, c.gamma, cnt);
+                     c.pitch, c.ldpi, c.gamma, cnt);
   FR_LAUNCHED("fr2d_check_kernel");
   return FR_OK;
 }

@@ -44,7 +44,8 @@ enum { GEO_RECT = 0, GEO_AFFINE = 1 };
 struct Geo2D {
   int nx, ny, ntx, nty;
   int strip;      // tile-order strip width in tiles (strip_tile)
-  long long ldp;  // elements per (k,l,m) plane: (nx+2)(ny+2)
+  int pitch;      // elements per row of a plane: nx+2 (caller arrays) or the padded pitch
+  long long ldp;  // elements per (k,l,m) plane: pitch (ny+2)
   double iJ11, iJ22, g, gm1;
   int periodic;
   double iJ12, iJ21;          // off-diagonal inverse-Jacobian entries (GEO_AFFINE)
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
   const int line = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const int ex = lane % TX, ey = lane / TX;
-  const unsigned nxg = geo.nx + 2;
+  const unsigned nxg = geo.pitch;
   const long long ldp = geo.ldp;
   const double *u = sa.u;
   const double g = geo.g, gm1 = geo.gm1;
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
   const int i0 = 1 + bx * TX, j0 = 1 + by * TY;
   const int ncx = min(TX, geo.nx - i0 + 1), ncy = min(TY, geo.ny - j0 + 1);
   const bool act = ex < ncx && ey < ncy;
-  // byte offset of the own element inside a plane ((nx+2)(ny+2) < 2^29 on the host)
+  // byte offset of the own element inside a plane (pitch (ny+2) < 2^29 on the host)
   const unsigned ebo = 8u * ((unsigned)(i0 + ex) + nxg * (unsigned)(j0 + ey));
 
   if constexpr (EPI == EPI_RESIDUAL) {
@@ -558,10 +559,10 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
 
 // ------------------------------------------------------------------ ghost-layer helpers
 // rule FR_GHOST_PERIODIC / _OUTFLOW / _INFLOW_LEFT over every ghost cell and every (k,l,m) plane.
-__global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, long long ldp, int planes, int rule) {
+__global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, int pitch, long long ldp, int planes, int rule) {
   const int nring = 2 * (nx + 2) + 2 * ny;
   const long long total = (long long)nring * planes;
-  const long long nxg = nx + 2;
+  const long long nxg = pitch;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
        t += (long long)gridDim.x * blockDim.x) {
     const int g = (int)(t % nring);
@@ -610,7 +611,11 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   // strips of 16 tiles on grids of 64 tiles (1024 elements) across and wider, measured 3-9 %
   // faster there at nsp = 4; narrower grids keep row-major order
   geo.strip = geo.ntx >= 64 ? 16 : geo.ntx;
-  geo.ldp = (long long)(c.nx + 2) * (c.ny + 2);
+  // the drop-in residual reads and writes caller arrays in the reference layout; the stages work
+  // on the context's padded arrays
+  const bool caller = EPI == EPI_RESIDUAL;
+  geo.pitch = caller ? c.nx + 2 : c.pitch;
+  geo.ldp = caller ? (long long)(c.nx + 2) * (c.ny + 2) : c.ldpi;
   geo.iJ11 = c.iJ11;
   geo.iJ22 = c.iJ22;
   geo.g = c.gamma;
@@ -625,7 +630,7 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   geo.q1 = c.iJ11 * c.n1[0] + c.iJ12 * c.n1[1];
   geo.q2 = c.iJ21 * c.n2[0] + c.iJ22 * c.n2[1];
   const unsigned nb = (unsigned)(geo.ntx * geo.nty);  // one tile per workgroup
-  const bool buf = c.len * sizeof(double) < (size_t)0xFFFFFFFFu;
+  const bool buf = (caller ? c.len : c.lenp) * sizeof(double) < (size_t)0xFFFFFFFFu;
   if (buf && !aff)
     hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_BUFFER, GEO_RECT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
   else if (!aff)
@@ -694,11 +699,11 @@ int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, 
 }
 
 int fr2d_launch_fill_ghost(const Fr2dCtx &c, double *d_u, int rule, hipStream_t st) {
-  const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
   const int planes = c.nsp * c.nsp * 4;
   const long long total = (long long)(2 * (c.nx + 2) + 2 * c.ny) * planes;
   const int blocks = (int)std::min<long long>((total + 255) / 256, 4096);
-  hipLaunchKernelGGL(fr2d_fill_ghost_kernel, dim3(blocks), dim3(256), 0, st, d_u, c.nx, c.ny, ldp, planes, rule);
+  hipLaunchKernelGGL(fr2d_fill_ghost_kernel, dim3(blocks), dim3(256), 0, st, d_u, c.nx, c.ny, c.pitch, c.ldpi, planes,
+                     rule);
   FR_LAUNCHED("fr2d_fill_ghost_kernel");
   return FR_OK;
 }

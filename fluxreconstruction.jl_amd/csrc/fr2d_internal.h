@@ -14,7 +14,16 @@ struct Fr2dCtx {
   double iJ12 = 0.0, iJ21 = 0.0;
   double n1[2] = {1.0, 0.0}, n2[2] = {0.0, 1.0};
   HostOps ops;
-  size_t len = 0;  // doubles per state array
+  size_t len = 0;  // doubles per state array in the reference layout, (nx+2)(ny+2) nsp^2 4
+  // The device-resident arrays (state, stage buffers, RK4 accumulator) use a padded form of the
+  // reference layout: rows of `pitch` elements (nx+2 rounded up to a multiple of 16) and planes
+  // of ldpi = pitch (ny+2) elements, each array starting 15 doubles into its allocation.  Element
+  // (1, j) of every plane then starts a 128-byte line, so the 16-element row segments the kernels
+  // read and write are aligned (the reference pitch leaves every segment across two lines).
+  int pitch = 0;
+  long long ldpi = 0;
+  size_t lenp = 0;  // doubles per padded array, ldpi nsp^2 4
+  static constexpr int kPadFront = 15;
   int device = 0;
   int num_cu = 0;  // compute units of the device
   // the caller's stream (caller_stream: any hipStream_t, NULL being HIP's null stream) or the
@@ -22,8 +31,10 @@ struct Fr2dCtx {
   bool caller_stream = false;
   hipStream_t stream = nullptr;
   hipStream_t own_stream = nullptr;
-  // device state and stage buffers of the integrator path
+  // device state and stage buffers of the integrator path (padded layout; raw_* are the
+  // allocations, d_* = raw_* + kPadFront)
   double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_acc = nullptr;
+  double *raw_u = nullptr, *raw_s1 = nullptr, *raw_s2 = nullptr, *raw_acc = nullptr;
   // staging buffers of the host-pointer residual
   double *d_in = nullptr, *d_out = nullptr;
   // per-cell flags / counters of the dissipation kernels
@@ -43,6 +54,8 @@ struct Fr2dCtx {
 
 inline hipStream_t fr2d_stream(const Fr2dCtx &c) { return c.caller_stream ? c.stream : c.own_stream; }
 
+// Every launcher below works on the padded arrays of the context (pitch, ldpi), except
+// fr2d_launch_residual, whose u / du are caller arrays in the reference layout.
 int fr2d_launch_residual(const Fr2dCtx &c, const double *d_u, double *d_du, hipStream_t st);
 // ghost_copy0/1 (optional, ignored on periodic grids): buffers that receive d_us's ghost layer
 int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, double *d_out, double a0,

@@ -12,7 +12,7 @@ namespace fr {
 // i.e. the last entry is 2p/rho.  Plain IEEE division: this is output, not the hot path.
 template <int NSP>
 __global__ void fr2d_primitives_kernel(const double *__restrict__ u, double *__restrict__ sol, int nx, int ny,
-                                       long long ldp, double gm1) {
+                                       int pitch, long long ldp, double gm1) {
   const long long ne = (long long)nx * ny;
   const long long n = ne * NSP * NSP;
   const long long NX = (long long)nx * NSP, NXY = NX * ny * NSP;
@@ -23,7 +23,7 @@ __global__ void fr2d_primitives_kernel(const double *__restrict__ u, double *__r
     const int pt = (int)(t / ne);  // k + NSP*l
     const int i = (int)(e % nx) + 1, j = (int)(e / nx) + 1;
     const int k = pt % NSP, l = pt / NSP;
-    const long long b = i + (long long)(nx + 2) * j + ldp * pt;
+    const long long b = i + (long long)pitch * j + ldp * pt;
     const double rho = u[b], mx = u[b + mstride], my = u[b + 2 * mstride], E = u[b + 3 * mstride];
     const double U = mx / rho, V = my / rho;
     const double p = gm1 * (E - 0.5 * (mx * U + my * V));
@@ -39,8 +39,8 @@ template <int NSP>
 static int launch_prim(const Fr2dCtx &c, const double *d_u, double *d_sol, hipStream_t st) {
   const long long n = (long long)c.nx * c.ny * NSP * NSP;
   const int blocks = (int)std::min<long long>((n + 255) / 256, 8LL * std::max(c.num_cu, 1) * 4);
-  hipLaunchKernelGGL(fr2d_primitives_kernel<NSP>, dim3(blocks), dim3(256), 0, st, d_u, d_sol, c.nx, c.ny,
-                     (long long)(c.nx + 2) * (c.ny + 2), c.gamma - 1.0);
+  hipLaunchKernelGGL(fr2d_primitives_kernel<NSP>, dim3(blocks), dim3(256), 0, st, d_u, d_sol, c.nx, c.ny, c.pitch,
+                     c.ldpi, c.gamma - 1.0);
   FR_LAUNCHED("fr2d_primitives_kernel");
   return FR_OK;
 }

@@ -283,13 +283,14 @@ __global__ void mlp_adam_kernel(MlpDev w, float lr, float b1, float b2, float ep
 }
 
 // features of every cell: x = u_hat / |u_hat|, u_hat = iV * rho nodal (nsp = 4), BF16
-__global__ __launch_bounds__(256) void mlp_features_kernel(const double *u, int ncell, long long ldp,
+__global__ __launch_bounds__(256) void mlp_features_kernel(const double *u, int ncell, int nxg, int pitch, long long ldp,
                                                            const double *__restrict__ iV, unsigned short *X) {
-  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;  // cell, reference order
   if (e >= ncell) return;
+  const long long ep = e % nxg + (long long)pitch * (e / nxg);             // its place in a plane
   double nod[16], s2 = 0.0;
 #pragma unroll
-  for (int q = 0; q < 16; ++q) nod[q] = u[e + ldp * q];
+  for (int q = 0; q < 16; ++q) nod[q] = u[ep + ldp * q];
   double mod[16];
 #pragma unroll
   for (int a = 0; a < 16; ++a) {
@@ -388,7 +389,8 @@ int fr2d_launch_mlp_sensor(const Fr2dCtx &c, const double *d_u, hipStream_t st) 
   if (c.nsp != 4) return fail(FR_E_UNSUPPORTED, "learned sensor: trained for p = 3 (16 modes)");
   const long long ldp = (long long)(c.nx + 2) * (c.ny + 2);
   const DissOps o = fr2d_diss_ops(c);
-  hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)((ldp + 255) / 256)), dim3(256), 0, st, d_u, (int)ldp, ldp,
+  hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)((ldp + 255) / 256)), dim3(256), 0, st, d_u, (int)ldp,
+                     c.nx + 2, c.pitch, c.ldpi,
                      o.iV, (unsigned short *)c.d_feat);
   FR_LAUNCHED("mlp_features_kernel");
   const MlpCtx *m = (const MlpCtx *)c.d_mlp;
@@ -554,7 +556,8 @@ int fr2d_sensor_features(fr2d_ctx *c, uint16_t *d_x) {
   const long long ldp = (long long)(x->nx + 2) * (x->ny + 2);
   const DissOps o = fr2d_diss_ops(*x);
   hipStream_t st = fr2d_stream(*x);
-  hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)((ldp + 255) / 256)), dim3(256), 0, st, x->d_u, (int)ldp, ldp,
+  hipLaunchKernelGGL(mlp_features_kernel, dim3((unsigned)((ldp + 255) / 256)), dim3(256), 0, st, x->d_u, (int)ldp,
+                     x->nx + 2, x->pitch, x->ldpi,
                      o.iV, (unsigned short *)d_x);
   FR_LAUNCHED("mlp_features_kernel");
   return FR_OK;

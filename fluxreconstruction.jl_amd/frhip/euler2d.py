@@ -145,7 +145,15 @@ class FREuler2D:
         return out
 
     def state_ptr(self) -> int:
+        """Device pointer of the state, in the padded layout of state_pitch()."""
         return _lib.lib().fr2d_state_dev(self.h) or 0
+
+    def state_pitch(self):
+        """(pitch, plane): element (i, j) of plane p of the device state is at
+        i + pitch*j + plane*p (rows padded to a multiple of 16 elements)."""
+        p, q = C.c_int(0), C.c_longlong(0)
+        check(_lib.lib().fr2d_state_pitch(self.h, C.byref(p), C.byref(q)))
+        return p.value, q.value
 
     def step(self, dt: float, nsteps: int = 1, scheme: str = "ssprk3"):
         check(_lib.lib().fr2d_step(self.h, _lib.RK[scheme], float(dt), int(nsteps)))

@@ -124,7 +124,14 @@ int fr2d_set_state(fr2d_ctx *ctx, const double *u);  /* host -> device */
 int fr2d_get_state(fr2d_ctx *ctx, double *u);        /* device -> host (synchronises) */
 int fr2d_set_state_dev(fr2d_ctx *ctx, const double *d_u);
 int fr2d_get_state_dev(fr2d_ctx *ctx, double *d_u);
-double *fr2d_state_dev(fr2d_ctx *ctx);               /* device pointer of the state (or NULL) */
+/* Device pointer of the state (or NULL).  The device-resident arrays keep the reference layout
+ * with padded rows: element (i, j) of plane p is at i + pitch*j + plane*p, with pitch = nx+2
+ * rounded up to a multiple of 16 and plane = pitch*(ny+2) (fr2d_state_pitch), which aligns the
+ * 16-element row segments the kernels move to 128-byte lines.  fr2d_set_state(_dev) and
+ * fr2d_get_state(_dev) convert from / to the unpadded reference layout.  No reference
+ * counterpart (device interop). */
+double *fr2d_state_dev(fr2d_ctx *ctx);
+int fr2d_state_pitch(const fr2d_ctx *ctx, int *pitch, long long *plane);
 /* nsteps explicit steps of size dt with scheme FR_RK_*, replacing step!(itg) of OrdinaryDiffEq
  * (shock-vortex.jl:294,305; Midpoint there, SSP-RK3 / RK4 / Euler here too); each stage is one
  * fused residual+update launch.  In FR_BC_GHOST mode the ghost layer is held fixed (L = 0 there). */

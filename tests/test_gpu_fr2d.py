@@ -67,6 +67,36 @@ def test_wide_grid_strip_order(bc):
     assert rel_l2(s.get_state(), uc) <= STATE_TOL
 
 
+@pytest.mark.parametrize("nx,ny", [(19, 7), (30, 5), (46, 3)])
+def test_padded_device_state(nx, ny):
+    """The device-resident state keeps the reference layout with rows padded to a multiple of 16
+    elements: read through fr2d_state_dev with the pitch of fr2d_state_pitch it equals
+    get_state, and set_state / fr2d_get_state_dev convert device arrays both ways."""
+    import ctypes as C
+
+    import torch
+
+    from frhip import _lib
+
+    ps, u, s, orc = setup(nx, ny, 2, "hll", "ghost", seed=nx)
+    dev = torch.from_numpy(u.ravel(order="F").copy()).cuda()
+    s.set_state(dev)
+    back = torch.empty_like(dev)
+    _lib.check(_lib.lib().fr2d_get_state_dev(s.h, C.c_void_p(back.data_ptr())))
+    torch.cuda.synchronize()
+    assert torch.equal(back, dev)
+    pitch, plane = s.state_pitch()
+    assert pitch % 16 == 0 and nx + 2 <= pitch < nx + 2 + 16 and plane == pitch * (ny + 2)
+    # the padded array itself, copied out with the process's HIP runtime (device to device)
+    n = plane * 9 * 4
+    padded = torch.empty(n, dtype=torch.float64, device="cuda")
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    assert hip.hipMemcpy(C.c_void_p(padded.data_ptr()), C.c_void_p(s.state_ptr()), n * 8, 3) == 0
+    a = padded.cpu().numpy().reshape((pitch, ny + 2, 3, 3, 4), order="F")[: nx + 2]
+    assert np.array_equal(a, u)
+
+
 def test_residual_garbage_du_overwritten():
     ps, u, s, orc = setup(20, 9, 3, "rusanov", "periodic")
     du = np.full(s.shape, np.nan, order="F")
