@@ -608,9 +608,9 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   geo.ny = c.ny;
   geo.ntx = (c.nx + TX - 1) / TX;
   geo.nty = (c.ny + TY - 1) / TY;
-  // strips of 16 tiles on grids of 64 tiles (1024 elements) across and wider, measured 3-9 %
-  // faster there at nsp = 4; narrower grids keep row-major order
-  geo.strip = geo.ntx >= 64 ? 16 : geo.ntx;
+  // strips of 16 tiles on grids of 32 tiles (512 elements) across and wider, measured 1-9 %
+  // faster there at nsp = 4 (padded rows); narrower grids keep row-major order
+  geo.strip = geo.ntx >= 32 ? 16 : geo.ntx;
   // the drop-in residual reads and writes caller arrays in the reference layout; the stages work
   // on the context's padded arrays
   const bool caller = EPI == EPI_RESIDUAL;
