@@ -151,10 +151,14 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
   const int r = threadIdx.x & 3;   // variable held by this lane
   const int c = threadIdx.x >> 2;  // cell slot of the workgroup
   const int nxg = nx + 2;
-  const long long e = blockIdx.x * 64LL + c;  // cell in the reference order (strength index)
-  if (e >= (long long)nxg * (ny + 2)) return;  // whole quads leave together
-  const int i = (int)(e % nxg), j = (int)(e / nxg);
-  const long long ep = i + (long long)pitch * j;  // its place in a plane of the padded state
+  // The workgroups walk the padded plane in place order, shifted by the front offset of the
+  // array (Fr2dCtx::kPadFront), so that the 16 cells of a wavefront are one aligned 128-byte segment of every
+  // plane; padding cells and the places before the plane leave at once (whole quads together).
+  const long long ep = blockIdx.x * 64LL + c - Fr2dCtx::kPadFront;  // place in a padded plane
+  if (ep < 0 || ep >= ldp) return;
+  const int i = (int)(ep % pitch), j = (int)(ep / pitch);
+  if (i >= nxg) return;
+  const long long e = i + (long long)nxg * j;  // the cell in the reference order (strength index)
   const bool ghost = i == 0 || i == nx + 1 || j == 0 || j == ny + 1;
   if (ghost) {
     if (rule < 0) {
@@ -388,10 +392,9 @@ __global__ __launch_bounds__(256) void fr2d_check_kernel(const double *u, int nx
 template <int NSP>
 static int quad_nsp(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa, int rule,
                     int limit, int *nfilt, int *nfail, hipStream_t st) {
-  const long long ncell = (long long)(c.nx + 2) * (c.ny + 2);
   DissOps o = fr2d_diss_ops(c);
-  // four lanes per cell, 64 cells per workgroup
-  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((ncell + 63) / 64)), dim3(256), 0, st,
+  // four lanes per cell, 64 places of the padded plane per workgroup (shifted by the front offset)
+  hipLaunchKernelGGL(fr2d_filter_refresh_limit_kernel<NSP>, dim3((unsigned)((c.ldpi + Fr2dCtx::kPadFront + 63) / 64)), dim3(256), 0, st,
                      d_u, c.nx, c.ny, c.pitch, c.ldpi, o.V, o.iV, o.wts, o.ll, o.lr, mode, lambda, S0, kappa,
                      (const float *)c.d_strength, rule, limit, c.gamma, nfilt, nfail);
   FR_LAUNCHED("fr2d_filter_refresh_limit_kernel");
