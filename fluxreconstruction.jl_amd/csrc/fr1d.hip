@@ -471,11 +471,16 @@ struct Batch1D {
     for (int n = 0; n < nsteps; ++n) {
       int rc = FR_OK;
       switch (scheme) {
-        case FR_RK_SSPRK3:
-          if ((rc = stg(U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
-          if ((rc = stg(U, S1, S2, 0.75, 0.25, 0.25 * dt, nullptr, nullptr, 0.0))) return rc;
-          if ((rc = stg(U, S2, U, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, nullptr, nullptr, 0.0))) return rc;
+        case FR_RK_SSPRK3: {
+          // consecutive steps alternate the state between U and S1 (every stage writes every
+          // cell), so the last stage never writes the u^n it reads; the call's last step writes U
+          double *const X = (n & 1) ? S1 : U, *const Y = (n & 1) ? U : S1;
+          double *const out = n == nsteps - 1 ? U : Y;
+          if ((rc = stg(X, X, Y, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
+          if ((rc = stg(X, Y, S2, 0.75, 0.25, 0.25 * dt, nullptr, nullptr, 0.0))) return rc;
+          if ((rc = stg(X, S2, out, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, nullptr, nullptr, 0.0))) return rc;
           break;
+        }
         case FR_RK_RK4:
           if ((rc = stg(U, U, S2, 1.0, 0.0, 0.5 * dt, U, A, dt / 6.0))) return rc;
           if ((rc = stg(U, S2, S1, 1.0, 0.0, 0.5 * dt, A, A, dt / 3.0))) return rc;

@@ -462,11 +462,15 @@ int fr_bgk_step(fr_bgk_ctx *c, double dt, int nsteps) {
   FR_REQUIRE(c && nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr_bgk_step: bad arguments");
   if (!c->d_s1) FR_HIP(hipMalloc(&c->d_s1, c->len * sizeof(double)));
   if (!c->d_s2) FR_HIP(hipMalloc(&c->d_s2, c->len * sizeof(double)));
+  // Consecutive steps alternate the state between d_u and d_s1 (every stage writes every value),
+  // so the last stage never writes the u^n it reads; the call's last step writes d_u.
   for (int n = 0; n < nsteps; ++n) {
     int rc;
-    if ((rc = c->stage(BgkStage{c->d_u, c->d_u, c->d_s1, 0.0, 1.0, dt, 0}, c->stream))) return rc;
-    if ((rc = c->stage(BgkStage{c->d_s1, c->d_u, c->d_s2, 0.75, 0.25, 0.25 * dt, 0}, c->stream))) return rc;
-    if ((rc = c->stage(BgkStage{c->d_s2, c->d_u, c->d_u, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, 0}, c->stream)))
+    double *const X = (n & 1) ? c->d_s1 : c->d_u, *const Y = (n & 1) ? c->d_u : c->d_s1;
+    double *const out = n == nsteps - 1 ? c->d_u : Y;
+    if ((rc = c->stage(BgkStage{X, X, Y, 0.0, 1.0, dt, 0}, c->stream))) return rc;
+    if ((rc = c->stage(BgkStage{Y, X, c->d_s2, 0.75, 0.25, 0.25 * dt, 0}, c->stream))) return rc;
+    if ((rc = c->stage(BgkStage{c->d_s2, X, out, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, 0}, c->stream)))
       return rc;
   }
   FR_HIP(hipStreamSynchronize(c->stream));

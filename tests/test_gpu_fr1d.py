@@ -128,6 +128,11 @@ def test_ensemble_batched_equals_looped_and_single():
     e.step(dt, 20, looped=True)
     looped = e.get_state()
     assert np.array_equal(batched, looped)
+    # steps of one call alternate the state between two arrays: odd splits give the same state
+    e.set_state(flat)
+    for k in (7, 1, 9, 3):
+        e.step(dt, k)
+    assert np.array_equal(e.get_state(), batched)
     # member 3 advanced alone through the single-problem context
     b = 3
     ps = frhip.FRPSpace1D(0.0, 1.0, int(ncell[b]), int(deg[b]))
