@@ -487,10 +487,14 @@ struct Batch1D {
           if ((rc = stg(U, S1, S2, 1.0, 0.0, dt, A, A, dt / 3.0))) return rc;
           if ((rc = stg(A, S2, U, 1.0, 0.0, dt / 6.0, nullptr, nullptr, 0.0))) return rc;
           break;
-        case FR_RK_MIDPOINT:
-          if ((rc = stg(U, U, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0))) return rc;
-          if ((rc = stg(U, S1, U, 1.0, 0.0, dt, nullptr, nullptr, 0.0))) return rc;
+        case FR_RK_MIDPOINT: {
+          // consecutive steps alternate the state between U and S2; the call's last step writes U
+          double *const X = (n & 1) ? S2 : U;
+          double *const out = n == nsteps - 1 ? U : ((n & 1) ? U : S2);
+          if ((rc = stg(X, X, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0))) return rc;
+          if ((rc = stg(X, S1, out, 1.0, 0.0, dt, nullptr, nullptr, 0.0))) return rc;
           break;
+        }
         case FR_RK_EULER:
           if ((rc = stg(U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
           FR_HIP(hipMemcpyAsync(U, S1, len * sizeof(double), hipMemcpyDeviceToDevice, st));

@@ -95,10 +95,17 @@ static int enqueue_steps(Fr2dCtx &c, int scheme, double dt, int nsteps, hipStrea
         if ((rc = fr2d_launch_stage(c, A, S2, U, 1.0, 0.0, dt / 6.0, nullptr, nullptr, 0.0, st))) return rc;
         break;
       }
-      case FR_RK_MIDPOINT:
-        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0, st, S1, S2))) return rc;
-        if ((rc = fr2d_launch_stage(c, U, S1, U, 1.0, 0.0, dt, nullptr, nullptr, 0.0, st))) return rc;
+      case FR_RK_MIDPOINT: {
+        // as SSP-RK3: consecutive steps alternate the state between U and S2, the call's last
+        // step writes U (the first stage gives both S1 and the step's output its ghost layer)
+        double *const X = (n & 1) ? S2 : U;
+        double *const out = n == nsteps - 1 ? U : ((n & 1) ? U : S2);
+        if ((rc = fr2d_launch_stage(c, X, X, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0, st, S1,
+                                    out == X ? S2 : out)))
+          return rc;
+        if ((rc = fr2d_launch_stage(c, X, S1, out, 1.0, 0.0, dt, nullptr, nullptr, 0.0, st))) return rc;
         break;
+      }
       case FR_RK_EULER:
         if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st, S1, S2))) return rc;
         FR_HIP(hipMemcpyAsync(U, S1, c.lenp * sizeof(double), hipMemcpyDeviceToDevice, st));

@@ -109,6 +109,29 @@ def test_euler_ssprk3_parity():
     assert np.linalg.norm(g.get_state() - uc) / np.linalg.norm(uc) <= 1e-10
 
 
+def test_euler_midpoint_split_calls():
+    """Midpoint steps of one call alternate the state between two arrays: 7 steps in one call,
+    as 3 + 4 and as single steps give the same state bitwise, within 1e-10 of the oracle."""
+    ncell, deg = 150, 2
+    ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
+    u0 = euler_state(ncell, deg, seed=6)
+    g = frhip.FREulerProblem(ps, 1.4, "period", "rusanov")
+    dt = 0.02 / ncell
+    states = []
+    for calls in ((7,), (3, 4), (1,) * 7):
+        g.set_state(u0)
+        for k in calls:
+            g.step(dt, k, "midpoint")
+        states.append(g.get_state())
+    assert np.array_equal(states[0], states[1]) and np.array_equal(states[0], states[2])
+    ops = oracle_ops(deg)
+    L = lambda v: O.fr1d_euler_rhs(v, ps.J, ops, 1.4, "rusanov", "period")  # noqa: E731
+    uc = u0.copy(order="F")
+    for _ in range(7):
+        uc = uc + dt * L(uc + (0.5 * dt) * L(uc))
+    assert np.linalg.norm(states[0] - uc) / np.linalg.norm(uc) <= 1e-10
+
+
 def _ensemble(nprob, seed=7):
     # Riemann data ramped over 2 cells: the plain (unlimited) scheme stays finite on every member,
     # which the exact piecewise-constant jumps do not guarantee at p = 4

@@ -187,28 +187,29 @@ def test_graph_replay_matches_eager():
     assert np.array_equal(s.get_state(), s2.get_state())
 
 
+@pytest.mark.parametrize("scheme", ["ssprk3", "midpoint"])
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
-def test_ssprk3_state_alternation(bc):
-    """SSP-RK3 steps of one call alternate the state between two device arrays and the call's
-    last step writes the state array: any split of 7 steps into calls (single steps, 3 + 4, one
-    call of 7, a 2-step graph replayed with an odd eager step after it) gives the same state
-    bitwise, and the ghost layer stays that of the initial state."""
+def test_state_alternation(scheme, bc):
+    """SSP-RK3 and midpoint steps of one call alternate the state between two device arrays and
+    the call's last step writes the state array: any split of 7 steps into calls (single steps,
+    3 + 4, one call of 7, a 2-step graph replayed with an odd eager step after it) gives the same
+    state bitwise, and the ghost layer stays that of the initial state."""
     ps, u, s, orc = setup(20, 12, 3, "hll", bc, amp=0.01, seed=9)
     dt, _ = ics.cfl_steps(u, 1.4, ps.dx0, 0.05, 1.0)
     states = []
     for calls in ((1,) * 7, (3, 4), (7,)):
         s.set_state(u)
         for k in calls:
-            s.step(dt, k)
+            s.step(dt, k, scheme)
         states.append(s.get_state())
     s.set_state(u)
-    s.step_graph(dt, 2, 3)
-    s.step(dt, 1)
+    s.step_graph(dt, 2, 3, scheme)
+    s.step(dt, 1, scheme)
     states.append(s.get_state())
     for st in states[1:]:
         assert np.array_equal(st, states[0])
     uc = u.copy(order="F")
-    orc.ssprk3(uc, dt, 7)
+    getattr(orc, scheme)(uc, dt, 7)
     assert rel_l2(states[0], uc) <= STATE_TOL
     if bc == "ghost":
         assert np.array_equal(states[0][0], u[0]) and np.array_equal(states[0][:, -1], u[:, -1])
