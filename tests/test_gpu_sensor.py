@@ -106,11 +106,13 @@ def test_solver_features_and_learned_filter():
     assert np.max(np.abs(ug - uc)) <= 1e-13 * np.max(np.abs(uc))
 
 
+@pytest.mark.parametrize("bc,iters", [("ghost", 12), ("ghost", 7), ("periodic", 5)])
 @pytest.mark.parametrize("mode", ["off", "modal", "learned"])
-def test_fused_filter_refresh_limit_equals_host_loop(mode):
+def test_fused_filter_refresh_limit_equals_host_loop(mode, bc, iters):
     """fr2d_filter_refresh_limit (filter, inflow-keeping ghost refresh and the next limiter in one
     pass) and the natively enqueued loop fr2d_run_host_loop reproduce the unfused host loop of
-    example/shock-vortex.jl:296-326 bitwise."""
+    example/shock-vortex.jl:296-326 bitwise, for odd and even iteration counts and on periodic
+    grids (whose ghost layer the loop refills after every step)."""
     import os
 
     ps = frhip.FRPSpace2D(0.0, 2.0, 48, 0.0, 1.0, 24, 3, 1, 1)
@@ -123,11 +125,11 @@ def test_fused_filter_refresh_limit_equals_host_loop(mode):
         mlp = S.SensorMLP.load()
     out = []
     for fused, native in ((False, False), (True, False), (True, True)):
-        s = frhip.FREuler2D(ps, 1.4, "hll", "ghost")
+        s = frhip.FREuler2D(ps, 1.4, "hll", bc)
         s.set_state(u0)
         if mlp is not None:
             S.attach(s, mlp)
-        S.run_shock_vortex(s, dt, 12, mode, fused=fused, native=native)
+        S.run_shock_vortex(s, dt, iters, mode, fused=fused, native=native)
         out.append(s.get_state())
         s.close()
     assert np.all(np.isfinite(out[0]))
