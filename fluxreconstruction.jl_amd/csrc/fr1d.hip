@@ -495,10 +495,14 @@ struct Batch1D {
           if ((rc = stg(X, S1, out, 1.0, 0.0, dt, nullptr, nullptr, 0.0))) return rc;
           break;
         }
-        case FR_RK_EULER:
-          if ((rc = stg(U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
-          FR_HIP(hipMemcpyAsync(U, S1, len * sizeof(double), hipMemcpyDeviceToDevice, st));
+        case FR_RK_EULER: {
+          // consecutive steps alternate the state between U and S1; a call that ends in S1
+          // copies it back once
+          double *const X = (n & 1) ? S1 : U, *const Y = (n & 1) ? U : S1;
+          if ((rc = stg(X, X, Y, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
+          if (n == nsteps - 1 && Y != U) FR_HIP(hipMemcpyAsync(U, Y, len * sizeof(double), hipMemcpyDeviceToDevice, st));
           break;
+        }
         default:
           return fail(FR_E_INVALID, "fr1d_step: unknown scheme");
       }

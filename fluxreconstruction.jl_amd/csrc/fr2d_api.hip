@@ -106,10 +106,15 @@ static int enqueue_steps(Fr2dCtx &c, int scheme, double dt, int nsteps, hipStrea
         if ((rc = fr2d_launch_stage(c, X, S1, out, 1.0, 0.0, dt, nullptr, nullptr, 0.0, st))) return rc;
         break;
       }
-      case FR_RK_EULER:
-        if ((rc = fr2d_launch_stage(c, U, U, S1, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st, S1, S2))) return rc;
-        FR_HIP(hipMemcpyAsync(U, S1, c.lenp * sizeof(double), hipMemcpyDeviceToDevice, st));
+      case FR_RK_EULER: {
+        // consecutive steps alternate the state between U and S1 (a stage cannot write the array
+        // whose halo it reads); a call that ends in S1 copies it back once
+        double *const X = (n & 1) ? S1 : U, *const Y = (n & 1) ? U : S1;
+        if ((rc = fr2d_launch_stage(c, X, X, Y, 0.0, 1.0, dt, nullptr, nullptr, 0.0, st, Y, S2))) return rc;
+        if (n == nsteps - 1 && Y != U)
+          FR_HIP(hipMemcpyAsync(U, Y, c.lenp * sizeof(double), hipMemcpyDeviceToDevice, st));
         break;
+      }
       default:
         return fail(FR_E_INVALID, "fr2d_step: unknown scheme");
     }

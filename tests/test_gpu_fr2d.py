@@ -187,13 +187,13 @@ def test_graph_replay_matches_eager():
     assert np.array_equal(s.get_state(), s2.get_state())
 
 
-@pytest.mark.parametrize("scheme", ["ssprk3", "midpoint"])
+@pytest.mark.parametrize("scheme", ["ssprk3", "midpoint", "euler"])
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_state_alternation(scheme, bc):
-    """SSP-RK3 and midpoint steps of one call alternate the state between two device arrays and
-    the call's last step writes the state array: any split of 7 steps into calls (single steps,
-    3 + 4, one call of 7, a 2-step graph replayed with an odd eager step after it) gives the same
-    state bitwise, and the ghost layer stays that of the initial state."""
+    """SSP-RK3, midpoint and Euler steps of one call alternate the state between two device
+    arrays and the call ends with the state in its own array: any split of 7 steps into calls
+    (single steps, 3 + 4, one call of 7, a 2-step graph replayed with an odd eager step after it)
+    gives the same state bitwise, and the ghost layer stays that of the initial state."""
     ps, u, s, orc = setup(20, 12, 3, "hll", bc, amp=0.01, seed=9)
     dt, _ = ics.cfl_steps(u, 1.4, ps.dx0, 0.05, 1.0)
     states = []
@@ -209,7 +209,13 @@ def test_state_alternation(scheme, bc):
     for st in states[1:]:
         assert np.array_equal(st, states[0])
     uc = u.copy(order="F")
-    getattr(orc, scheme)(uc, dt, 7)
+    if scheme == "euler":
+        for _ in range(7):
+            uc = orc.stage(uc, uc, 0.0, 1.0, dt)
+        if bc == "periodic":
+            orc.fill_periodic(uc)
+    else:
+        getattr(orc, scheme)(uc, dt, 7)
     assert rel_l2(states[0], uc) <= STATE_TOL
     if bc == "ghost":
         assert np.array_equal(states[0][0], u[0]) and np.array_equal(states[0][:, -1], u[:, -1])
