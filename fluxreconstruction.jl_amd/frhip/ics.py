@@ -36,17 +36,26 @@ def _to_state(w):
     return np.asfortranarray(w, dtype=np.float64)
 
 
-def isentropic_vortex(ps, gamma=1.4, strength=5.0, center=None, rho_inf=1.0, u_inf=1.0, v_inf=1.0, p_inf=1.0):
+def isentropic_vortex(ps, gamma=1.4, strength=5.0, center=None, rho_inf=1.0, u_inf=1.0, v_inf=1.0, p_inf=1.0,
+                      t=0.0):
     """Isentropic vortex of Shu (strength eps, unit radius) on uniform flow, at every solution
     point including the ghost layer:
         du = -eps/(2 pi) e^{(1-r^2)/2} (y - yc),  dv = eps/(2 pi) e^{(1-r^2)/2} (x - xc),
         T = T_inf - (gamma-1) eps^2 / (8 gamma pi^2) e^{1-r^2},  rho = rho_inf (T/T_inf)^{1/(gamma-1)},
-        p = rho T   (T = p/rho)."""
+        p = rho T   (T = p/rho).
+    t > 0 gives the exact solution at time t on the periodic domain of ps: the initial vortex
+    carried by (u_inf, v_inf) t, every point measured from the periodic image of the centre
+    nearest to it (minimum-image distance)."""
     if center is None:
         center = (0.5 * (ps.x0 + ps.x1), 0.5 * (ps.y0 + ps.y1))
     x = ps.xpg[..., 0]
     y = ps.xpg[..., 1]
     xc, yc = center
+    if t:
+        Lx, Ly = ps.x1 - ps.x0, ps.y1 - ps.y0
+        # offsets from the carried centre wrapped into [-L/2, L/2)
+        x = xc + np.mod(x - (xc + u_inf * t) + 0.5 * Lx, Lx) - 0.5 * Lx
+        y = yc + np.mod(y - (yc + v_inf * t) + 0.5 * Ly, Ly) - 0.5 * Ly
     r2 = (x - xc) ** 2 + (y - yc) ** 2
     T_inf = p_inf / rho_inf
     e = strength / (2.0 * np.pi) * np.exp(0.5 * (1.0 - r2))
