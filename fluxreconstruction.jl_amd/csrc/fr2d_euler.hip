@@ -137,11 +137,14 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
     else
       return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(base + ldp * plane) + bo);
   };
+  // stage results go to the padded arrays (aligned row segments) and are read by the next launch
+  // only: stored through L2 (kStoreAgent); residuals go to caller arrays in the reference layout
+  constexpr int kStAux = EPI == EPI_RESIDUAL ? 0 : kStoreAgent;
   auto ST = [&](double *base, int k, int l, int m, unsigned bo, double v) {
     const int plane = k + NSP * (l + NSP * m);
     if constexpr (ADDR == ADDR_BUFFER)
       __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(fr_u2, v), plane_rsrc(base, 0xFFFFFFFFu), bo,
-                                            (int)(ldp8 * (unsigned)plane), 0);
+                                            (int)(ldp8 * (unsigned)plane), kStAux);
     else
       *reinterpret_cast<double *>(reinterpret_cast<char *>(base + ldp * plane) + bo) = v;
   };
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
   auto STL = [&](double *base, int pl, int m, unsigned bo, double v) {
     if constexpr (ADDR == ADDR_BUFFER)
       __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(fr_u2, v), plane_rsrc(base, 0xFFFFFFFFu),
-                                            bo + ldp8 * (unsigned)pl, (int)(ldp8 * (unsigned)(NSP * NSP * m)), 0);
+                                            bo + ldp8 * (unsigned)pl, (int)(ldp8 * (unsigned)(NSP * NSP * m)), kStAux);
     else
       *reinterpret_cast<double *>(reinterpret_cast<char *>(base + ldp * (pl + NSP * NSP * m)) + bo) = v;
   };
