@@ -352,6 +352,10 @@ struct Batch1D {
   Work1D *d_work[3] = {nullptr, nullptr, nullptr};
   int nwork[3] = {0, 0, 0};
   std::vector<int> member_first[3], member_count[3];
+  // the last step loop captured by fr_ens_run_graph, replayed while its parameters match
+  hipGraphExec_t gexec = nullptr;
+  int g_opts = -1, g_nsteps = 0, g_threads = 0, g_grid = -1;
+  double g_dt = 0.0;
 
   int build(int flux_, int eq_) {
     flux = flux_;
@@ -414,6 +418,7 @@ struct Batch1D {
     if (d_count) (void)hipFree(d_count);
     for (auto *w : d_work)
       if (w) (void)hipFree(w);
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
 
@@ -461,13 +466,37 @@ struct Batch1D {
     return FR_OK;
   }
 
-  int enqueue_steps(int scheme, int mode, double dt, int nsteps, hipStream_t st) {
+  // Per-cell dissipation pass over every member (mode 0: one launch; 1: one launch per member):
+  // dmode 1 sensor + filter with the filter time step dt, 2 positivity limiter.  count[0] += cells
+  // flagged / limited, count[1] += cells whose mean density is not positive (left unchanged).
+  int diss_launch(double *u, int dmode, double dt, int mode, hipStream_t st) const {
+    const int v = tindex(threads);
+    auto one = [&](int w0, int nw, int grid_) {
+      if (nw <= 0) return;
+      const int g = grid_ > 0 ? std::min(grid_, nw) : nw;
+      hipLaunchKernelGGL(fr1d_diss_kernel, dim3(g), dim3(threads), 0, st, d_probs, d_work[v] + w0, nw, d_ops, u, dmode,
+                         gamma, dt, d_count);
+    };
+    if (mode == 0) {
+      one(0, nwork[v], grid);
+    } else {
+      for (size_t b = 0; b < probs.size(); ++b) one(member_first[v][b], member_count[v][b], 0);
+    }
+    FR_LAUNCHED("fr1d_diss_kernel");
+    return FR_OK;
+  }
+
+  // opts (SSP-RK3 only): FR_ENS_FILTER runs the sensor + filter on the state before every step
+  // (example/euler_highlevel.jl:37-54), FR_ENS_LIMIT the positivity limiter on the input of every
+  // stage (the state before the step and the two intermediate stage results)
+  int enqueue_steps(int scheme, int mode, double dt, int nsteps, hipStream_t st, int opts = 0) {
     double *U = d_u, *S1 = d_s1, *S2 = d_s2, *A = d_acc;
     auto stg = [&](const double *u0, const double *us, double *out, double a0, double a1, double b,
                    const double *ai, double *ao, double b2) {
       Stage1D s{us, out, u0, ai, ao, a0, a1, b, b2, 0};
       return stage(s, mode, st);
     };
+    const bool filt = (opts & FR_ENS_FILTER) != 0, lim = (opts & FR_ENS_LIMIT) != 0;
     for (int n = 0; n < nsteps; ++n) {
       int rc = FR_OK;
       switch (scheme) {
@@ -476,8 +505,12 @@ struct Batch1D {
           // cell), so the last stage never writes the u^n it reads; the call's last step writes U
           double *const X = (n & 1) ? S1 : U, *const Y = (n & 1) ? U : S1;
           double *const out = n == nsteps - 1 ? U : Y;
+          if (filt && (rc = diss_launch(X, 1, dt, mode, st))) return rc;
+          if (lim && (rc = diss_launch(X, 2, 0.0, mode, st))) return rc;
           if ((rc = stg(X, X, Y, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
+          if (lim && (rc = diss_launch(Y, 2, 0.0, mode, st))) return rc;
           if ((rc = stg(X, Y, S2, 0.75, 0.25, 0.25 * dt, nullptr, nullptr, 0.0))) return rc;
+          if (lim && (rc = diss_launch(S2, 2, 0.0, mode, st))) return rc;
           if ((rc = stg(X, S2, out, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, nullptr, nullptr, 0.0))) return rc;
           break;
         }
@@ -514,11 +547,8 @@ struct Batch1D {
   int dissipation(int mode, double dt, int *n_out) {
     if (!d_count) FR_HIP(hipMalloc(&d_count, 2 * sizeof(int)));
     FR_HIP(hipMemsetAsync(d_count, 0, 2 * sizeof(int), stream));
-    const int v = tindex(threads);
-    const int g = grid > 0 ? std::min(grid, nwork[v]) : nwork[v];
-    hipLaunchKernelGGL(fr1d_diss_kernel, dim3(g), dim3(threads), 0, stream, d_probs, d_work[v], nwork[v], d_ops, d_u,
-                       mode, gamma, dt, d_count);
-    FR_LAUNCHED("fr1d_diss_kernel");
+    int rc = diss_launch(d_u, mode, dt, 0, stream);
+    if (rc) return rc;
     int h[2] = {0, 0};
     FR_HIP(hipMemcpyAsync(h, d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, stream));
     FR_HIP(hipStreamSynchronize(stream));
@@ -705,32 +735,77 @@ int fr_ens_step(fr_ens_ctx *c, int mode, double dt, int nsteps) {
   return FR_OK;
 }
 
-int fr_ens_step_graph(fr_ens_ctx *c, double dt, int nsteps, int nrep) {
-  FR_REQUIRE(c && nsteps >= 1 && nrep >= 0 && std::isfinite(dt), FR_E_INVALID, "fr_ens_step_graph: bad arguments");
+// counters of the loop entry points: zeroed before the loop, read after it
+static int ens_read_counts(fr_ens_ctx *c, const char *who, int *n_flagged, int *n_limited) {
+  int h[2] = {0, 0};
+  FR_HIP(hipMemcpyAsync(h, c->d_count, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  if (n_flagged) *n_flagged = h[0];
+  if (n_limited) *n_limited = h[1];
+  if (h[1] > 0)
+    return fail(FR_E_LIMITER, std::string(who) + ": " + std::to_string(h[1]) +
+                                  " limiter failure(s) (cell mean density not positive) over the loop");
+  return FR_OK;
+}
+
+int fr_ens_run(fr_ens_ctx *c, int mode, int opts, double dt, int nsteps, int *n_flagged, int *n_limited) {
+  FR_REQUIRE(c && (mode == 0 || mode == 1) && nsteps >= 0 && std::isfinite(dt) && dt >= 0.0, FR_E_INVALID,
+             "fr_ens_run: bad arguments");
+  FR_REQUIRE((opts & ~(FR_ENS_FILTER | FR_ENS_LIMIT)) == 0, FR_E_INVALID, "fr_ens_run: unknown option bits");
   int rc = c->ensure_stage_buffers(false);
   if (rc) return rc;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t exec = nullptr;
-  FR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  rc = c->enqueue_steps(FR_RK_SSPRK3, 0, dt, nsteps, c->stream);
-  hipError_t e = hipStreamEndCapture(c->stream, &graph);
-  if (rc) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rc;
-  }
-  FR_HIP(e);
-  e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  if (e != hipSuccess) {
+  if (!c->d_count) FR_HIP(hipMalloc(&c->d_count, 2 * sizeof(int)));
+  FR_HIP(hipMemsetAsync(c->d_count, 0, 2 * sizeof(int), c->stream));
+  if ((rc = c->enqueue_steps(FR_RK_SSPRK3, mode, dt, nsteps, c->stream, opts))) return rc;
+  return ens_read_counts(c, "fr_ens_run", n_flagged, n_limited);
+}
+
+int fr_ens_run_graph(fr_ens_ctx *c, int opts, double dt, int nsteps, int nrep, int *n_flagged, int *n_limited) {
+  FR_REQUIRE(c && nsteps >= 1 && nrep >= 0 && std::isfinite(dt) && dt >= 0.0, FR_E_INVALID,
+             "fr_ens_run_graph: bad arguments");
+  FR_REQUIRE((opts & ~(FR_ENS_FILTER | FR_ENS_LIMIT)) == 0, FR_E_INVALID, "fr_ens_run_graph: unknown option bits");
+  int rc = c->ensure_stage_buffers(false);
+  if (rc) return rc;
+  if (!c->d_count) FR_HIP(hipMalloc(&c->d_count, 2 * sizeof(int)));
+  // The graph holds the context's own buffers and work lists, which never move once allocated, so a
+  // capture stays valid while the options, dt, the step count and the work mapping are unchanged.
+  if (!(c->gexec && c->g_opts == opts && c->g_dt == dt && c->g_nsteps == nsteps && c->g_threads == c->threads &&
+        c->g_grid == c->grid)) {
+    if (c->gexec) {
+      FR_HIP(hipStreamSynchronize(c->stream));
+      (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    FR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    rc = c->enqueue_steps(FR_RK_SSPRK3, 0, dt, nsteps, c->stream, opts);
+    hipError_t e = hipStreamEndCapture(c->stream, &graph);
+    if (rc) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    FR_HIP(e);
+    e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    if (e != hipSuccess) {
+      c->gexec = nullptr;
+      return hip_fail(e, "hipGraphInstantiate", __FILE__, __LINE__);
+    }
+    c->g_opts = opts;
+    c->g_dt = dt;
+    c->g_nsteps = nsteps;
+    c->g_threads = c->threads;
+    c->g_grid = c->grid;
   }
-  for (int r = 0; r < nrep && e == hipSuccess; ++r) e = hipGraphLaunch(exec, c->stream);
-  hipError_t e2 = hipStreamSynchronize(c->stream);
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
-  FR_HIP(e);
-  FR_HIP(e2);
-  return FR_OK;
+  // the counters are cleared outside the graph (no memset node is recorded into it)
+  FR_HIP(hipMemsetAsync(c->d_count, 0, 2 * sizeof(int), c->stream));
+  for (int r = 0; r < nrep; ++r) FR_HIP(hipGraphLaunch(c->gexec, c->stream));
+  return ens_read_counts(c, "fr_ens_run_graph", n_flagged, n_limited);
+}
+
+int fr_ens_step_graph(fr_ens_ctx *c, double dt, int nsteps, int nrep) {
+  FR_REQUIRE(c && nsteps >= 1 && nrep >= 0 && std::isfinite(dt), FR_E_INVALID, "fr_ens_step_graph: bad arguments");
+  return fr_ens_run_graph(c, 0, dt, nsteps, nrep, nullptr, nullptr);
 }
 
 // Times candidate (threads per workgroup, grid) mappings with HIP events over a few SSP-RK3

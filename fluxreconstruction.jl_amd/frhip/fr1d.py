@@ -38,7 +38,11 @@ _lib.SIGNATURES.update({
     "fr1d_positivity": (C.c_int, [_vp, _ip]),
     "fr_ens_sensor_filter": (C.c_int, [_vp, C.c_double, _ip]),
     "fr_ens_positivity": (C.c_int, [_vp, _ip]),
+    "fr_ens_run": (C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _ip]),
+    "fr_ens_run_graph": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _ip]),
 })
+
+ENS_FILTER, ENS_LIMIT = 1, 2
 
 
 def _bind():
@@ -47,7 +51,8 @@ def _bind():
                  "fr1d_set_state",
                  "fr1d_get_state", "fr1d_step", "fr_ens_create", "fr_ens_destroy", "fr_ens_state_len", "fr_ens_offset",
                  "fr_ens_set_state", "fr_ens_get_state", "fr_ens_step", "fr_ens_step_graph", "fr_ens_autotune",
-                 "fr1d_sensor_filter", "fr1d_positivity", "fr_ens_sensor_filter", "fr_ens_positivity"):
+                 "fr1d_sensor_filter", "fr1d_positivity", "fr_ens_sensor_filter", "fr_ens_positivity", "fr_ens_run",
+                 "fr_ens_run_graph"):
         res, args = _lib.SIGNATURES[name]
         fn = getattr(L, name)
         fn.restype = res
@@ -199,6 +204,24 @@ class Ensemble1D:
 
     def step_graph(self, dt: float, nsteps: int, nrep: int = 1):
         check(_lib.lib().fr_ens_step_graph(self.h, float(dt), int(nsteps), int(nrep)))
+
+    def run(self, dt: float, nsteps: int, filter: bool = True, limit: bool = False, looped: bool = False):
+        """The reference's per-step 1D loop (example/euler_highlevel.jl:37-54) over every member:
+        sensor + filter before every SSP-RK3 step (filter), positivity limiter on the input of every
+        stage (limit).  Returns (filtered cells, limited cells) summed over the loop."""
+        nf, nl = C.c_int(0), C.c_int(0)
+        opts = (ENS_FILTER if filter else 0) | (ENS_LIMIT if limit else 0)
+        check(_lib.lib().fr_ens_run(self.h, 1 if looped else 0, opts, float(dt), int(nsteps), C.byref(nf),
+                                    C.byref(nl)))
+        return nf.value, nl.value
+
+    def run_graph(self, dt: float, nsteps: int, nrep: int = 1, filter: bool = True, limit: bool = False):
+        """run() of nsteps steps captured once in a hipGraph (kept by the context) and replayed nrep
+        times; nrep = 0 only captures."""
+        nf, nl = C.c_int(0), C.c_int(0)
+        opts = (ENS_FILTER if filter else 0) | (ENS_LIMIT if limit else 0)
+        check(_lib.lib().fr_ens_run_graph(self.h, opts, float(dt), int(nsteps), int(nrep), C.byref(nf), C.byref(nl)))
+        return nf.value, nl.value
 
     def sensor_filter(self, dt: float) -> int:
         n = C.c_int(0)

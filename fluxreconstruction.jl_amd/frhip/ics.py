@@ -165,12 +165,13 @@ def prim_conserve1(prim, gamma):
     return w
 
 
-def riemann_ensemble(nprob=512, seed=2024, gamma=1.4, ncell_range=(64, 2048), degs=(2, 3, 4), smooth=0.0):
+def riemann_ensemble(nprob=512, seed=2024, gamma=1.4, ncell_range=(64, 2048), degs=(2, 3, 4)):
     """Seeded ensemble of 1D Riemann problems on [0, 1]: member b has ncell[b] uniform cells of
     degree deg[b]; piecewise-constant states rho, p in [0.1, 1], u in [-0.5, 0.5] left / right of
-    an interface x0 in [0.3, 0.7].  smooth > 0 replaces the jump by a tanh ramp of that many cells
-    (primitive variables blended), which keeps the unlimited scheme finite for parity runs.
-    Returns (ncell, deg, dx, [u_b]) with u_b(i, p, k)."""
+    an interface x0 in [0.3, 0.7].  As in the reference's Riemann setup
+    (example/euler_highlevel.jl:22-31, `ps.x[i] <= 0.5`), every solution point of a cell takes the
+    state of the cell's centre, so the jump lies on the cell face nearest x0 and each cell starts
+    constant.  Returns (ncell, deg, dx, [u_b]) with u_b(i, p, k)."""
     rng = np.random.default_rng(seed)
     ncell = rng.integers(ncell_range[0], ncell_range[1] + 1, size=nprob).astype(np.int32)
     deg = rng.choice(np.asarray(degs), size=nprob).astype(np.int32)
@@ -180,13 +181,8 @@ def riemann_ensemble(nprob=512, seed=2024, gamma=1.4, ncell_range=(64, 2048), de
     members = []
     for b in range(nprob):
         n, d = int(ncell[b]), int(deg[b])
-        r, _ = np.polynomial.legendre.leggauss(d + 1)
         xc = (np.arange(n) + 0.5) * dx[b]
-        x = xc[:, None] + 0.5 * dx[b] * r[None, :]
-        if smooth > 0.0:
-            h = 0.5 * (1.0 + np.tanh((x - x0[b]) / (smooth * dx[b])))
-        else:
-            h = (x > x0[b]).astype(np.float64)
+        h = np.repeat((xc > x0[b]).astype(np.float64)[:, None], d + 1, axis=1)
         mix = lambda k: (1.0 - h) * states[b, 0, k] + h * states[b, 1, k]  # noqa: E731
         rho = 0.1 + 0.9 * mix(0)
         vel = -0.5 + mix(1)

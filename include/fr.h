@@ -277,8 +277,20 @@ int fr_ens_set_state(fr_ens_ctx *ctx, const double *u);  /* concatenated member 
 int fr_ens_get_state(fr_ens_ctx *ctx, double *u);
 /* mode 0: one batched launch per stage; 1: one launch per member per stage (looped baseline) */
 int fr_ens_step(fr_ens_ctx *ctx, int mode, double dt, int nsteps);
-/* capture nsteps batched steps in a hipGraph and replay nrep times */
+/* capture nsteps batched steps in a hipGraph and replay nrep times (fr_ens_run_graph, no options) */
 int fr_ens_step_graph(fr_ens_ctx *ctx, double dt, int nsteps, int nrep);
+/* The per-step host loop of example/euler_highlevel.jl:37-54 over every member, with SSP-RK3 in
+ * place of Midpoint: FR_ENS_FILTER runs fr1d_sensor_filter's sensor + filter (filter time step dt)
+ * on the state before every step; FR_ENS_LIMIT runs positive_limiter (src/dissipation.jl:61-123)
+ * on the input of every stage.  mode 0: one batched launch per pass; 1: one launch per member per
+ * pass (looped baseline).  n_flagged / n_limited (optional) sum the filtered / limited cells over
+ * the loop.  A cell whose mean density is not positive is left unchanged and reported as
+ * FR_E_LIMITER after the loop. */
+enum { FR_ENS_FILTER = 1, FR_ENS_LIMIT = 2 };
+int fr_ens_run(fr_ens_ctx *ctx, int mode, int opts, double dt, int nsteps, int *n_flagged, int *n_limited);
+/* the batched loop of nsteps steps captured in a hipGraph (kept while opts, dt, nsteps and the
+ * work mapping are unchanged, so later calls only replay it) and replayed nrep times */
+int fr_ens_run_graph(fr_ens_ctx *ctx, int opts, double dt, int nsteps, int nrep, int *n_flagged, int *n_limited);
 /* time candidate work mappings on the device and keep the fastest; returns its index */
 int fr_ens_autotune(fr_ens_ctx *ctx, double dt, int *chosen);
 /* fr1d_sensor_filter / fr1d_positivity over every member (each with its own degree) */

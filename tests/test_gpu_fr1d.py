@@ -133,17 +133,45 @@ def test_euler_midpoint_split_calls():
 
 
 def _ensemble(nprob, seed=7):
-    # Riemann data ramped over 2 cells: the plain (unlimited) scheme stays finite on every member,
-    # which the exact piecewise-constant jumps do not guarantee at p = 4
-    ncell, deg, dx, members = ics.riemann_ensemble(nprob, seed=seed, ncell_range=(64, 600), smooth=2.0)
+    # the configured piecewise-constant Riemann data (jump on a cell face), members of 64-600 cells
+    ncell, deg, dx, members = ics.riemann_ensemble(nprob, seed=seed, ncell_range=(64, 600))
     return ncell, deg, dx, members
+
+
+def _max_speed(members, gamma=1.4):
+    return max(float(np.max(np.abs(m[..., 1] / m[..., 0]) + np.sqrt(
+        gamma * (gamma - 1.0) * (m[..., 2] - 0.5 * m[..., 1] ** 2 / m[..., 0]) / m[..., 0]))) for m in members)
+
+
+def _oracle_loop(member, deg, dx, dt, nsteps, filt, lim):
+    """example/euler_highlevel.jl:37-54 on one member with the oracle: sensor + filter before every
+    SSP-RK3 step, positivity limiter on the input of every stage."""
+    ops = oracle_ops(int(deg))
+    J = np.full(member.shape[0], dx / 2)
+    L = lambda v: O.fr1d_euler_rhs(v, J, ops, 1.4, "rusanov", "dirichlet")  # noqa: E731
+    lim_ = lambda v: O.fr1d_positivity(v, 1.4, ops.w / 2.0, ops.ll, ops.lr)  # noqa: E731
+    uc = member.copy(order="F")
+    nf = 0
+    for _ in range(nsteps):
+        if filt:
+            nf += O.fr1d_sensor_filter(uc, ops, dt)
+        if lim:
+            assert lim_(uc) == 0
+        a = np.asfortranarray((0.0 + 1.0 * uc) + dt * L(uc))
+        if lim:
+            assert lim_(a) == 0
+        c = np.asfortranarray((0.75 * uc + 0.25 * a) + (0.25 * dt) * L(a))
+        if lim:
+            assert lim_(c) == 0
+        uc = np.asfortranarray(((1 / 3) * uc + (2 / 3) * c) + ((2 / 3) * dt) * L(c))
+    return uc, nf
 
 
 def test_ensemble_batched_equals_looped_and_single():
     ncell, deg, dx, members = _ensemble(24)
     e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
     flat = e.pack(members)
-    dt = 0.05 * float(np.min(dx)) / 2.5
+    dt = 0.05 * float(np.min(dx)) / _max_speed(members)
     e.set_state(flat)
     e.step(dt, 20)
     batched = e.get_state()
@@ -169,7 +197,7 @@ def test_ensemble_oracle_parity():
     ncell, deg, dx, members = _ensemble(6, seed=11)
     e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
     e.set_state(e.pack(members))
-    dt = 0.05 * float(np.min(dx)) / 2.5
+    dt = 0.05 * float(np.min(dx)) / _max_speed(members)
     e.step(dt, 10)
     out = e.get_state()
     for b in range(len(members)):
@@ -188,7 +216,7 @@ def test_ensemble_graph_and_autotune_keep_results():
     ncell, deg, dx, members = _ensemble(32, seed=3)
     e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
     flat = e.pack(members)
-    dt = 0.05 * float(np.min(dx)) / 2.5
+    dt = 0.05 * float(np.min(dx)) / _max_speed(members)
     e.set_state(flat)
     e.step(dt, 6)
     ref = e.get_state()
@@ -202,6 +230,73 @@ def test_ensemble_graph_and_autotune_keep_results():
     assert np.array_equal(e.get_state(), flat)  # autotune restores the state
     e.step(dt, 6)
     assert np.array_equal(e.get_state(), ref)
+
+
+@pytest.mark.parametrize("opts", ["filter", "filter+limit"])
+def test_ensemble_loop_oracle_parity(opts):
+    """The reference's per-step 1D loop over an ensemble of the configured piecewise-constant
+    Riemann problems (mixed degree 2-4, 64-600 cells), one batched launch per pass, against the
+    oracle's loop on every member at the benchmark's CFL: rel L2 <= 1e-10 after 20 steps, same
+    filtered-cell count.  (Two p = 4 members of this draw lose positivity at steps 25-26 in the
+    oracle's own loop, limiter or not: the filter's strength is proportional to dt.)"""
+    ncell, deg, dx, members = _ensemble(20, seed=21)
+    lim = opts == "filter+limit"
+    e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
+    e.set_state(e.pack(members))
+    dt = 0.05 * float(np.min(dx)) / _max_speed(members)
+    nf, nl = e.run(dt, 20, filter=True, limit=lim)
+    out = e.get_state()
+    tot = 0
+    for b in range(len(members)):
+        uc, k = _oracle_loop(members[b], deg[b], dx[b], dt, 20, True, lim)
+        tot += k
+        assert np.linalg.norm(e.member(out, b) - uc) / np.linalg.norm(uc) <= 1e-10, b
+    assert nf == tot and nf > 0
+
+
+def test_ensemble_loop_graph_looped_and_split_agree():
+    """The captured loop (kept between calls) equals the eager batched loop bitwise, over replays
+    with a different dt in between (which captures again) and with the limiter; the looped
+    baseline (one launch per member) equals the batched launch bitwise."""
+    ncell, deg, dx, members = _ensemble(16, seed=5)
+    e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
+    flat = e.pack(members)
+    dt = 0.05 * float(np.min(dx)) / _max_speed(members)
+    for lim in (False, True):
+        e.set_state(flat)
+        ref_counts = e.run(dt, 12, filter=True, limit=lim)
+        ref = e.get_state()
+        e.set_state(flat)
+        counts = [0, 0]
+        for k in range(3):
+            c = e.run_graph(dt, 4, 1, filter=True, limit=lim)
+            counts = [counts[0] + c[0], counts[1] + c[1]]
+            if k == 0:  # an interleaved capture with another dt leaves the state untouched
+                save = e.get_state()
+                e.run_graph(0.5 * dt, 4, 0, filter=True, limit=lim)
+                assert np.array_equal(e.get_state(), save)
+        assert np.array_equal(e.get_state(), ref)
+        assert tuple(counts) == ref_counts
+        e.set_state(flat)
+        assert e.run(dt, 12, filter=True, limit=lim, looped=True) == ref_counts
+        assert np.array_equal(e.get_state(), ref)
+
+
+def test_ensemble_configured_512_stay_finite():
+    """The configured ensemble (512 members, 64-2048 cells, p in {2, 3, 4}, seed 2024) through
+    10 steps of the filtered loop at the benchmark's dt: every member finite with positive density
+    and pressure."""
+    ncell, deg, dx, members = ics.riemann_ensemble(512, seed=2024)
+    e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
+    e.set_state(e.pack(members))
+    dt = 0.05 * float(np.min(dx)) / _max_speed(members)
+    e.run_graph(dt, 10, 1, filter=True)
+    out = e.get_state()
+    for b in range(512):
+        m = e.member(out, b)
+        assert np.all(np.isfinite(m)), b
+        p = 0.4 * (m[..., 2] - 0.5 * m[..., 1] ** 2 / m[..., 0])
+        assert m[..., 0].min() > 0.0 and p.min() > 0.0, b
 
 
 def sod_state(ncell, deg, gamma=1.4):
@@ -255,12 +350,25 @@ def test_positivity_1d_parity(deg):
 
 def test_ensemble_dissipation_matches_single():
     """fr_ens_sensor_filter / fr_ens_positivity over mixed-degree members equal the single-problem
-    calls on each member bitwise."""
+    calls on each member bitwise (on states 15 filtered steps into the Riemann problems, where the
+    waves have left the jump cells with flagged modes, and with a negative density and a negative
+    pressure planted in two cells of every member for the limiter)."""
     ncell, deg, dx, members = ics.riemann_ensemble(12, seed=4, ncell_range=(64, 400))
     e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
-    flat = e.pack(members)
-    dt = 0.05 * float(np.min(dx)) / 2.5
+    dt = 0.05 * float(np.min(dx)) / _max_speed(members)
+    e.set_state(e.pack(members))
+    e.run(dt, 15, filter=True)
+    flat = e.get_state()
+    assert np.all(np.isfinite(flat))
+    members = [e.member(flat, b).copy(order="F") for b in range(len(members))]
+    bad = [m.copy(order="F") for m in members]
+    for m in bad:
+        m[5, 1, 0] = -0.05
+        m[17, 0, 2] *= 0.1
     for what in ("filter", "positivity"):
+        if what == "positivity":
+            flat = e.pack(bad)
+            members = bad
         e.set_state(flat)
         n = e.sensor_filter(dt) if what == "filter" else e.positivity()
         out = e.get_state()
@@ -272,7 +380,7 @@ def test_ensemble_dissipation_matches_single():
             tot += g.sensor_filter(dt) if what == "filter" else g.positivity()
             assert np.array_equal(g.get_state(), e.member(out, b)), (what, b)
             g.close()
-        assert n == tot
+        assert n == tot and n > 0
 
 
 def test_euler_residual_device_entry_point():
