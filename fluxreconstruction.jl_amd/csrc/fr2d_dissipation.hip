@@ -90,12 +90,16 @@ __global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int 
 
 
 // Source cell of ghost cell (gi, gj) under a ghost refresh rule (as fr2d_fill_ghost_kernel);
-// keep = the rule leaves the cell as it is (the inflow column of FR_GHOST_INFLOW_LEFT).
-__device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int rule, int &si, int &sj, bool &keep) {
+// keep = the rule leaves the cell as it is (the inflow column of FR_GHOST_INFLOW_LEFT); negy =
+// the y-momentum is copied negated (the y ghost rows of FR_GHOST_WAVE).
+__device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int rule, int &si, int &sj, bool &keep,
+                                             bool &negy) {
   keep = false;
-  if (rule == FR_GHOST_PERIODIC) {
+  negy = false;
+  if (rule == FR_GHOST_PERIODIC || rule == FR_GHOST_WAVE) {
     si = (gi == 0) ? nx : (gi == nx + 1 ? 1 : gi);
     sj = (gj == 0) ? ny : (gj == ny + 1 ? 1 : gj);
+    negy = rule == FR_GHOST_WAVE && (gj == 0 || gj == ny + 1);
   } else if (rule == FR_GHOST_OUTFLOW) {
     si = (gi == 0) ? 1 : (gi == nx + 1 ? nx : gi);
     sj = (gj == 0) ? 1 : (gj == ny + 1 ? ny : gj);
@@ -167,8 +171,8 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
       // a kept ghost stays: it may be the source of other ghosts (the corners of
       // FR_GHOST_INFLOW_LEFT copy the inflow column)
       int si, sj;
-      bool keep;
-      ghost_source(i, j, nx, ny, rule, si, sj, keep);
+      bool keep, negy;
+      ghost_source(i, j, nx, ny, rule, si, sj, keep, negy);
       if (!keep) return;  // refreshed by the threads of its source cell
     }
   }
@@ -243,12 +247,13 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
         if (dupj || (gi == i && gj == j)) continue;
         if (!(gi == 0 || gi == nx + 1 || gj == 0 || gj == ny + 1)) continue;
         int si, sj;
-        bool keep;
-        ghost_source(gi, gj, nx, ny, rule, si, sj, keep);
+        bool keep, negy;
+        ghost_source(gi, gj, nx, ny, rule, si, sj, keep, negy);
         if (keep || si != i || sj != j) continue;
         double *const gr = u + (long long)gi + (long long)pitch * gj + ldp * (NP * r);
+        const double sg = (negy && r == 2) ? -1.0 : 1.0;
 #pragma unroll
-        for (int q = 0; q < NP; ++q) gr[ldp * q] = x[q];
+        for (int q = 0; q < NP; ++q) gr[ldp * q] = sg * x[q];
       }
     }
   }

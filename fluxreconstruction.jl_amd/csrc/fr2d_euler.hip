@@ -561,7 +561,8 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
 }
 
 // ------------------------------------------------------------------ ghost-layer helpers
-// rule FR_GHOST_PERIODIC / _OUTFLOW / _INFLOW_LEFT over every ghost cell and every (k,l,m) plane.
+// rule FR_GHOST_PERIODIC / _OUTFLOW / _INFLOW_LEFT / _WAVE over every ghost cell and every
+// (k,l,m) plane (planes = 4 nsp^2, variable m = plane / nsp^2).
 __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, int pitch, long long ldp, int planes, int rule) {
   const int nring = 2 * (nx + 2) + 2 * ny;
   const long long total = (long long)nring * planes;
@@ -585,9 +586,14 @@ __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, int pitch, lon
       j = g - 2 * (nx + 2) - ny + 1;
     }
     int si = i, sj = j;
-    if (rule == FR_GHOST_PERIODIC) {
+    bool neg = false;
+    if (rule == FR_GHOST_PERIODIC || rule == FR_GHOST_WAVE) {
+      // euler2d_wave.jl:127-134 fills the x ghosts first and then the whole y ghost rows from the
+      // filled columns, so a corner takes the diagonally opposite interior cell; the wave rule
+      // negates the y-momentum of the y ghost rows
       si = (i == 0) ? nx : (i == nx + 1 ? 1 : i);
       sj = (j == 0) ? ny : (j == ny + 1 ? 1 : j);
+      neg = rule == FR_GHOST_WAVE && (j == 0 || j == ny + 1) && pl / (planes / 4) == 2;
     } else if (rule == FR_GHOST_OUTFLOW) {
       si = (i == 0) ? 1 : (i == nx + 1 ? nx : i);
       sj = (j == 0) ? 1 : (j == ny + 1 ? ny : j);
@@ -598,7 +604,8 @@ __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, int pitch, lon
       si = (i == nx + 1) ? nx : i;
       sj = (j == 0) ? 1 : (j == ny + 1 ? ny : j);
     }
-    u[(long long)i + nxg * j + ldp * pl] = u[(long long)si + nxg * sj + ldp * pl];
+    const double v = u[(long long)si + nxg * sj + ldp * pl];
+    u[(long long)i + nxg * j + ldp * pl] = neg ? -v : v;
   }
 }
 

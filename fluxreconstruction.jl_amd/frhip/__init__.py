@@ -30,6 +30,7 @@ from .space import (  # noqa: F401
     L2_error,
     Linf_error,
 )
+from .ops import interp_face, poly_derivative, filter_exp  # noqa: F401
 from .euler2d import FREuler2D  # noqa: F401
 from .fr1d import FR1D, FREulerProblem, FRAdvectionProblem, Ensemble1D  # noqa: F401
 from . import ics  # noqa: F401

@@ -7,6 +7,7 @@ the library context and ``FREuler2D.dudt(du, u, p, t)`` keeps the same argument 
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 
 import numpy as np
@@ -97,6 +98,24 @@ class FREuler2D:
     def stream_handle(self) -> int:
         return _lib.lib().fr2d_get_stream(self.h) or 0
 
+    @contextlib.contextmanager
+    def _torch_ordered(self):
+        """Orders a call on torch tensors with torch's current stream: the context's stream waits
+        for the work queued so far on torch's current stream (which wrote the inputs), and torch's
+        current stream waits for the call's work (which writes the outputs).  Nothing is added when
+        the context already runs on torch's current stream."""
+        import torch
+
+        cur = torch.cuda.current_stream()
+        mine = self.stream_handle()
+        if mine == cur.cuda_stream:
+            yield
+            return
+        ctx = torch.cuda.default_stream() if mine == 0 else torch.cuda.ExternalStream(mine)
+        ctx.wait_stream(cur)
+        yield
+        cur.wait_stream(ctx)
+
     # ---------------------------------------------------------------- right-hand side
     def _check_host(self, a):
         if a.shape != self.shape or a.dtype != np.float64 or not a.flags["F_CONTIGUOUS"]:
@@ -115,7 +134,9 @@ class FREuler2D:
         if _is_torch(u):
             self._check_dev(u)
             self._check_dev(du)
-            check(_lib.lib().fr2d_euler_rhs_dev(self.h, C.c_void_p(u.data_ptr()), C.c_void_p(du.data_ptr()), float(t)))
+            with self._torch_ordered():
+                check(_lib.lib().fr2d_euler_rhs_dev(self.h, C.c_void_p(u.data_ptr()), C.c_void_p(du.data_ptr()),
+                                                    float(t)))
         else:
             self._check_host(u)
             self._check_host(du)
@@ -133,7 +154,8 @@ class FREuler2D:
     def set_state(self, u):
         if _is_torch(u):
             self._check_dev(u)
-            check(_lib.lib().fr2d_set_state_dev(self.h, C.c_void_p(u.data_ptr())))
+            with self._torch_ordered():
+                check(_lib.lib().fr2d_set_state_dev(self.h, C.c_void_p(u.data_ptr())))
         else:
             u = np.asfortranarray(u, dtype=np.float64)
             self._check_host(u)
@@ -221,7 +243,8 @@ class FREuler2D:
                 raise ValueError("primitives: out must be a GPU tensor of nx*nsp*ny*nsp*4 doubles")
             if str(out.dtype) != "torch.float64" or not out.is_cuda or not out.is_contiguous():
                 raise ValueError("primitives: out must be a contiguous float64 tensor on the GPU")
-            check(_lib.lib().fr2d_primitives_dev(self.h, C.c_void_p(out.data_ptr())))
+            with self._torch_ordered():
+                check(_lib.lib().fr2d_primitives_dev(self.h, C.c_void_p(out.data_ptr())))
             return x, y, out
         sol = np.zeros((nx * n, ny * n, 4), order="F")
         check(_lib.lib().fr2d_primitives(self.h, dptr(sol)))

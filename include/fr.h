@@ -45,8 +45,10 @@ enum { FR_RK_SSPRK3 = 0, FR_RK_RK4 = 1, FR_RK_MIDPOINT = 2, FR_RK_EULER = 3 };
 enum {
   FR_GHOST_PERIODIC = 0, /* copy the opposite interior row/column (euler2d_wave.jl:127-131, no sign flips) */
   FR_GHOST_OUTFLOW = 1,  /* copy the adjacent interior row/column on every side */
-  FR_GHOST_INFLOW_LEFT = 2 /* shock-vortex.jl:322-325: bottom, top and right copy the adjacent row/column,
-                              the left (supersonic inflow) column keeps its values */
+  FR_GHOST_INFLOW_LEFT = 2, /* shock-vortex.jl:322-325: bottom, top and right copy the adjacent row/column,
+                               the left (supersonic inflow) column keeps its values */
+  FR_GHOST_WAVE = 3 /* euler2d_wave.jl:127-134: FR_GHOST_PERIODIC with the y-momentum (variable 3) of the
+                       bottom and top ghost rows, corners included, negated */
 };
 /* sensor modes for fr2d_sensor_filter */
 enum { FR_SENSOR_OFF = 0, FR_SENSOR_MODAL = 1, FR_SENSOR_LEARNED = 2 };
@@ -82,6 +84,26 @@ int fr_inv(int n, const double *A, double *Ainv);
 int fr_rs_jacobi_quad(double r, double s, const double *vert, double *J);
 /* Persson-Peraire switch; returns 1 (shock) / 0 (src/dissipation.jl:13-23) */
 int fr_shock_detector(double Se, int deg, double S0, double kappa);
+/* filter_exp(N, s, V, Nc, invV) (src/Polynomial/poly_filter.jl:11-24): the nodal exponential
+ * filter F = V diag(sigma) invV (nv x nv, column-major) with sigma of KitBase filter_exp1d for a
+ * line basis (nv = N+1) or filter_exp2d (poly_filter.jl:32-47) for a triangle basis
+ * (nv = (N+1)(N+2)/2); invV NULL: inverted here.  Requires 0 <= Nc < N, s >= 1.  Other nv return
+ * FR_E_UNSUPPORTED (the reference leaves the filter undefined there).  Applied to device states
+ * with fr_poly_derivative_dev (pdm = F). */
+int fr_filter_exp(int N, int s, int Nc, const double *V, int nv, const double *invV, double *F);
+
+/* ------------------------------------------------------------------ row contractions (device)
+ * f(row, q, b) is a column-major (nrow, n, nbatch) device array, 1 <= n <= 16; the operators are
+ * host arrays copied into the launch.  Asynchronous on `stream` (a hipStream_t, NULL for HIP's null
+ * stream); input and output must not overlap. */
+/* interp_face!(fδ, f, ll, lr) of src/interpolate.jl:1-10 on every row: fδ(row, 1, b) = f(row, :, b)·ll,
+ * fδ(row, 2, b) = f(row, :, b)·lr, fδ column-major (nrow, 2, nbatch) */
+int fr_interp_face_dev(const double *d_f, long long nrow, int n, int nbatch, const double *ll, const double *lr,
+                       double *d_ff, void *stream);
+/* poly_derivative!(df, f, pdm) of src/derivative.jl:1-10 on every row: df(row, i, b) =
+ * f(row, :, b)·pdm(i, :), pdm n x n column-major, df column-major (nrow, n, nbatch) */
+int fr_poly_derivative_dev(const double *d_f, long long nrow, int n, int nbatch, const double *pdm, double *d_df,
+                           void *stream);
 
 /* ------------------------------------------------------------------ 2D Euler on uniform quads */
 typedef struct fr2d_ctx fr2d_ctx;

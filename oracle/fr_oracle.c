@@ -757,6 +757,38 @@ void or_filter_exp1d(int N, int s, int Nc, double *diag) {
   for (int i = Nc; i <= N; ++i) diag[i] = exp(-alpha * pow((double)(i - Nc) / (N - Nc), s));
 }
 
+/* filter_exp2d (src/Polynomial/poly_filter.jl:32-47): modes i = 0..N outer, j = 0..N-i inner */
+void or_filter_exp2d(int N, int s, int Nc, double *diag) {
+  double alpha = -log(2.220446049250313e-16);
+  int sk = 0;
+  for (int i = 0; i <= N; ++i)
+    for (int j = 0; j <= N - i; ++j) {
+      diag[sk] = 1.0;
+      if (i + j >= Nc) diag[sk] = exp(-alpha * pow((double)(i + j - Nc) / (N - Nc), s));
+      sk += 1;
+    }
+}
+
+/* filter_exp (src/Polynomial/poly_filter.jl:11-24): F = V * diagm(filterdiag) * invV; returns -1
+ * when nv is neither N+1 nor (N+1)(N+2)/2 (filterdiag undefined in the reference) */
+int or_filter_exp(int N, int s, int Nc, const double *V, int nv, const double *invV, double *F) {
+  double diag[1024];
+  if (nv > 1024) return -1;
+  if (nv == N + 1)
+    or_filter_exp1d(N, s, Nc, diag);
+  else if (nv == (N + 1) * (N + 2) / 2)
+    or_filter_exp2d(N, s, Nc, diag);
+  else
+    return -1;
+  for (int j = 0; j < nv; ++j)
+    for (int i = 0; i < nv; ++i) {
+      double a = 0.0;
+      for (int k = 0; k < nv; ++k) a += V[i + nv * k] * diag[k] * invV[k + nv * j];
+      F[i + nv * j] = a;
+    }
+  return 0;
+}
+
 /* Modal sensor + filter, example/shock-vortex.jl:308-321 generalised from the hard-coded 3x3
  * (deg = 2) slices to nsp x nsp: u_hat = iV * rho(:), su = u_hat[end]^2 / sum(u_hat^2),
  * shock_detector(log10(su), deg, S0, kappa); on shock every variable is filtered in modal

@@ -90,6 +90,8 @@ void or_fr2d_midpoint(const or_fr2d *d, double *u, double dt, int nsteps);
 int or_shock_detector(double Se, int deg, double S0, double kappa);
 void or_modal_filter_l2(double *u, int n, double lambda);
 void or_filter_exp1d(int N, int s, int Nc, double *diag);
+void or_filter_exp2d(int N, int s, int Nc, double *diag);
+int or_filter_exp(int N, int s, int Nc, const double *V, int nv, const double *invV, double *F);
 /* per-element modal sensor + L2 filter over every cell (ghosts included, as the reference loop);
    returns the number of filtered cells, writes the per-cell flag if flags != NULL */
 int or_fr2d_sensor_filter(const or_fr2d *d, const double *V, const double *iV, double *u,

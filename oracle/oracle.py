@@ -376,6 +376,28 @@ def shock_detector(Se, deg, S0=None, kappa=4.0) -> bool:
     return bool(lib().or_shock_detector(float(Se), int(deg), float(S0), float(kappa)))
 
 
+def filter_exp1d(N, s, Nc=0):
+    d = np.zeros(N + 1)
+    lib().or_filter_exp1d(C.c_int(N), C.c_int(s), C.c_int(Nc), _p(d))
+    return d
+
+
+def filter_exp2d(N, s, Nc=0):
+    d = np.zeros((N + 1) * (N + 2) // 2)
+    lib().or_filter_exp2d(C.c_int(N), C.c_int(s), C.c_int(Nc), _p(d))
+    return d
+
+
+def filter_exp(N, s, V, Nc=0, invV=None):
+    """src/Polynomial/poly_filter.jl:11-24; None where the reference leaves the filter undefined."""
+    V = np.asfortranarray(V, dtype=np.float64)
+    nv = V.shape[0]
+    iv = np.asfortranarray(inv(V) if invV is None else invV, dtype=np.float64)
+    F = np.zeros((nv, nv), order="F")
+    rc = lib().or_filter_exp(C.c_int(N), C.c_int(s), C.c_int(Nc), _p(V), C.c_int(nv), _p(iv), _p(F))
+    return None if rc else F
+
+
 def modal_filter_l2(u, lam):
     a = _f64(u).copy()
     lib().or_modal_filter_l2(_p(a), C.c_int(a.size), C.c_double(lam))

@@ -118,6 +118,44 @@ def test_residual_device_entry_point():
     assert rel_inf(du, orc.rhs(u)) <= RES_TOL
 
 
+def test_torch_tensors_ordered_without_stream_selection():
+    """dudt, set_state and primitives on torch tensors, with the context on its own stream (no
+    use_torch_stream): each call waits for what torch's current stream queued before it (a NaN
+    fill of du, a copy into u) and torch's stream waits for the call (a copy out of du queued just
+    after it), on the default stream and on a side stream."""
+    import torch
+
+    ps, u, s, orc = setup(128, 96, 3, "rusanov", "periodic")
+    ref = orc.rhs(u)
+    host = torch.from_numpy(u.ravel(order="F").copy())
+    side = torch.cuda.Stream()
+    for st in (torch.cuda.current_stream(), side):
+        with torch.cuda.stream(st):
+            ud = torch.empty(host.numel(), dtype=torch.float64, device="cuda")
+            ud.copy_(host, non_blocking=False)
+            ud.mul_(1.0)  # queued work that writes u right before the call
+            dud = torch.empty_like(ud)
+            dud.fill_(float("nan"))
+            s.dudt(dud, ud)
+            out = dud.clone()  # queued right after the call on torch's stream
+        torch.cuda.synchronize()
+        du = out.cpu().numpy().reshape(s.shape, order="F")
+        assert np.all(np.isfinite(du))
+        assert rel_inf(du, ref) <= RES_TOL
+        with torch.cuda.stream(st):
+            ud2 = torch.zeros_like(ud)
+            ud2.add_(ud)  # u written on torch's stream just before set_state
+            s.set_state(ud2)
+        assert np.array_equal(s.get_state(), u)
+        n = ps.nx * ps.ny * 16 * 4
+        with torch.cuda.stream(st):
+            prim = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+            s.primitives(out=prim)
+            pc = prim.clone()
+        torch.cuda.synchronize()
+        assert np.all(np.isfinite(pc.cpu().numpy()))
+
+
 def test_c2_full_size_residual():
     """256 x 256, p = 3, Rusanov, periodic: the benchmark configuration."""
     ps, u, s, orc = setup(256, 256, 3, "rusanov", "periodic", amp=0.0)
@@ -312,6 +350,13 @@ def _ghost_rule_numpy(u, rule):
         v[:, 0] = v[:, 1]
         v[:, ny + 1] = v[:, ny]
         v[0, 0], v[0, ny + 1], v[nx + 1, 0], v[nx + 1, ny + 1] = v[1, 1], v[1, ny], v[nx, 1], v[nx, ny]
+    elif rule == "wave":  # example/euler2d_wave.jl:127-132, in its order (variable 3 is index 2)
+        v[0] = v[nx]
+        v[nx + 1] = v[1]
+        v[:, 0] = v[:, ny]
+        v[:, 0, :, :, 2] *= -1
+        v[:, ny + 1] = v[:, 1]
+        v[:, ny + 1, :, :, 2] *= -1
     else:  # example/shock-vortex.jl:322-325, in its order
         v[:, 0] = v[:, 1]
         v[:, ny + 1] = v[:, ny]
@@ -319,7 +364,7 @@ def _ghost_rule_numpy(u, rule):
     return v
 
 
-@pytest.mark.parametrize("rule", ["periodic", "outflow", "inflow_left"])
+@pytest.mark.parametrize("rule", ["periodic", "outflow", "inflow_left", "wave"])
 def test_fill_ghost_rules(rule):
     nx, ny, deg = 13, 9, 3
     ps = frhip.FRPSpace2D(0.0, 1.0, nx, 0.0, 1.0, ny, deg, 1, 1)
