@@ -4,12 +4,35 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 
 #include "fr2d_internal.h"
 
 namespace fr {
+
+// ------------------------------------------------------------------ scratch of the allocation-free step
+// fr2d_step_dev_noalloc takes its two stage buffers from this array: kNoAllocStateMax doubles
+// each, one copy per GPU, held from module load until the process ends.  Host code never passes
+// the array's name to a kernel (in host code it is a stand-in, not a device address): the device
+// address comes from hipGetSymbolAddress, called once per device at context creation with that
+// device current, and is kept in noalloc_ptr.
+constexpr size_t kNoAllocStateMax = 6291456;  // 48 MiB of doubles per stage buffer
+static __device__ __align__(256) double scratch[2 * kNoAllocStateMax];
+constexpr int kNoAllocDevices = 64;
+static std::mutex noalloc_mu;
+static double *noalloc_ptr[kNoAllocDevices] = {};
+
+// resolves the scratch address of the current device once (fr2d_create)
+static void noalloc_setup(int dev) {
+  if (dev < 0 || dev >= kNoAllocDevices) return;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (noalloc_ptr[dev]) return;
+  void *p = nullptr;
+  if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) noalloc_ptr[dev] = static_cast<double *>(p);
+  else (void)hipGetLastError();  // reported by fr2d_step_dev_noalloc, which cannot run without it
+}
 
 
 // layout of Fr2dCtx::d_dops: V (np^2) | iV (np^2) | wp/4 (np) | ll (nsp) | lr (nsp)
@@ -71,7 +94,7 @@ static int copy_from_padded(const Fr2dCtx &c, double *dst, const double *src, hi
 // of every step copies the state's ghost layer into both stage buffers (from its halo loads), and
 // no stage writes a ghost cell of its result, so FR_BC_GHOST keeps the ghost layer fixed during a
 // step as the reference does (du = 0 there).
-static int enqueue_steps(Fr2dCtx &c, int scheme, double dt, int nsteps, hipStream_t st) {
+static int enqueue_steps(const Fr2dCtx &c, int scheme, double dt, int nsteps, hipStream_t st) {
   double *U = c.d_u, *S1 = c.d_s1, *S2 = c.d_s2;
   for (int n = 0; n < nsteps; ++n) {
     int rc = FR_OK;
@@ -133,7 +156,7 @@ static int prepare_step(Fr2dCtx &c, int scheme, hipStream_t st) {
   return ensure_stage_buffers(c, scheme == FR_RK_RK4);
 }
 
-static int finish_step(Fr2dCtx &c, hipStream_t st) {
+static int finish_step(const Fr2dCtx &c, hipStream_t st) {
   if (c.bc == FR_BC_PERIODIC) return fr2d_launch_fill_ghost(c, c.d_u, FR_GHOST_PERIODIC, st);
   return FR_OK;
 }
@@ -202,6 +225,7 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
   if (d->V) std::memcpy(c->ops.V, d->V, sizeof(double) * np * np);
   if (d->iV) std::memcpy(c->ops.iV, d->iV, sizeof(double) * np * np);
   hipError_t e = hipGetDevice(&c->device);
+  if (e == hipSuccess) noalloc_setup(c->device);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&c->num_cu, hipDeviceAttributeMultiprocessorCount, c->device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking);
   if (e == hipSuccess) e = hipMalloc(&c->d_count, 4 * sizeof(int));
@@ -375,6 +399,39 @@ int fr2d_step_graph(fr2d_ctx *c, int scheme, double dt, int nsteps, int nrep) {
   }
   for (int r = 0; r < nrep; ++r) FR_HIP(hipGraphLaunch(c->gexec, st));
   return finish_step(*c, st);
+}
+
+int fr2d_step_dev_noalloc(fr2d_ctx *c, double *d_u, int scheme, double dt, int nsteps) {
+  FR_REQUIRE(c && d_u, FR_E_INVALID, "fr2d_step_dev_noalloc: null argument");
+  FR_REQUIRE(nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_step_dev_noalloc: bad dt / nsteps");
+  FR_REQUIRE(scheme == FR_RK_SSPRK3 || scheme == FR_RK_MIDPOINT || scheme == FR_RK_EULER, FR_E_UNSUPPORTED,
+             "fr2d_step_dev_noalloc: SSP-RK3, Midpoint and Euler only (RK4 needs a third array)");
+  FR_REQUIRE(c->len <= kNoAllocStateMax, FR_E_UNSUPPORTED,
+             "fr2d_step_dev_noalloc: the state has " + std::to_string(c->len) + " doubles, above the " +
+                 std::to_string(kNoAllocStateMax) + " of each scratch stage buffer");
+  double *base = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(noalloc_mu);
+    if (c->device >= 0 && c->device < kNoAllocDevices) base = noalloc_ptr[c->device];
+  }
+  FR_REQUIRE(base, FR_E_HIP, "fr2d_step_dev_noalloc: the scratch array of this device has no address");
+  // the context's launch parameters with the caller's array in the reference layout (rows of
+  // nx+2) as the state and the scratch as the stage buffers, which need no initial values: every
+  // interior value is written before it is read, the first stage of every step copies the state's
+  // ghost layer into both buffers, and periodic grids never read a ghost layer
+  Fr2dCtx r = *c;
+  r.pitch = c->nx + 2;
+  r.ldpi = (long long)(c->nx + 2) * (c->ny + 2);
+  r.lenp = c->len;
+  r.d_u = d_u;
+  r.d_s1 = base;
+  r.d_s2 = base + kNoAllocStateMax;
+  r.d_acc = nullptr;
+  r.gexec = nullptr;
+  hipStream_t st = stream_of(*c);
+  int rc = enqueue_steps(r, scheme, dt, nsteps, st);
+  if (rc) return rc;
+  return finish_step(r, st);
 }
 
 int fr2d_fill_ghost(fr2d_ctx *c, int rule) {

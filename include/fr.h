@@ -162,6 +162,17 @@ int fr2d_step(fr2d_ctx *ctx, int scheme, double dt, int nsteps);
  * stream (asynchronous, like fr2d_step); the context keeps the instantiated graph and reuses it
  * while scheme, dt and nsteps are unchanged.  No reference counterpart (a replay of fr2d_step). */
 int fr2d_step_graph(fr2d_ctx *ctx, int scheme, double dt, int nsteps, int nrep);
+/* No counterpart in the reference: fr2d_step for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  nsteps steps
+ * of FR_RK_SSPRK3, FR_RK_MIDPOINT or FR_RK_EULER on the caller's device state d_u (reference
+ * layout, read and written in place; the same arithmetic as fr2d_step, bitwise) with the stage
+ * buffers taken from a fixed scratch array of the library: 2 x 6,291,456 doubles (96 MiB) per
+ * GPU, held until the process ends.  No allocation and no host synchronisation; asynchronous on
+ * the context's stream.  A state above 6,291,456 doubles or FR_RK_RK4 (a third array) returns
+ * FR_E_UNSUPPORTED.  Every call on one GPU shares that GPU's scratch (which keeps the last call's
+ * contents and needs no initial values): calls that overlap on one GPU race, whatever their
+ * stream, thread or context, and must be serialised by the caller. */
+int fr2d_step_dev_noalloc(fr2d_ctx *ctx, double *d_u, int scheme, double dt, int nsteps);
 /* refresh the ghost layer of the device state (FR_GHOST_*): the ghost copies of the reference
  * host loops (shock-vortex.jl:322-325, euler2d_wave.jl:125-135) */
 int fr2d_fill_ghost(fr2d_ctx *ctx, int rule);
