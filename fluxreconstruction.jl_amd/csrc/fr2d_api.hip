@@ -153,7 +153,15 @@ static int prepare_step(Fr2dCtx &c, int scheme, hipStream_t st) {
   FR_REQUIRE(c.d_u, FR_E_INVALID, "fr2d_step: no state (call fr2d_set_state first)");
   // the stage buffers take the state's ghost layer inside the first stage of every step
   (void)st;
-  return ensure_stage_buffers(c, scheme == FR_RK_RK4);
+  int rc = ensure_stage_buffers(c, scheme == FR_RK_RK4);
+  // the first stepping call of a context picks the staggered-start delay on the device, on the
+  // context's own stream (so a caller stream under capture records nothing of it); it reads the
+  // state and writes only the first stage buffer, which the steps overwrite
+  if (!rc && c.stagger < 0) {
+    if ((rc = fr2d_tune_stagger(c, c.own_stream))) return rc;
+    FR_HIP(hipStreamSynchronize(c.own_stream));
+  }
+  return rc;
 }
 
 static int finish_step(const Fr2dCtx &c, hipStream_t st) {
@@ -399,6 +407,12 @@ int fr2d_step_graph(fr2d_ctx *c, int scheme, double dt, int nsteps, int nrep) {
   }
   for (int r = 0; r < nrep; ++r) FR_HIP(hipGraphLaunch(c->gexec, st));
   return finish_step(*c, st);
+}
+
+int fr2d_get_stagger(const fr2d_ctx *c, int *stagger) {
+  FR_REQUIRE(c && stagger, FR_E_INVALID, "fr2d_get_stagger: null argument");
+  *stagger = c->stagger;
+  return FR_OK;
 }
 
 int fr2d_step_dev_noalloc(fr2d_ctx *c, double *d_u, int scheme, double dt, int nsteps) {

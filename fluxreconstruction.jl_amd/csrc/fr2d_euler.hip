@@ -44,6 +44,8 @@ enum { GEO_RECT = 0, GEO_AFFINE = 1 };
 struct Geo2D {
   int nx, ny, ntx, nty;
   int strip;      // tile-order strip width in tiles (strip_tile)
+  int stag_cu;    // staggered start: workgroups per slot (the CU count), 0 = off
+  int stag;       // staggered start: units of s_sleep(8) per slot
   int pitch;      // elements per row of a plane: nx+2 (caller arrays) or the padded pitch
   long long ldp;  // elements per (k,l,m) plane: pitch (ny+2)
   double iJ11, iJ22, g, gm1;
@@ -166,6 +168,11 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
       *reinterpret_cast<double *>(reinterpret_cast<char *>(base + ldp * (pl + NSP * NSP * m)) + bo) = v;
   };
 
+  if (EPI != EPI_RESIDUAL && geo.stag_cu) {
+    // staggered start (single-round grids): the workgroup's slot on its CU times the delay
+    const int n = (int)(blockIdx.x / (unsigned)geo.stag_cu) * geo.stag;
+    for (int q = 0; q < n; ++q) __builtin_amdgcn_s_sleep(8);
+  }
   int bx, by;
   strip_tile(xcd_tile(blockIdx.x, gridDim.x), geo.ntx, geo.nty, geo.strip, bx, by);
   const int i0 = 1 + bx * TX, j0 = 1 + by * TY;
@@ -640,6 +647,9 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   geo.q1 = c.iJ11 * c.n1[0] + c.iJ12 * c.n1[1];
   geo.q2 = c.iJ21 * c.n2[0] + c.iJ22 * c.n2[1];
   const unsigned nb = (unsigned)(geo.ntx * geo.nty);  // one tile per workgroup
+  const bool stag = !caller && c.stagger > 0 && fr2d_single_round(c);
+  geo.stag_cu = stag ? c.num_cu : 0;
+  geo.stag = stag ? c.stagger : 0;
   const bool buf = (caller ? c.len : c.lenp) * sizeof(double) < (size_t)0xFFFFFFFFu;
   if (buf && !aff)
     hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_BUFFER, GEO_RECT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
@@ -706,6 +716,68 @@ int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, 
   sa.acc_out = acc_out;
   sa.b2 = b2;
   return launch(c, sa, false, st);
+}
+
+bool fr2d_single_round(const Fr2dCtx &c) {
+  // four workgroups per CU for the nsp = 4 stage instances (Rusanov, HLL): more tiles than one
+  // workgroup per CU and no more than all four slots
+  const long long nb = (long long)((c.nx + TX - 1) / TX) * ((c.ny + TY - 1) / TY);
+  return c.nsp == 4 && c.flux != FR_FLUX_ROE && c.num_cu > 0 && nb > c.num_cu && nb <= 4LL * c.num_cu;
+}
+
+// Times the stage launches of SSP-RK3 steps (state and second stage buffer -> first stage buffer,
+// both buffers scratch here) for each stagger candidate, interleaved over four passes after a
+// warm-up, and keeps the fastest.  The state is only read; st must not be capturing (the timing
+// synchronises on an event).
+int fr2d_tune_stagger(Fr2dCtx &c, hipStream_t st) {
+  if (!fr2d_single_round(c)) {
+    c.stagger = 0;
+    return FR_OK;
+  }
+  static const int cand[] = {0, 3, 4, 5, 6, 7};  // units of s_sleep(8): 0 to 3584 clocks per slot
+  constexpr int NC = sizeof(cand) / sizeof(cand[0]), LAUNCH = 18, PASSES = 4;
+  float best[NC];
+  for (int k = 0; k < NC; ++k) best[k] = 1e30f;
+  hipEvent_t e0, e1;
+  FR_HIP(hipEventCreate(&e0));
+  FR_HIP(hipEventCreate(&e1));
+  const int keep = c.stagger;
+  int rc = FR_OK;
+  // the launch mix of an SSP-RK3 step: one stage reading the state (16 B/DOF), two reading the
+  // state and a second array (24 B/DOF); results go to the first stage buffer
+  auto run = [&](int n) {
+    for (int q = 0; q < n && rc == FR_OK; ++q)
+      rc = (q % 3 == 0) ? fr2d_launch_stage(c, c.d_u, c.d_u, c.d_s1, 0.0, 1.0, 1e-9, nullptr, nullptr, 0.0, st)
+                        : fr2d_launch_stage(c, c.d_s2, c.d_u, c.d_s1, 0.75, 0.25, 1e-9, nullptr, nullptr, 0.0, st);
+  };
+  c.stagger = 0;
+  run(1500);  // the clocks settle under sustained load (about 27 ms at 256^2)
+  for (int pass = 0; pass < PASSES && rc == FR_OK; ++pass)
+    for (int k = 0; k < NC && rc == FR_OK; ++k) {
+      c.stagger = cand[k];
+      hipError_t e = hipEventRecord(e0, st);
+      run(LAUNCH);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = hip_fail(e, "fr2d_tune_stagger", __FILE__, __LINE__);
+        break;
+      }
+      best[k] = std::min(best[k], ms);
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    c.stagger = keep;
+    return rc;
+  }
+  int bi = 0;
+  for (int k = 1; k < NC; ++k)
+    if (best[k] < best[bi]) bi = k;
+  c.stagger = cand[bi];
+  return FR_OK;
 }
 
 int fr2d_launch_fill_ghost(const Fr2dCtx &c, double *d_u, int rule, hipStream_t st) {

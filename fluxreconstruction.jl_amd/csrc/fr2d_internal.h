@@ -26,6 +26,12 @@ struct Fr2dCtx {
   static constexpr int kPadFront = 15;
   int device = 0;
   int num_cu = 0;  // compute units of the device
+  // Staggered start of the RK stage launches whose tiles are all resident at once (at most four
+  // workgroups per CU): workgroup slot s of a CU (s = block / num_cu) first sleeps s * stagger
+  // units of 64 x 8 clocks, so the load phases of the workgroups sharing a CU do not coincide.
+  // -1: not tuned yet (fr2d_tune_stagger picks it on the device), 0: off.  Only the timing
+  // changes, never the arithmetic.
+  int stagger = -1;
   // the caller's stream (caller_stream: any hipStream_t, NULL being HIP's null stream) or the
   // context's own non-blocking stream
   bool caller_stream = false;
@@ -62,6 +68,10 @@ int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, 
                       double a1, double b, const double *acc_in, double *acc_out, double b2, hipStream_t st,
                       double *ghost_copy0 = nullptr, double *ghost_copy1 = nullptr);
 int fr2d_launch_fill_ghost(const Fr2dCtx &c, double *d_u, int rule, hipStream_t st);
+// whether the RK stage launches of c have every tile resident at once (the staggered start
+// applies), and the timing of the stagger candidates on the device (keeps the fastest in c.stagger)
+bool fr2d_single_round(const Fr2dCtx &c);
+int fr2d_tune_stagger(Fr2dCtx &c, hipStream_t st);
 // eval_out != nullptr: write the per-cell strength multiplier (modal: 0/1, learned: MLP output)
 // instead of filtering
 int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double lambda, double S0, double kappa,

@@ -177,6 +177,13 @@ class FREuler2D:
         check(_lib.lib().fr2d_state_pitch(self.h, C.byref(p), C.byref(q)))
         return p.value, q.value
 
+    def stagger(self) -> int:
+        """Staggered-start delay of the RK stage launches picked on the device by the context's
+        first stepping call (units of 512 clocks per workgroup slot; 0 off, -1 not tuned yet)."""
+        v = C.c_int(0)
+        check(_lib.lib().fr2d_get_stagger(self.h, C.byref(v)))
+        return v.value
+
     def step(self, dt: float, nsteps: int = 1, scheme: str = "ssprk3"):
         check(_lib.lib().fr2d_step(self.h, _lib.RK[scheme], float(dt), int(nsteps)))
 

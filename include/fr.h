@@ -162,6 +162,12 @@ int fr2d_step(fr2d_ctx *ctx, int scheme, double dt, int nsteps);
  * stream (asynchronous, like fr2d_step); the context keeps the instantiated graph and reuses it
  * while scheme, dt and nsteps are unchanged.  No reference counterpart (a replay of fr2d_step). */
 int fr2d_step_graph(fr2d_ctx *ctx, int scheme, double dt, int nsteps, int nrep);
+/* No reference counterpart (launch tuning of this library): the staggered-start delay the
+ * context's first stepping call picked on the device for its RK stage launches, in units of 512
+ * clocks per workgroup slot of a CU (0: off, -1: not tuned yet).  Grids whose tiles are all
+ * resident at once start the workgroups sharing a CU one after another, so that their load phases
+ * do not coincide; the delay changes the timing only, never the results. */
+int fr2d_get_stagger(const fr2d_ctx *ctx, int *stagger);
 /* No counterpart in the reference: fr2d_step for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  nsteps steps
  * of FR_RK_SSPRK3, FR_RK_MIDPOINT or FR_RK_EULER on the caller's device state d_u (reference

@@ -169,6 +169,25 @@ def test_c2_full_size_residual():
     assert np.all(np.abs(tot) <= 1e-12 * scale)
 
 
+def test_staggered_start_tuned_and_results_unchanged():
+    """The first stepping call tunes the staggered start of single-round grids (256 x 256, p = 3:
+    1024 tiles, four per CU) to one of its candidates and leaves it off elsewhere; the steps still
+    match the oracle (the delay changes the timing only)."""
+    ps, u, s, orc = setup(256, 256, 3, "rusanov", "periodic", amp=0.0)
+    assert s.stagger() == -1
+    dt, _ = ics.cfl_steps(u, 1.4, ps.dx0, 0.05, 1.0)
+    s.set_state(u)
+    s.step(dt, 3)
+    assert s.stagger() in (0, 3, 4, 5, 6, 7)
+    uc = u.copy(order="F")
+    orc.ssprk3(uc, dt, 3)
+    assert rel_l2(s.get_state(), uc) <= STATE_TOL
+    ps2, u2, s2, _ = setup(24, 20, 3, "rusanov", "periodic", amp=0.0)
+    s2.set_state(u2)
+    s2.step(dt, 1)
+    assert s2.stagger() == 0
+
+
 @pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler"])
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_stepping_parity(scheme, bc):
