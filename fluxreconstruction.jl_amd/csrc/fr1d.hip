@@ -17,9 +17,10 @@
 
 namespace fr {
 
-constexpr int NSP_MAX = 5;
-// device operator table: entry nsp (2..5) at OPS_STRIDE*(nsp-2):
-// ll[5] lr[5] dhl[5] dhr[5] dl[25] | w/2[5] (Gauss weights summing to one) V1[25] iV1[25]
+constexpr int NSP_MAX = 9;  // degrees 1..8 (example/euler_highlevel.jl runs degree 7)
+// device operator table: entry nsp (2..NSP_MAX) at OPS_STRIDE*(nsp-2):
+// ll lr dhl dhr [NSP_MAX each] dl [NSP_MAX^2] | w/2 [NSP_MAX] (Gauss weights summing to one)
+// V1 iV1 [NSP_MAX^2 each]
 constexpr int OFF_W = 4 * NSP_MAX + NSP_MAX * NSP_MAX;
 constexpr int OFF_V = OFF_W + NSP_MAX, OFF_IV = OFF_V + NSP_MAX * NSP_MAX;
 constexpr int OPS_STRIDE = OFF_IV + NSP_MAX * NSP_MAX;
@@ -163,7 +164,10 @@ __device__ __forceinline__ void chunk1d(const Prob1D &pb, int first, const doubl
   __syncthreads();  // the shared buffers are reused by the next chunk of a persistent loop
 }
 
-template <int EQ, int FLUX>
+// NMAX: the largest nsp the instance handles.  Problem sets of degree <= 4 use the NMAX = 5
+// instance, whose register budget is that of those degrees (119 VGPRs for Euler); sets with
+// higher degrees (up to 8) the NMAX = NSP_MAX one (179 VGPRs, fewer waves per SIMD).
+template <int EQ, int FLUX, int NMAX>
 __global__ __launch_bounds__(256) void fr1d_kernel(const Prob1D *__restrict__ probs, const Work1D *__restrict__ work,
                                                    int nwork, const double *__restrict__ ops, Phys1D ph, Stage1D sa) {
   constexpr int NV = (EQ == FR_EQ_EULER) ? 3 : 1;
@@ -176,7 +180,17 @@ __global__ __launch_bounds__(256) void fr1d_kernel(const Prob1D *__restrict__ pr
       case 2: chunk1d<2, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
       case 3: chunk1d<3, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
       case 4: chunk1d<4, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-      default: chunk1d<5, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+      case 5: chunk1d<5, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+      default:
+        if constexpr (NMAX > 5) {
+          switch (pb.nsp) {
+            case 6: chunk1d<6, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+            case 7: chunk1d<7, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+            case 8: chunk1d<8, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+            default: chunk1d<9, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
+          }
+        }
+        break;
     }
   }
 }
@@ -306,6 +320,7 @@ __device__ __forceinline__ void diss1d_cell(double *__restrict__ u, long long n,
     }
 }
 
+template <int NMAX>
 __global__ __launch_bounds__(256) void fr1d_diss_kernel(const Prob1D *__restrict__ probs, const Work1D *__restrict__ work,
                                                         int nwork, const double *__restrict__ ops, double *u, int mode,
                                                         double g, double dt, int *count) {
@@ -323,7 +338,17 @@ __global__ __launch_bounds__(256) void fr1d_diss_kernel(const Prob1D *__restrict
         case 2: diss1d_cell<2>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
         case 3: diss1d_cell<3>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
         case 4: diss1d_cell<4>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
-        default: diss1d_cell<5>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+        case 5: diss1d_cell<5>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+        default:
+          if constexpr (NMAX > 5) {
+            switch (pb.nsp) {
+              case 6: diss1d_cell<6>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+              case 7: diss1d_cell<7>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+              case 8: diss1d_cell<8>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+              default: diss1d_cell<9>(um, pb.ncell, e, o, mode, g, dt, flag, bad); break;
+            }
+          }
+          break;
       }
     }
     const unsigned long long bf = __ballot(flag), bb = __ballot(bad);
@@ -339,6 +364,7 @@ struct Batch1D {
   int eq = FR_EQ_EULER, flux = FR_FLUX_HLL, bc = FR_BC1_DIRICHLET, correction = FR_CORR_RADAU;
   double gamma = 1.4, a = 1.0;
   std::vector<Prob1D> probs;
+  int max_nsp = 2;  // largest nsp of the problem set (selects the kernel instance)
   size_t len = 0;
   hipStream_t stream = nullptr;      // the stream every call runs on
   hipStream_t own_stream = nullptr;  // the context's own non-blocking stream
@@ -362,11 +388,12 @@ struct Batch1D {
     eq = eq_;
     len = 0;
     for (auto &p : probs) {
+      max_nsp = std::max(max_nsp, p.nsp);
       p.off = (long long)len;
       len += (size_t)p.ncell * p.nsp * (eq == FR_EQ_EULER ? 3 : 1);
     }
-    std::vector<double> tab(OPS_STRIDE * 4, 0.0);
-    for (int nsp = 2; nsp <= 5; ++nsp) {
+    std::vector<double> tab(OPS_STRIDE * (NSP_MAX - 1), 0.0);
+    for (int nsp = 2; nsp <= NSP_MAX; ++nsp) {
       HostOps o;
       if (!build_ops(nsp - 1, correction, o)) return fail(FR_E_INVALID, "fr1d: operator construction failed");
       double *t = tab.data() + OPS_STRIDE * (nsp - 2);
@@ -424,6 +451,22 @@ struct Batch1D {
 
   static int tindex(int T) { return T == 64 ? 0 : (T == 128 ? 1 : 2); }
 
+  template <int NMAX>
+  void launch_eq(const Stage1D &sa, int T, int g, const Work1D *wl, int nw, const Phys1D &ph, hipStream_t st) const {
+    if (eq == FR_EQ_ADVECTION)
+      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_ADVECTION, FR_FLUX_HLL, NMAX>), dim3(g), dim3(T), 0, st, d_probs, wl, nw,
+                         d_ops, ph, sa);
+    else if (flux == FR_FLUX_HLL)
+      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_HLL, NMAX>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops,
+                         ph, sa);
+    else if (flux == FR_FLUX_RUSANOV)
+      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_RUSANOV, NMAX>), dim3(g), dim3(T), 0, st, d_probs, wl, nw,
+                         d_ops, ph, sa);
+    else
+      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_ROE, NMAX>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops,
+                         ph, sa);
+  }
+
   // one launch over work items [w0, w0 + nw) of the list for T threads
   int launch(const Stage1D &sa, int T, int grid_, int w0, int nw, hipStream_t st) const {
     const int v = tindex(T);
@@ -431,19 +474,10 @@ struct Batch1D {
     const Work1D *wl = d_work[v] + w0;
     const int g = grid_ > 0 ? std::min(grid_, nw) : nw;
     if (nw <= 0) return FR_OK;
-    if (eq == FR_EQ_ADVECTION) {
-      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_ADVECTION, FR_FLUX_HLL>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops,
-                         ph, sa);
-    } else if (flux == FR_FLUX_HLL) {
-      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_HLL>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops, ph,
-                         sa);
-    } else if (flux == FR_FLUX_RUSANOV) {
-      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_RUSANOV>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops,
-                         ph, sa);
-    } else {
-      hipLaunchKernelGGL((fr1d_kernel<FR_EQ_EULER, FR_FLUX_ROE>), dim3(g), dim3(T), 0, st, d_probs, wl, nw, d_ops, ph,
-                         sa);
-    }
+    if (max_nsp <= 5)
+      launch_eq<5>(sa, T, g, wl, nw, ph, st);
+    else
+      launch_eq<NSP_MAX>(sa, T, g, wl, nw, ph, st);
     FR_LAUNCHED("fr1d_kernel");
     return FR_OK;
   }
@@ -474,8 +508,12 @@ struct Batch1D {
     auto one = [&](int w0, int nw, int grid_) {
       if (nw <= 0) return;
       const int g = grid_ > 0 ? std::min(grid_, nw) : nw;
-      hipLaunchKernelGGL(fr1d_diss_kernel, dim3(g), dim3(threads), 0, st, d_probs, d_work[v] + w0, nw, d_ops, u, dmode,
-                         gamma, dt, d_count);
+      if (max_nsp <= 5)
+        hipLaunchKernelGGL(fr1d_diss_kernel<5>, dim3(g), dim3(threads), 0, st, d_probs, d_work[v] + w0, nw, d_ops, u,
+                           dmode, gamma, dt, d_count);
+      else
+        hipLaunchKernelGGL(fr1d_diss_kernel<NSP_MAX>, dim3(g), dim3(threads), 0, st, d_probs, d_work[v] + w0, nw, d_ops,
+                           u, dmode, gamma, dt, d_count);
     };
     if (mode == 0) {
       one(0, nwork[v], grid);
@@ -584,7 +622,7 @@ int fr1d_create(fr1d_ctx **out, const fr1d_desc *d) {
   *out = nullptr;
   FR_REQUIRE(d->equation == FR_EQ_EULER || d->equation == FR_EQ_ADVECTION, FR_E_INVALID, "fr1d_create: unknown equation");
   FR_REQUIRE(d->ncell >= 2, FR_E_INVALID, "fr1d_create: ncell must be >= 2");
-  FR_REQUIRE(d->deg >= 1 && d->deg <= 4, FR_E_UNSUPPORTED, "fr1d_create: degree must be in 1..4");
+  FR_REQUIRE(d->deg >= 1 && d->deg <= NSP_MAX - 1, FR_E_UNSUPPORTED, "fr1d_create: degree must be in 1..8");
   FR_REQUIRE(d->dx > 0.0 && std::isfinite(d->dx), FR_E_INVALID, "fr1d_create: dx must be > 0");
   FR_REQUIRE(d->bc == FR_BC1_DIRICHLET || d->bc == FR_BC1_PERIOD, FR_E_INVALID, "fr1d_create: unknown bc");
   FR_REQUIRE(d->correction >= FR_CORR_RADAU && d->correction <= FR_CORR_HUYNH, FR_E_INVALID, "fr1d_create: unknown correction");
@@ -676,7 +714,7 @@ int fr_ens_create(fr_ens_ctx **out, int nprob, const int *ncell, const int *deg,
   FR_REQUIRE(bc == FR_BC1_DIRICHLET || bc == FR_BC1_PERIOD, FR_E_INVALID, "fr_ens_create: unknown bc");
   for (int b = 0; b < nprob; ++b) {
     FR_REQUIRE(ncell[b] >= 2, FR_E_INVALID, "fr_ens_create: every member needs ncell >= 2");
-    FR_REQUIRE(deg[b] >= 1 && deg[b] <= 4, FR_E_UNSUPPORTED, "fr_ens_create: member degree must be in 1..4");
+    FR_REQUIRE(deg[b] >= 1 && deg[b] <= NSP_MAX - 1, FR_E_UNSUPPORTED, "fr_ens_create: member degree must be in 1..8");
     FR_REQUIRE(dx[b] > 0.0, FR_E_INVALID, "fr_ens_create: member dx must be > 0");
   }
   int ndev = 0;

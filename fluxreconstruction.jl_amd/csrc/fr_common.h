@@ -42,12 +42,13 @@ double djacobi_p(double x, double alpha, double beta, int N);
 bool invert(int n, const double *A, double *Ainv);
 
 // Operators of one degree: Gauss points, ll/lr, dl(m,k) column-major, correction derivatives,
-// quad Vandermonde (nsp^2 x nsp^2) and its inverse, tensor weights wp(k,l).
+// tensor weights wp(k,l) and the 1D Vandermonde pair for degrees 1..8; the quad Vandermonde
+// (nsp^2 x nsp^2) and its inverse for degrees 1..4 only (the 2D spaces; zero above).
 struct HostOps {
   int deg = 0, nsp = 0;
-  double r[8], w[8], ll[8], lr[8], dl[64], dhl[8], dhr[8];
-  double V[625], iV[625], wp[64];
-  double V1[64], iV1[64];
+  double r[16], w[16], ll[16], lr[16], dl[256], dhl[16], dhr[16];
+  double V[625], iV[625], wp[256];
+  double V1[256], iV1[256];
 };
 bool build_ops(int deg, int correction, HostOps &o);
 

@@ -194,7 +194,7 @@ bool invert(int n, const double *A, double *Ainv) {
 
 // FRPSpace1D / FRPSpace2D operator data (src/struct.jl:40-85, 130-227).
 bool build_ops(int deg, int correction, HostOps &o) {
-  if (deg < 1 || deg > 4) return false;
+  if (deg < 1 || deg > 8) return false;
   const int n = deg + 1;
   o.deg = deg;
   o.nsp = n;
@@ -206,17 +206,21 @@ bool build_ops(int deg, int correction, HostOps &o) {
   for (int j = 0; j < n; ++j)
     for (int i = 0; i < n; ++i) o.V1[i + n * j] = jacobi_p(o.r[i], 0.0, 0.0, j);
   if (!invert(n, o.V1, o.iV1)) return false;
+  for (int l = 0; l < n; ++l)
+    for (int k = 0; k < n; ++k) o.wp[k + n * l] = o.w[k] * o.w[l];
   // nodes of the quad: node q = k + n*l sits at (r_k, r_l); mode sk = a*n + b is P_a(r) P_b(s)
   const int np = n * n;
+  if (n > 5) {  // the 2D spaces stop at degree 4
+    std::memset(o.V, 0, sizeof(o.V));
+    std::memset(o.iV, 0, sizeof(o.iV));
+    return true;
+  }
   for (int a = 0; a < n; ++a)
     for (int b = 0; b < n; ++b)
       for (int l = 0; l < n; ++l)
         for (int k = 0; k < n; ++k)
           o.V[(k + n * l) + np * (a * n + b)] = jacobi_p(o.r[k], 0.0, 0.0, a) * jacobi_p(o.r[l], 0.0, 0.0, b);
-  if (!invert(np, o.V, o.iV)) return false;
-  for (int l = 0; l < n; ++l)
-    for (int k = 0; k < n; ++k) o.wp[k + n * l] = o.w[k] * o.w[l];
-  return true;
+  return invert(np, o.V, o.iV);
 }
 
 }  // namespace fr

@@ -267,7 +267,7 @@ enum { FR_EQ_EULER = 0, FR_EQ_ADVECTION = 1 };
 enum { FR_BC1_DIRICHLET = 0, FR_BC1_PERIOD = 1 };
 typedef struct {
   int equation;      /* FR_EQ_* */
-  int ncell, deg;    /* 1 <= deg <= 4 */
+  int ncell, deg;    /* 1 <= deg <= 8 (example/euler_highlevel.jl runs degree 7) */
   double dx;         /* uniform cell size; J = dx/2 (src/struct.jl:42) */
   double gamma;      /* Euler */
   double a;          /* advection speed */
@@ -305,7 +305,7 @@ int fr1d_positivity(fr1d_ctx *ctx, int *n_limited);
 /* ------------------------------------------------------------------ batched 1D Euler ensembles */
 typedef struct fr_ens_ctx fr_ens_ctx;
 /* nprob independent 1D Euler problems (layout of each member as fr1d); member b has ncell[b]
- * cells of degree deg[b] (1..4) and cell size dx[b].  No reference counterpart: the reference
+ * cells of degree deg[b] (1..8) and cell size dx[b].  No reference counterpart: the reference
  * runs one FREulerProblem per script; an ensemble advances many in one launch per stage. */
 int fr_ens_create(fr_ens_ctx **out, int nprob, const int *ncell, const int *deg, const double *dx,
                   double gamma, int flux, int bc);

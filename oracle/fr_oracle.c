@@ -1105,7 +1105,7 @@ int or_fr1d_sensor_filter(int ncell, int nsp, const double *V, const double *iV,
   const double lambda = dt * exp(0.875 * deg) * 0.15;
   int count = 0;
   for (int i = 0; i < ncell; ++i) {
-    double mod[8], nod[8];
+    double mod[16], nod[16];
     double num = 0.0, den = 0.0;
     for (int a = 0; a < nsp; ++a) {
       double s = 0.0;
@@ -1140,7 +1140,7 @@ int or_fr1d_positivity(int ncell, int nsp, double g, const double *weights, cons
                        double *u) {
   int bad = 0;
   for (int i = 0; i < ncell; ++i) {
-    double ue[3 * 8];
+    double ue[3 * 16];
     for (int k = 0; k < 3; ++k)
       for (int p = 0; p < nsp; ++p) ue[p + nsp * k] = u[i + (size_t)ncell * (p + nsp * k)];
     if (or_positive_limiter1d_euler(ue, nsp, g, weights, ll, lr)) {
