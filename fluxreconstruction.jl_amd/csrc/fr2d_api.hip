@@ -261,7 +261,7 @@ int fr2d_destroy(fr2d_ctx *c) {
   if (!c) return FR_OK;
   hipStream_t st = stream_of(*c);
   if (st) (void)hipStreamSynchronize(st);
-  double *bufs[] = {c->raw_u, c->raw_s1, c->raw_s2, c->raw_acc, c->d_in, c->d_out, c->d_dops};
+  double *bufs[] = {c->raw_u, c->raw_s1, c->raw_s2, c->raw_acc, c->d_in, c->d_out, c->d_dops, c->d_mj_ref, c->raw_mj_pad};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->d_count) (void)hipFree(c->d_count);
@@ -587,6 +587,40 @@ int fr2d_set_geometry(fr2d_ctx *c, const double *iJ, const double *n1, const dou
     FR_HIP(hipGraphExecDestroy(c->gexec));
     c->gexec = nullptr;
   }
+  return FR_OK;
+}
+
+int fr2d_set_metrics(fr2d_ctx *c, const double *iJ) {
+  FR_REQUIRE(c && iJ, FR_E_INVALID, "fr2d_set_metrics: null argument");
+  const size_t nx2 = (size_t)c->nx + 2, cells = nx2 * (size_t)(c->ny + 2), np = (size_t)c->nsp * c->nsp;
+  // entry (i, j, k, l, e) at i + (nx+2) (j + (ny+2) (k + nsp (l + nsp e))), e = a + 2 b of the 2x2
+  // matrix; the interior cells must hold finite matrices of positive determinant
+  for (size_t p = 0; p < np; ++p)
+    for (int j = 1; j <= c->ny; ++j)
+      for (int i = 1; i <= c->nx; ++i) {
+        const size_t at = (size_t)i + nx2 * (size_t)j + cells * p;
+        const double a = iJ[at], cc = iJ[at + cells * np], b = iJ[at + 2 * cells * np], d = iJ[at + 3 * cells * np];
+        FR_REQUIRE(std::isfinite(a) && std::isfinite(b) && std::isfinite(cc) && std::isfinite(d), FR_E_INVALID,
+                   "fr2d_set_metrics: iJ not finite");
+        FR_REQUIRE(a * d - b * cc > 0.0, FR_E_INVALID, "fr2d_set_metrics: iJ must have a positive determinant");
+      }
+  hipStream_t st = stream_of(*c);
+  if (!c->d_mj_ref) FR_HIP(hipMalloc(&c->d_mj_ref, c->len * sizeof(double)));
+  if (!c->d_mj_pad) {
+    int rc = alloc_padded(*c, c->raw_mj_pad, c->d_mj_pad);
+    if (rc) return rc;
+  }
+  // a captured step sequence holds the old geometry in its kernel arguments
+  if (c->gexec) {
+    FR_HIP(hipStreamSynchronize(st));
+    FR_HIP(hipGraphExecDestroy(c->gexec));
+    c->gexec = nullptr;
+  }
+  FR_HIP(hipMemcpyAsync(c->d_mj_ref, iJ, c->len * sizeof(double), hipMemcpyHostToDevice, st));
+  int rc = copy_to_padded(*c, c->d_mj_pad, iJ, hipMemcpyHostToDevice, st);
+  if (rc) return rc;
+  FR_HIP(hipStreamSynchronize(st));
+  c->geom = 2;
   return FR_OK;
 }
 

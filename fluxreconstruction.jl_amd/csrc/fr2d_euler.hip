@@ -37,9 +37,12 @@ static_assert(2 * (TX + TY) <= 64, "the halo tasks of one line must fit in one w
 // instruction on addresses
 enum { ADDR_FLAT = 0, ADDR_BUFFER = 1 };
 
-// cell geometry: rectangles (iJ diagonal, faces normal to the axes) or uniform parallelograms
-// (full 2x2 iJ, faces solved in the frames of the unit normals n1, n2)
-enum { GEO_RECT = 0, GEO_AFFINE = 1 };
+// cell geometry: rectangles (iJ diagonal, faces normal to the axes), uniform parallelograms
+// (full 2x2 iJ, faces solved in the frames of the unit normals n1, n2), or point-varying metrics
+// (a 2x2 iJ per solution point, read from an array shaped like a state whose four planes per
+// point hold r_x, s_x, r_y, s_y; faces as for rectangles, the face flux of a point scaled by its
+// own r_x (x faces) or s_y (y faces), as example/shock-vortex.jl:102-115 writes it)
+enum { GEO_RECT = 0, GEO_AFFINE = 1, GEO_POINT = 2 };
 
 struct Geo2D {
   int nx, ny, ntx, nty;
@@ -53,6 +56,7 @@ struct Geo2D {
   double iJ12, iJ21;          // off-diagonal inverse-Jacobian entries (GEO_AFFINE)
   double n1c, n1s, n2c, n2s;  // unit normals of the x faces and y faces (GEO_AFFINE)
   double q1, q2;              // contravariant scales of the face fluxes, (iJ n1)_1 and (iJ n2)_2
+  const double *mj;           // per-point metric in the layout of the state arrays (GEO_POINT)
 };
 
 // epilogues: residual du; RK stage out = a0*u0 + a1*u + b*du (b != 0); the general stage form
@@ -86,14 +90,16 @@ __device__ __forceinline__ void strip_tile(int t, int ntx, int nty, int sw, int 
 
 // Four workgroups per CU at nsp = 4: HLL and Rusanov fit 128 VGPRs (HLL with at most a few
 // spilled values); Roe and the RK4 accumulator form would spill heavily and keep the default.
-template <int NSP, int FLUX, int EPI>
+// The point-metric instances keep the default as well: their metric values stay live from the
+// x sweep to the corrections.
+template <int NSP, int FLUX, int EPI, int GEO>
 constexpr int fr2d_min_blocks() {
-  return (NSP == 4 && FLUX != FR_FLUX_ROE && EPI != EPI_ACC) ? 4 : 1;
+  return (NSP == 4 && FLUX != FR_FLUX_ROE && EPI != EPI_ACC && GEO != GEO_POINT) ? 4 : 1;
 }
 
 template <int NSP, int FLUX, int EPI, int ADDR, int GEO>
-__global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void fr2d_kernel(const Geo2D geo,
-                                                                                         const StageArgs sa) {
+__global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI, GEO>())) void fr2d_kernel(const Geo2D geo,
+                                                                                              const StageArgs sa) {
   // operators are compile-time constants (Gauss-Legendre points, Radau correction), so they
   // occupy no scalar registers for the length of the kernel
   constexpr StdOps2D<NSP> ops{};
@@ -114,11 +120,13 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
   const double g = geo.g, gm1 = geo.gm1;
   // scale of the x and y contributions in the result: -iJ (residual, RK4 form) or -b*iJ (stage,
   // b != 0 on the host); the stage carries a1*u inside the x part as (a1 / sxs)*u.  Parallelogram
-  // cells carry the metric inside the contravariant fluxes and the face scales q1, q2 instead.
+  // cells carry the metric inside the contravariant fluxes and the face scales q1, q2 instead;
+  // point metrics carry it inside the fluxes and scale each point's face terms.
   constexpr bool AFF = GEO == GEO_AFFINE;
+  constexpr bool PT = GEO == GEO_POINT;
   const double bsc = EPI == EPI_STAGE ? sa.b : 1.0;
-  const double sxs = AFF ? -bsc : -bsc * geo.iJ11;
-  const double sys = AFF ? -bsc : -bsc * geo.iJ22;
+  const double sxs = (AFF || PT) ? -bsc : -bsc * geo.iJ11;
+  const double sys = (AFF || PT) ? -bsc : -bsc * geo.iJ22;
   const double ua = EPI == EPI_STAGE ? sa.a1 / sxs : 0.0;
 
   auto SU = [&](int k, int l, int m) -> double & { return sU[((k * NSP + l) * 4 + m) * TE + lane]; };
@@ -320,17 +328,25 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
 #pragma unroll
       for (int m = 0; m < 4; ++m) SU(k, line, m) = ur[k][m];
   }
+  // point metrics: r_x of the row's points, which also scales their x-face terms
+  double jx[NSP];
   {
     double f1[NSP][4];
 #pragma unroll
     for (int k = 0; k < NSP; ++k) {
       XState xs;
       xstate(ur[k], gm1, xs);
-      if constexpr (AFF) {  // contravariant flux iJ11 F + iJ12 G
+      if constexpr (AFF || PT) {  // contravariant flux iJ11 F + iJ12 G
         const double v = ur[k][2] * xs.rinv;
         const double G[4] = {ur[k][2], ur[k][1] * v, ur[k][2] * v + xs.p, (ur[k][3] + xs.p) * v};
+        double a = geo.iJ11, b = geo.iJ12;
+        if constexpr (PT) {
+          a = act ? LD(geo.mj, k, line, 0, ebo) : 0.0;
+          b = act ? LD(geo.mj, k, line, 2, ebo) : 0.0;
+          jx[k] = a;
+        }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) f1[k][m] = geo.iJ11 * xs.F[m] + geo.iJ12 * G[m];
+        for (int m = 0; m < 4; ++m) f1[k][m] = a * xs.F[m] + b * G[m];
       } else {
 #pragma unroll
         for (int m = 0; m < 4; ++m) f1[k][m] = xs.F[m];
@@ -387,7 +403,10 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
       const double cr = has_r ? SX(m, ey, ex) : 0.0;
 #pragma unroll
       for (int k = 0; k < NSP; ++k) {
-        res[k][m] = sxs * (dxr[k][m] + cl[m] * ops.dhl[k] + cr * ops.dhr[k]);
+        if constexpr (PT)
+          res[k][m] = sxs * (dxr[k][m] + jx[k] * (cl[m] * ops.dhl[k] + cr * ops.dhr[k]));
+        else
+          res[k][m] = sxs * (dxr[k][m] + cl[m] * ops.dhl[k] + cr * ops.dhr[k]);
         fr_pin(res[k][m]);
       }
     }
@@ -424,7 +443,8 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
 #pragma unroll
     for (int m = 0; m < 4; ++m) SY(m, 0, hpos) = htr[m];
   }
-  // the y part of the result, scaled like the x part
+  // the y part of the result, scaled like the x part; point metrics: s_y of the column's points
+  double jy[NSP];
   {
     double f2[NSP][4];
 #pragma unroll
@@ -436,11 +456,17 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
       f2[q][1] = uc[q][1] * vv;
       f2[q][2] = uc[q][2] * vv + p;
       f2[q][3] = (uc[q][3] + p) * vv;
-      if constexpr (AFF) {  // contravariant flux iJ21 F + iJ22 G
+      if constexpr (AFF || PT) {  // contravariant flux iJ21 F + iJ22 G
         const double uu = uc[q][1] * r;
         const double F[4] = {uc[q][1], uc[q][1] * uu + p, uc[q][2] * uu, (uc[q][3] + p) * uu};
+        double a = geo.iJ21, b = geo.iJ22;
+        if constexpr (PT) {  // point (k, l) = (line, q) of the own element
+          a = act ? LD(geo.mj, line, q, 1, ebo) : 0.0;
+          b = act ? LD(geo.mj, line, q, 3, ebo) : 0.0;
+          jy[q] = b;
+        }
 #pragma unroll
-        for (int m = 0; m < 4; ++m) f2[q][m] = geo.iJ21 * F[m] + geo.iJ22 * f2[q][m];
+        for (int m = 0; m < 4; ++m) f2[q][m] = a * F[m] + b * f2[q][m];
       }
     }
 #pragma unroll
@@ -514,14 +540,19 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI>())) void
     for (int m = 0; m < 4; ++m) {
       const double cb = sys * SY(m, ey, ex), ct = sys * SY(m, ey + 1, ex);
 #pragma unroll
-      for (int k = 0; k < NSP; ++k) dyr[k][m] += cb * ops.dhl[k] + ct * ops.dhr[k];
+      for (int k = 0; k < NSP; ++k) {
+        if constexpr (PT)
+          dyr[k][m] += jy[k] * (cb * ops.dhl[k] + ct * ops.dhr[k]);
+        else
+          dyr[k][m] += cb * ops.dhl[k] + ct * ops.dhr[k];
+      }
     }
     if (ex == ncx - 1) {  // right face of the row's last element, solved by the right task
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
         const double cr = sxs * SX(m, ey, ex);
 #pragma unroll
-        for (int k = 0; k < NSP; ++k) res[k][m] += cr * ops.dhr[k];
+        for (int k = 0; k < NSP; ++k) res[k][m] += (PT ? cr * jx[k] : cr) * ops.dhr[k];
       }
     }
   }
@@ -619,7 +650,6 @@ __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, int pitch, lon
 // ------------------------------------------------------------------ launch dispatch
 template <int NSP, int FLUX, int EPI>
 static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
-  const bool aff = c.geom != 0;
   Geo2D geo;
   geo.nx = c.nx;
   geo.ny = c.ny;
@@ -646,19 +676,27 @@ static int launch_nsp(const Fr2dCtx &c, const StageArgs &sa, hipStream_t st) {
   geo.n2s = c.n2[1];
   geo.q1 = c.iJ11 * c.n1[0] + c.iJ12 * c.n1[1];
   geo.q2 = c.iJ21 * c.n2[0] + c.iJ22 * c.n2[1];
+  // the metric copy in the layout of this launch's arrays (the padded copy equals the
+  // reference-layout one when the padded pitch is nx + 2)
+  geo.mj = geo.pitch == c.nx + 2 ? c.d_mj_ref : c.d_mj_pad;
+  if (c.geom == 2 && !geo.mj) return fail(FR_E_INVALID, "fr2d: point metrics missing on the device");
   const unsigned nb = (unsigned)(geo.ntx * geo.nty);  // one tile per workgroup
   const bool stag = !caller && c.stagger > 0 && fr2d_single_round(c);
   geo.stag_cu = stag ? c.num_cu : 0;
   geo.stag = stag ? c.stagger : 0;
   const bool buf = (caller ? c.len : c.lenp) * sizeof(double) < (size_t)0xFFFFFFFFu;
-  if (buf && !aff)
+  if (c.geom == 0 && buf)
     hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_BUFFER, GEO_RECT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
-  else if (!aff)
+  else if (c.geom == 0)
     hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_FLAT, GEO_RECT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
-  else if (buf)
+  else if (c.geom == 1 && buf)
     hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_BUFFER, GEO_AFFINE>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
-  else
+  else if (c.geom == 1)
     hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_FLAT, GEO_AFFINE>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
+  else if (buf)
+    hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_BUFFER, GEO_POINT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
+  else
+    hipLaunchKernelGGL((fr2d_kernel<NSP, FLUX, EPI, ADDR_FLAT, GEO_POINT>), dim3(nb), dim3(NSP * 64), 0, st, geo, sa);
   FR_LAUNCHED("fr2d_kernel");
   return FR_OK;
 }
@@ -719,10 +757,11 @@ int fr2d_launch_stage(const Fr2dCtx &c, const double *d_u0, const double *d_us, 
 }
 
 bool fr2d_single_round(const Fr2dCtx &c) {
-  // four workgroups per CU for the nsp = 4 stage instances (Rusanov, HLL): more tiles than one
-  // workgroup per CU and no more than all four slots
+  // four workgroups per CU for the nsp = 4 stage instances (Rusanov, HLL, cells without point
+  // metrics): more tiles than one workgroup per CU and no more than all four slots
   const long long nb = (long long)((c.nx + TX - 1) / TX) * ((c.ny + TY - 1) / TY);
-  return c.nsp == 4 && c.flux != FR_FLUX_ROE && c.num_cu > 0 && nb > c.num_cu && nb <= 4LL * c.num_cu;
+  return c.nsp == 4 && c.flux != FR_FLUX_ROE && c.geom != 2 && c.num_cu > 0 && nb > c.num_cu &&
+         nb <= 4LL * c.num_cu;
 }
 
 // Times the stage launches of SSP-RK3 steps (state and second stage buffer -> first stage buffer,

@@ -9,10 +9,13 @@ struct Fr2dCtx {
   int nx = 0, ny = 0, deg = 0, nsp = 0, flux = 0, bc = 0;
   double gamma = 1.4, iJ11 = 1.0, iJ22 = 1.0;
   // cell geometry: 0 rectangles (iJ11, iJ22); 1 uniform parallelograms with the full inverse
-  // Jacobian and the unit normals of the x faces (n1) and y faces (n2)
+  // Jacobian and the unit normals of the x faces (n1) and y faces (n2); 2 point-varying metrics
+  // (d_mj_ref / d_mj_pad: the inverse Jacobian of every solution point in the reference and in
+  // the padded layout, four planes per point like a state's variables)
   int geom = 0;
   double iJ12 = 0.0, iJ21 = 0.0;
   double n1[2] = {1.0, 0.0}, n2[2] = {0.0, 1.0};
+  double *d_mj_ref = nullptr, *d_mj_pad = nullptr, *raw_mj_pad = nullptr;
   HostOps ops;
   size_t len = 0;  // doubles per state array in the reference layout, (nx+2)(ny+2) nsp^2 4
   // The device-resident arrays (state, stage buffers, RK4 accumulator) use a padded form of the

@@ -318,6 +318,29 @@ int fr_rs_jacobi_quad(double r, double s, const double *vert, double *J) {
   return FR_OK;
 }
 
+int fr_rs_jacobi_mesh(int ncx, int ncy, const double *vert, int n, const double *r, double *J, double *iJ) {
+  FR_REQUIRE(ncx > 0 && ncy > 0 && n > 0 && vert && r && J, FR_E_INVALID, "fr_rs_jacobi_mesh: bad arguments");
+  const size_t cells = (size_t)ncx * ncy, pts = cells * n * n;
+  for (int l = 0; l < n; ++l)
+    for (int k = 0; k < n; ++k)
+      for (size_t c = 0; c < cells; ++c) {
+        double v[8];
+        for (int q = 0; q < 8; ++q) v[q] = vert[c + cells * q];  // vertex q % 4, coordinate q / 4
+        double m[4];
+        fr_rs_jacobi_quad(r[k], r[l], v, m);
+        const size_t at = c + cells * ((size_t)k + n * l);
+        for (int e = 0; e < 4; ++e) J[at + pts * e] = m[e];
+        if (iJ) {  // inverse of [a b; c d] = [d -b; -c a] / (ad - bc)
+          const double a = m[0], cc = m[1], b = m[2], d = m[3], det = a * d - b * cc;
+          iJ[at] = d / det;
+          iJ[at + pts] = -cc / det;
+          iJ[at + 2 * pts] = -b / det;
+          iJ[at + 3 * pts] = a / det;
+        }
+      }
+  return FR_OK;
+}
+
 int fr_shock_detector(double Se, int deg, double S0, double kappa) {
   (void)deg;
   double sigma;

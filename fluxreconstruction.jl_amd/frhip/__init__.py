@@ -10,6 +10,7 @@ from .space import (  # noqa: F401
     Quad,
     PSpace1D,
     PSpace2D,
+    CSpace2D,
     FRPSpace1D,
     FRPSpace2D,
     legendre_point,
@@ -25,6 +26,7 @@ from .space import (  # noqa: F401
     vandermonde_matrix,
     dvandermonde_matrix,
     rs_jacobi,
+    rs_jacobi_mesh,
     shock_detector,
     L1_error,
     L2_error,

@@ -92,6 +92,7 @@ SIGNATURES = {
     "fr_vandermonde_quad": (C.c_int, [C.c_int, _dp, _dp, C.c_int, _dp]),
     "fr_inv": (C.c_int, [C.c_int, _dp, _dp]),
     "fr_rs_jacobi_quad": (C.c_int, [C.c_double, C.c_double, _dp, _dp]),
+    "fr_rs_jacobi_mesh": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
     "fr_shock_detector": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_double]),
     "fr2d_create": (C.c_int, [C.POINTER(_vp), C.POINTER(Fr2dDesc)]),
     "fr2d_destroy": (C.c_int, [_vp]),
@@ -117,6 +118,7 @@ SIGNATURES = {
     "fr2d_check_state": (C.c_int, [_vp]),
     "fr2d_primitives": (C.c_int, [_vp, _dp]),
     "fr2d_set_geometry": (C.c_int, [_vp, _dp, _dp, _dp]),
+    "fr2d_set_metrics": (C.c_int, [_vp, _dp]),
     "fr2d_run_host_loop": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                      C.c_int, _ip]),
     "fr2d_primitives_dev": (C.c_int, [_vp, _vp]),

@@ -49,7 +49,8 @@ class FREuler2D:
         d.gamma = self.gamma
         d.flux = _lib.FLUX[flux]
         d.bc = _lib.BC[bc]
-        iJ = np.asarray(ps.iJ_elem, dtype=np.float64)
+        point = getattr(ps, "iJ_elem", 0) is None  # per-point metrics (an FRPSpace2D over any mesh)
+        iJ = np.eye(2) if point else np.asarray(ps.iJ_elem, dtype=np.float64)
         rect = iJ[0, 1] == 0.0 and iJ[1, 0] == 0.0
         d.iJ11 = float(iJ[0, 0]) if rect else 1.0
         d.iJ22 = float(iJ[1, 1]) if rect else 1.0
@@ -58,7 +59,9 @@ class FREuler2D:
         check(_lib.lib().fr2d_create(C.byref(h), C.byref(d)))
         self.h = h
         self.len = _lib.lib().fr2d_state_len(h)
-        if not rect:
+        if point:
+            self.set_metrics(ps.iJ)
+        elif not rect:
             self.set_geometry(iJ)
 
     def set_geometry(self, iJ, n1=None, n2=None):
@@ -70,6 +73,18 @@ class FREuler2D:
         arrs += [None if n is None else np.ascontiguousarray(n, dtype=np.float64).reshape(2) for n in (n1, n2)]
         self._geom = arrs  # kept alive for the duration of the call only; the library copies them
         check(_lib.lib().fr2d_set_geometry(self.h, *[None if a is None else dptr(a) for a in arrs]))
+
+    def set_metrics(self, iJ):
+        """Point-varying metrics: iJ[i, j, k, l] is the 2x2 inverse Jacobian of solution point
+        (k, l) of cell (i, j), an array of shape (nx+2, ny+2, nsp, nsp, 2, 2) (FRPSpace2D.iJ over a
+        curvilinear mesh; src/struct.jl:134-142).  The residual is example/shock-vortex.jl:26-118
+        with each point's metric (fr2d_set_metrics)."""
+        iJ = np.asarray(iJ, dtype=np.float64)
+        want = (self.ps.nx + 2, self.ps.ny + 2, self.nsp, self.nsp, 2, 2)
+        if iJ.shape != want:
+            raise ValueError(f"iJ must have shape {want}")
+        flat = np.ascontiguousarray(iJ.ravel(order="F"))
+        check(_lib.lib().fr2d_set_metrics(self.h, dptr(flat)))
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:

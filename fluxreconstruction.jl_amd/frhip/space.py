@@ -206,11 +206,63 @@ class PSpace2D:
         self.vertices = v
 
 
+class CSpace2D:
+    """Polar (curvilinear) 2D mesh, KitBase's CSpace2D(r0, r1, nr, θ0, θ1, nθ, ngr, ngθ) as the
+    reference's curvilinear scripts build it (dev/cylinder.jl:23, dev/curve_euler.jl:24): cells of
+    constant radial width dr and angle dθ with ngr / ngθ ghost layers; centres r[i, j], θ[i, j],
+    x = r cos θ, y = r sin θ; dr, dθ and the arc length darc = r dθ per cell (the scripts read
+    ps.nr, ps.nθ, ps.dθ[1, j], ps.x, ps.y); vertices[i, j, v, d] in the order of PSpace2D with r
+    in place of x and θ in place of y (v = 1 (r-, θ-), 2 (r+, θ-), 3 (r+, θ+), 4 (r-, θ+)).  KitBase
+    is not part of the reference tree, so these conventions follow its use in the scripts and are
+    not pinned against KitBase itself.  nx, ny, ngx, ngy alias nr, nθ, ngr, ngθ for the solvers."""
+
+    def __init__(self, r0, r1, nr, θ0, θ1, nθ, ngr=0, ngθ=0):
+        self.r0, self.r1, self.nr = float(r0), float(r1), int(nr)
+        self.θ0, self.θ1, self.nθ = float(θ0), float(θ1), int(nθ)
+        self.ngr, self.ngθ = int(ngr), int(ngθ)
+        dr, dθ = (self.r1 - self.r0) / nr, (self.θ1 - self.θ0) / nθ
+        ii = np.arange(1 - ngr, nr + ngr + 1)
+        jj = np.arange(1 - ngθ, nθ + ngθ + 1)
+        R, T = np.meshgrid(self.r0 + (ii - 0.5) * dr, self.θ0 + (jj - 0.5) * dθ, indexing="ij")
+        self.r = np.asfortranarray(R)
+        self.θ = np.asfortranarray(T)
+        self.x = np.asfortranarray(R * np.cos(T))
+        self.y = np.asfortranarray(R * np.sin(T))
+        self.dr = np.full_like(self.r, dr)
+        self.dθ = np.full_like(self.θ, dθ)
+        self.darc = np.asfortranarray(R * dθ)
+        v = np.zeros(R.shape + (4, 2), order="F")
+        for q, (sr, st) in enumerate(((-1, -1), (1, -1), (1, 1), (-1, 1))):
+            rv, tv = R + sr * dr / 2, T + st * dθ / 2
+            v[..., q, 0], v[..., q, 1] = rv * np.cos(tv), rv * np.sin(tv)
+        self.vertices = v
+
+    nx = property(lambda self: self.nr)
+    ny = property(lambda self: self.nθ)
+    ngx = property(lambda self: self.ngr)
+    ngy = property(lambda self: self.ngθ)
+
+
+def rs_jacobi_mesh(vertices, r):
+    """J[i, j][k, l] = rs_jacobi(r[k], r[l], vertices[i, j]) of every cell and solution point and
+    its inverse (src/struct.jl:134-142, geo_jacobi.jl:90-98), as (ncx, ncy, n, n, 2, 2) arrays."""
+    v = np.asfortranarray(vertices, dtype=np.float64)
+    if v.ndim != 4 or v.shape[2:] != (4, 2):
+        raise ValueError("vertices must have shape (ncx, ncy, 4, 2)")
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    ncx, ncy, n = v.shape[0], v.shape[1], r.size
+    J = np.zeros(ncx * ncy * n * n * 4)
+    iJ = np.zeros_like(J)
+    check(_lib.lib().fr_rs_jacobi_mesh(ncx, ncy, dptr(_f(v)), n, dptr(r), dptr(J), dptr(iJ)))
+    shape = (ncx, ncy, n, n, 2, 2)
+    return J.reshape(shape, order="F"), iJ.reshape(shape, order="F")
+
+
 class _Forward:
     """Forward KitBase mesh fields through the FR space (src/tools.jl:5-37)."""
 
     _base_fields = ("x0", "x1", "nx", "y0", "y1", "ny", "ng", "ngx", "ngy", "x", "y", "dx", "dy", "dx0", "dy0",
-                    "vertices")
+                    "vertices", "r0", "r1", "nr", "θ0", "θ1", "nθ", "ngr", "ngθ", "r", "θ", "dr", "dθ", "darc")
 
     def __getattr__(self, name):
         if name in self._base_fields and "base" in self.__dict__:
@@ -242,16 +294,23 @@ class FRPSpace1D(_Forward):
 
 
 class FRPSpace2D(_Forward):
-    """2D FR space on uniform quads (src/struct.jl:99-245).
+    """2D FR space (src/struct.jl:99-245): FRPSpace2D(x0, x1, nx, y0, y1, ny, deg, ngx, ngy) on
+    uniform rectangles, or FRPSpace2D(base, deg) over any mesh with a vertices[i, j, v, d] array
+    (PSpace2D, CSpace2D, ...).
 
     Fields: deg, np, xpl, xpg(i, j, k, l, d), wp(k, l), dl, ll, lr, dll, dlr, dhl, dhr, V, iV
-    (tensor Vandermonde, modes i-outer / j-inner), J / iJ per element (uniform rectangles: the
-    2x2 matrices diag(dx/2, dy/2) and its inverse, exposed as (nx+2g, ny+2g, 2, 2) views)."""
+    (tensor Vandermonde, modes i-outer / j-inner) and the metrics.  On uniform rectangles
+    (`uniform`) J / iJ are one 2x2 matrix per element (diag(dx/2, dy/2) and its inverse, exposed
+    as (nx+2g, ny+2g, 2, 2) views; J_elem / iJ_elem).  Over any other mesh they are per solution
+    point as in the reference, J[i, j, k, l] = rs_jacobi(r[k], r[l], vertices[i, j]) with iJ its
+    inverse, arrays of shape (nx+2g, ny+2g, n, n, 2, 2), and Ji the Jacobians the reference forms
+    at its interface points (struct.jl:144-158)."""
 
-    def __init__(self, x0, x1, nx, y0, y1, ny, deg, ngx=0, ngy=0):
-        base = x0 if isinstance(x0, PSpace2D) else PSpace2D(x0, x1, nx, y0, y1, ny, ngx, ngy)
-        if isinstance(x0, PSpace2D):
-            deg = x1
+    def __init__(self, x0, x1=None, nx=None, y0=None, y1=None, ny=None, deg=None, ngx=0, ngy=0):
+        if hasattr(x0, "vertices"):  # FRPSpace2D(base, deg)
+            base, deg = x0, x1
+        else:
+            base = PSpace2D(x0, x1, nx, y0, y1, ny, ngx, ngy)
         self.base = base
         self.deg = int(deg)
         self.np = (deg + 1) ** 2
@@ -259,12 +318,17 @@ class FRPSpace2D(_Forward):
         r, w = gausslegendre(n)
         self.xpl = r
         self.wp = np.asfortranarray(np.outer(w, w))
-        Jc = rs_jacobi(0.0, 0.0, base.vertices[0, 0])
-        self.J_elem = Jc
-        self.iJ_elem = inv(Jc)
         shape = base.x.shape
-        self.J = np.broadcast_to(Jc, shape + (2, 2))
-        self.iJ = np.broadcast_to(self.iJ_elem, shape + (2, 2))
+        self.uniform = isinstance(base, PSpace2D)
+        if self.uniform:
+            Jc = rs_jacobi(0.0, 0.0, base.vertices[0, 0])
+            self.J_elem = Jc
+            self.iJ_elem = inv(Jc)
+            self.J = np.broadcast_to(Jc, shape + (2, 2))
+            self.iJ = np.broadcast_to(self.iJ_elem, shape + (2, 2))
+        else:
+            self.J_elem = self.iJ_elem = None
+            self.J, self.iJ = rs_jacobi_mesh(base.vertices, r)
         v = base.vertices
         rk = r[:, None]
         rl = r[None, :]
@@ -289,3 +353,21 @@ class FRPSpace2D(_Forward):
     @property
     def nsp(self) -> int:
         return self.deg + 1
+
+    @property
+    def Ji(self) -> np.ndarray:
+        """The Jacobians the reference forms at its interface points (struct.jl:144-158), shape
+        (nx+2g, ny+2g, 4, n, 2, 2).  rs_jacobi(ri, si, vertices) there takes the linear index f over
+        the first axis of the 4 x n point matrices only (geo_jacobi.jl:93-94), so Ji[i, j][f, q] is
+        the Jacobian at (ri[f, 1], si[f, 1]) for every q: (r_1, 0), (1, r_1), (r_n, 1), (0, r_n).
+        Computed on first use; no kernel reads it."""
+        if getattr(self, "_Ji", None) is None:
+            r, n, v = self.xpl, self.deg + 1, self.base.vertices
+            pts = ((r[0], 0.0), (1.0, r[0]), (r[-1], 1.0), (0.0, r[-1]))
+            Ji = np.zeros(v.shape[:2] + (4, n, 2, 2), order="F")
+            for f, (rr, ss) in enumerate(pts):
+                J, _ = rs_jacobi_mesh(v, np.array([rr, ss]))
+                # the tensor grid's point (k, l) = (0, 1) is (r[0], r[1]) = (rr, ss)
+                Ji[:, :, f, :] = J[:, :, 0, 1][:, :, None]
+            self._Ji = Ji
+        return self._Ji

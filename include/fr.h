@@ -82,6 +82,12 @@ int fr_vandermonde_quad(int N, const double *r, const double *s, int n, double *
 int fr_inv(int n, const double *A, double *Ainv);
 /* bilinear quad Jacobian [xr xs; yr ys] at (r,s), vertices 4x2 (src/Geometry/geo_jacobi.jl:77-88) */
 int fr_rs_jacobi_quad(double r, double s, const double *vert, double *J);
+/* the per-point Jacobians of an FRPSpace2D over any quad mesh (src/struct.jl:134-142,
+ * J[i, j][k, l] = rs_jacobi(r[k], r[l], vertices[i, j]), iJ its inverse; geo_jacobi.jl:90-98):
+ * vert is the mesh's (ncx, ncy, 4, 2) vertex array, J and iJ (NULL: not formed) are
+ * (ncx, ncy, n, n, 2, 2), all column-major.  The 2x2 inverse is the closed form
+ * [d -b; -c a] / (ad - bc). */
+int fr_rs_jacobi_mesh(int ncx, int ncy, const double *vert, int n, const double *r, double *J, double *iJ);
 /* Persson-Peraire switch; returns 1 (shock) / 0 (src/dissipation.jl:13-23) */
 int fr_shock_detector(double Se, int deg, double S0, double kappa);
 /* filter_exp(N, s, V, Nc, invV) (src/Polynomial/poly_filter.jl:11-24): the nodal exponential
@@ -228,6 +234,18 @@ int fr2d_check_state(fr2d_ctx *ctx);
  * global_frame) and scaled by (iJ n1)_1 / (iJ n2)_2.  Without this call the cells are the
  * rectangles of example/shock-vortex.jl (iJ11, iJ22 of fr2d_desc). */
 int fr2d_set_geometry(fr2d_ctx *ctx, const double *iJ, const double *n1, const double *n2);
+/* Cells with point-varying metrics: the FRPSpace2D of any quad mesh (src/struct.jl:130-134,
+ * J[i, j][k, l] = rs_jacobi(r[k], r[l], vertices[i, j]) and iJ its inverse, per solution point;
+ * src/Geometry/geo_jacobi.jl:77-98).  iJ holds iJ[i, j][k, l] of every cell in the reference state
+ * layout with the 2x2 entry in place of the variable: element (i, j, k, l, e) at
+ * i + (nx+2) (j + (ny+2) (k + nsp (l + nsp e))), e = 0 r_x, 1 s_x, 2 r_y, 3 s_y (column-major),
+ * ghost cells included but not read.  The residual is then example/shock-vortex.jl:26-118 as
+ * written, with the metric of each point: volume flux iJ[k, l] [F, G] (:48-55), interface fluxes
+ * as on rectangles (:73-89), the x-face flux of point (k, l) scaled by its own iJ[k, l][1, 1] and
+ * the y-face flux by iJ[k, l][2, 2] (:102-115).  Interior entries must be finite with a positive
+ * determinant.  The library keeps two device copies (both layouts); replaces fr2d_set_geometry's
+ * cells. */
+int fr2d_set_metrics(fr2d_ctx *ctx, const double *iJ);
 /* Primitive variables of every solution point on the global point grid, the post-processing block
  * of example/shock-vortex.jl:333-352 (its `sol`): column-major (nx*nsp, ny*nsp, 4) with
  * sol[(i-1)*nsp + k, (j-1)*nsp + l, :] = (rho, U, V, 1/lambda), lambda = rho/(2p) as in KitBase

@@ -492,9 +492,17 @@ static void or_fr2d_rhs_ghost(const or_fr2d *d, const double *u, double *du) {
           double w[4], Fg[4], Gg[4];
           for (int m = 0; m < 4; ++m) w[m] = U5(u, i, j, k, l, m);
           or_euler_flux2(w, g, Fg, Gg);
+          /* f = iJ[i, j][k, l] * [fg, gg] (:50-53) */
+          double a11 = iJ11, a21 = iJ21, a12 = iJ12, a22 = iJ22;
+          if (d->iJp) {
+            a11 = U5(d->iJp, i, j, k, l, 0);
+            a21 = U5(d->iJp, i, j, k, l, 1);
+            a12 = U5(d->iJp, i, j, k, l, 2);
+            a22 = U5(d->iJp, i, j, k, l, 3);
+          }
           for (int s = 0; s < 4; ++s) {
-            F6(i, j, k, l, s, 0) = iJ11 * Fg[s] + iJ12 * Gg[s];
-            F6(i, j, k, l, s, 1) = iJ21 * Fg[s] + iJ22 * Gg[s];
+            F6(i, j, k, l, s, 0) = a11 * Fg[s] + a12 * Gg[s];
+            F6(i, j, k, l, s, 1) = a21 * Fg[s] + a22 * Gg[s];
           }
         }
 
@@ -592,19 +600,21 @@ static void or_fr2d_rhs_ghost(const or_fr2d *d, const double *u, double *du) {
           }
 
   /* correction assembly (:102-115); the common fluxes scaled by q1 = iJ11, q2 = iJ22 in the
-   * shock-vortex form */
+   * shock-vortex form, by iJ[i, j][k, l][1, 1] and [2, 2] of the point with a point metric */
 #pragma omp parallel for collapse(2) schedule(static)
   for (int m = 0; m < 4; ++m)
     for (int l = 0; l < nsp; ++l)
       for (int k = 0; k < nsp; ++k)
         for (int j = 1; j <= ny; ++j)
           for (int i = 1; i <= nx; ++i) {
+            const double p1 = d->iJp ? U5(d->iJp, i, j, k, l, 0) : q1;
+            const double p2 = d->iJp ? U5(d->iJp, i, j, k, l, 3) : q2;
             U5(du, i, j, k, l, m) =
                 -(R5(rhs1, i, j, k, l, m) + R5(rhs2, i, j, k, l, m) +
-                  (FX(i, j, l, m) * q1 - FF(i, j, 3, l, m, 0)) * dhl[k] +
-                  (FX(i + 1, j, l, m) * q1 - FF(i, j, 1, l, m, 0)) * dhr[k] +
-                  (FY(i, j, k, m) * q2 - FF(i, j, 0, k, m, 1)) * dhl[l] +
-                  (FY(i, j + 1, k, m) * q2 - FF(i, j, 2, k, m, 1)) * dhr[l]);
+                  (FX(i, j, l, m) * p1 - FF(i, j, 3, l, m, 0)) * dhl[k] +
+                  (FX(i + 1, j, l, m) * p1 - FF(i, j, 1, l, m, 0)) * dhr[k] +
+                  (FY(i, j, k, m) * p2 - FF(i, j, 0, k, m, 1)) * dhl[l] +
+                  (FY(i, j + 1, k, m) * p2 - FF(i, j, 2, k, m, 1)) * dhr[l]);
           }
 
   free(f);

@@ -74,6 +74,11 @@ typedef struct {
    * n1 = (1, 0), n2 = (0, 1) the two agree bitwise. */
   int geom;
   double n1[2], n2[2];
+  /* optional point-varying metric (geom 0 only): iJ[i, j][k, l] of every solution point in the
+   * state layout with the 2x2 entry e = a + 2b in place of the variable (r_x, s_x, r_y, s_y); NULL
+   * takes the uniform iJ above.  src/struct.jl:134-142 builds it per point for any mesh, and
+   * example/shock-vortex.jl:48-55,102-115 reads it per point. */
+  const double *iJp;
 } or_fr2d;
 
 size_t or_fr2d_len(const or_fr2d *d);

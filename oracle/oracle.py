@@ -246,6 +246,7 @@ class OrFr2d(C.Structure):
         ("geom", C.c_int),
         ("n1", C.c_double * 2),
         ("n2", C.c_double * 2),
+        ("iJp", _dp),
     ]
 
 
@@ -255,10 +256,13 @@ class Fr2d:
     Rectangular cells (example/shock-vortex.jl:26-118) by default.  With `iJ` (the uniform 2x2
     inverse Jacobian [[r_x, r_y], [s_x, s_y]] of parallelogram cells) and the face normals n1, n2,
     the interface terms of dev/parallelogram.jl:116-162 (fluxes solved in the face-normal frames,
-    scaled by (iJ n1)_1 and (iJ n2)_2)."""
+    scaled by (iJ n1)_1 and (iJ n2)_2).  With `iJp`, the inverse Jacobian of every solution point,
+    shape (nx+2, ny+2, nsp, nsp, 2, 2) (src/struct.jl:134-142 for any quad mesh): the residual of
+    example/shock-vortex.jl:26-118 with the metric of each point (volume flux iJ[k, l] [F, G],
+    face fluxes of point (k, l) scaled by its own iJ11 / iJ22); dx, dy are then unused."""
 
     def __init__(self, nx, ny, deg, dx, dy, gamma=1.4, flux="hll", bc="ghost", ops: Ops | None = None,
-                 iJ=None, n1=None, n2=None):
+                 iJ=None, n1=None, n2=None, iJp=None):
         self.ops = ops or Ops(deg)
         self.nx, self.ny, self.nsp = nx, ny, deg + 1
         self._keep = [np.ascontiguousarray(a, dtype=np.float64) for a in
@@ -284,6 +288,14 @@ class Fr2d:
             d.n1[0], d.n1[1] = float(n1[0]), float(n1[1])
             d.n2[0], d.n2[1] = float(n2[0]), float(n2[1])
         d.ll, d.lr, d.dl, d.dhl, d.dhr = [_p(a) for a in self._keep]
+        if iJp is not None:
+            if d.geom != 0:
+                raise ValueError("point metrics take the rectangle face terms (no iJ / n1 / n2)")
+            iJp = np.asfortranarray(iJp, dtype=np.float64)
+            if iJp.shape != (nx + 2, ny + 2, deg + 1, deg + 1, 2, 2):
+                raise ValueError("iJp must have shape (nx+2, ny+2, nsp, nsp, 2, 2)")
+            self._iJp = iJp
+            d.iJp = _p(iJp)
         self.d = d
         self.shape = (nx + 2, ny + 2, deg + 1, deg + 1, 4)
 

@@ -111,3 +111,45 @@ def fr2d_rhs(u, ops, g, iJ11, iJ22, flux="hll", periodic=False):
     du = np.zeros_like(u)
     du[I, J] = -(d1 + d2 + corr)
     return np.asfortranarray(du)
+
+
+def fr2d_rhs_metric(u, ops, g, iJ, flux="hll", periodic=False):
+    """The same residual with a 2x2 inverse Jacobian per solution point, iJ of shape
+    (nx+2, ny+2, n, n, 2, 2) (example/shock-vortex.jl:48-55 and :102-115 read iJ[i, j][k, l])."""
+    u = np.array(u, dtype=np.float64)
+    nx, ny = u.shape[0] - 2, u.shape[1] - 2
+    if periodic:
+        u[0, 1:-1] = u[nx, 1:-1]
+        u[nx + 1, 1:-1] = u[1, 1:-1]
+        u[:, 0] = u[:, ny]
+        u[:, ny + 1] = u[:, 1]
+    ll, lr, dl, dhl, dhr = ops["ll"], ops["lr"], ops["dl"], ops["dhl"], ops["dhr"]
+    F, G = _fluxes(u, g)
+    a = iJ[..., None, :, :]  # (..., k, l, 1, 2, 2) against the variables
+    f1 = a[..., 0, 0] * F + a[..., 0, 1] * G
+    f2 = a[..., 1, 0] * F + a[..., 1, 1] * G
+    uL = np.einsum("ijklm,k->ijlm", u, ll)
+    uR = np.einsum("ijklm,k->ijlm", u, lr)
+    uB = np.einsum("ijklm,l->ijkm", u, ll)
+    uT = np.einsum("ijklm,l->ijkm", u, lr)
+    f1L = np.einsum("ijklm,k->ijlm", f1, ll)
+    f1R = np.einsum("ijklm,k->ijlm", f1, lr)
+    f2B = np.einsum("ijklm,l->ijkm", f2, ll)
+    f2T = np.einsum("ijklm,l->ijkm", f2, lr)
+    Fx = riemann_x(flux, uR[0:nx + 1, 1:ny + 1], uL[1:nx + 2, 1:ny + 1], g)
+    Gy = riemann_y(flux, uT[1:nx + 1, 0:ny + 1], uB[1:nx + 1, 1:ny + 2], g)
+    I = slice(1, nx + 1)
+    J = slice(1, ny + 1)
+    d1 = np.einsum("kq,ijqlm->ijklm", dl, f1[I, J])
+    d2 = np.einsum("lq,ijkqm->ijklm", dl, f2[I, J])
+    j11 = iJ[I, J, :, :, 0, 0][..., None]  # (nx, ny, k, l, 1)
+    j22 = iJ[I, J, :, :, 1, 1][..., None]
+    cl = Fx[0:nx][:, :, None] * j11 - f1L[I, J][:, :, None]  # (nx, ny, k, l, m)
+    cr = Fx[1:nx + 1][:, :, None] * j11 - f1R[I, J][:, :, None]
+    cb = Gy[:, 0:ny][:, :, :, None] * j22 - f2B[I, J][:, :, :, None]
+    ct = Gy[:, 1:ny + 1][:, :, :, None] * j22 - f2T[I, J][:, :, :, None]
+    corr = dhl[None, None, :, None, None] * cl + dhr[None, None, :, None, None] * cr \
+        + dhl[None, None, None, :, None] * cb + dhr[None, None, None, :, None] * ct
+    du = np.zeros_like(u)
+    du[I, J] = -(d1 + d2 + corr)
+    return np.asfortranarray(du)
