@@ -87,12 +87,17 @@ def test_padded_device_state(nx, ny):
     assert torch.equal(back, dev)
     pitch, plane = s.state_pitch()
     assert pitch % 16 == 0 and nx + 2 <= pitch < nx + 2 + 16 and plane == pitch * (ny + 2)
-    # the padded array itself, copied out with the process's HIP runtime (device to device)
+    # the padded array itself, read on the device through the library's row contraction with
+    # the 1 x 1 identity (a copy: x * 1.0 is exact) from the raw device pointer of the state
+    from frhip import ops
+
     n = plane * 9 * 4
-    padded = torch.empty(n, dtype=torch.float64, device="cuda")
-    hip = C.CDLL("libamdhip64.so")
-    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-    assert hip.hipMemcpy(C.c_void_p(padded.data_ptr()), C.c_void_p(s.state_ptr()), n * 8, 3) == 0
+    padded = torch.full((n,), float("nan"), dtype=torch.float64, device="cuda")
+    one = np.ones((1, 1))
+    torch.cuda.synchronize()
+    _lib.check(ops._bind().fr_poly_derivative_dev(C.c_void_p(s.state_ptr()), n, 1, 1, _lib.dptr(one),
+                                                  C.c_void_p(padded.data_ptr()), None))
+    torch.cuda.synchronize()
     a = padded.cpu().numpy().reshape((pitch, ny + 2, 3, 3, 4), order="F")[: nx + 2]
     assert np.array_equal(a, u)
 
