@@ -180,7 +180,7 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
   FR_REQUIRE(out && d, FR_E_INVALID, "fr2d_create: null argument");
   *out = nullptr;
   FR_REQUIRE(d->nx >= 1 && d->ny >= 1, FR_E_INVALID, "fr2d_create: nx, ny must be >= 1");
-  FR_REQUIRE(d->deg >= 1 && d->deg <= 4, FR_E_UNSUPPORTED, "fr2d_create: degree must be in 1..4");
+  FR_REQUIRE(d->deg >= 1 && d->deg <= 5, FR_E_UNSUPPORTED, "fr2d_create: degree must be in 1..5");
   FR_REQUIRE(d->flux >= FR_FLUX_HLL && d->flux <= FR_FLUX_ROE, FR_E_INVALID, "fr2d_create: unknown flux");
   FR_REQUIRE(d->bc == FR_BC_GHOST || d->bc == FR_BC_PERIODIC, FR_E_INVALID, "fr2d_create: unknown bc");
   FR_REQUIRE(d->gamma > 1.0 && std::isfinite(d->gamma), FR_E_INVALID, "fr2d_create: gamma must be > 1");
@@ -240,7 +240,7 @@ int fr2d_create(fr2d_ctx **out, const fr2d_desc *d) {
   const size_t nd = 2 * (size_t)np * np + np + 2 * n;
   if (e == hipSuccess) e = hipMalloc(&c->d_dops, nd * sizeof(double));
   if (e == hipSuccess) {
-    double host[2 * 625 + 25 + 10];
+    double host[2 * kMaxNp2D * kMaxNp2D + kMaxNp2D + 12];
     std::memcpy(host, c->ops.V, sizeof(double) * np * np);
     std::memcpy(host + np * np, c->ops.iV, sizeof(double) * np * np);
     for (int q = 0; q < np; ++q) host[2 * np * np + q] = c->ops.wp[q] / 4.0;

@@ -413,6 +413,7 @@ static int launch_quad(const Fr2dCtx &c, double *d_u, int mode, double lambda, d
     case 3: return quad_nsp<3>(c, d_u, mode, lambda, S0, kappa, rule, limit, nfilt, nfail, st);
     case 4: return quad_nsp<4>(c, d_u, mode, lambda, S0, kappa, rule, limit, nfilt, nfail, st);
     case 5: return quad_nsp<5>(c, d_u, mode, lambda, S0, kappa, rule, limit, nfilt, nfail, st);
+    case 6: return quad_nsp<6>(c, d_u, mode, lambda, S0, kappa, rule, limit, nfilt, nfail, st);
     default: return fail(FR_E_UNSUPPORTED, "dissipation: degree out of range");
   }
 }
@@ -438,6 +439,7 @@ int fr2d_launch_sensor_filter(const Fr2dCtx &c, double *d_u, int mode, double la
     case 3: return sensor_eval_nsp<3>(c, d_u, mode, lambda, S0, kappa, eval_out, st);
     case 4: return sensor_eval_nsp<4>(c, d_u, mode, lambda, S0, kappa, eval_out, st);
     case 5: return sensor_eval_nsp<5>(c, d_u, mode, lambda, S0, kappa, eval_out, st);
+    case 6: return sensor_eval_nsp<6>(c, d_u, mode, lambda, S0, kappa, eval_out, st);
     default: return fail(FR_E_UNSUPPORTED, "sensor: degree out of range");
   }
 }
@@ -466,6 +468,7 @@ int fr2d_launch_check(const Fr2dCtx &c, const double *d_u, int *d_count, hipStre
     case 3: return check_nsp<3>(c, d_u, d_count, st);
     case 4: return check_nsp<4>(c, d_u, d_count, st);
     case 5: return check_nsp<5>(c, d_u, d_count, st);
+    case 6: return check_nsp<6>(c, d_u, d_count, st);
     default: return fail(FR_E_UNSUPPORTED, "check: degree out of range");
   }
 }

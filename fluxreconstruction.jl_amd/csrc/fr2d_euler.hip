@@ -725,6 +725,7 @@ static int launch(const Fr2dCtx &c, const StageArgs &sa, bool residual, hipStrea
     case 3: return launch_flux<3>(c, sa, residual, st);
     case 4: return launch_flux<4>(c, sa, residual, st);
     case 5: return launch_flux<5>(c, sa, residual, st);
+    case 6: return launch_flux<6>(c, sa, residual, st);
     default: return fail(FR_E_UNSUPPORTED, "fr2d: degree out of range");
   }
 }

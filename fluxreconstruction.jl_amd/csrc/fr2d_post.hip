@@ -51,6 +51,7 @@ int fr2d_launch_primitives(const Fr2dCtx &c, const double *d_u, double *d_sol, h
     case 3: return launch_prim<3>(c, d_u, d_sol, st);
     case 4: return launch_prim<4>(c, d_u, d_sol, st);
     case 5: return launch_prim<5>(c, d_u, d_sol, st);
+    case 6: return launch_prim<6>(c, d_u, d_sol, st);
     default: return fail(FR_E_UNSUPPORTED, "fr2d_primitives: degree out of range");
   }
 }

@@ -43,11 +43,12 @@ bool invert(int n, const double *A, double *Ainv);
 
 // Operators of one degree: Gauss points, ll/lr, dl(m,k) column-major, correction derivatives,
 // tensor weights wp(k,l) and the 1D Vandermonde pair for degrees 1..8; the quad Vandermonde
-// (nsp^2 x nsp^2) and its inverse for degrees 1..4 only (the 2D spaces; zero above).
+// (nsp^2 x nsp^2) and its inverse for degrees 1..5 only (the 2D spaces; zero above).
+constexpr int kMaxNp2D = 36;  // nsp^2 of the highest 2D degree (5)
 struct HostOps {
   int deg = 0, nsp = 0;
   double r[16], w[16], ll[16], lr[16], dl[256], dhl[16], dhr[16];
-  double V[625], iV[625], wp[256];
+  double V[kMaxNp2D * kMaxNp2D], iV[kMaxNp2D * kMaxNp2D], wp[256];
   double V1[256], iV1[256];
 };
 bool build_ops(int deg, int correction, HostOps &o);

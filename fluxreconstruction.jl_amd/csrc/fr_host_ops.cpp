@@ -210,7 +210,7 @@ bool build_ops(int deg, int correction, HostOps &o) {
     for (int k = 0; k < n; ++k) o.wp[k + n * l] = o.w[k] * o.w[l];
   // nodes of the quad: node q = k + n*l sits at (r_k, r_l); mode sk = a*n + b is P_a(r) P_b(s)
   const int np = n * n;
-  if (n > 5) {  // the 2D spaces stop at degree 4
+  if (np > kMaxNp2D) {  // the 2D spaces stop at degree 5
     std::memset(o.V, 0, sizeof(o.V));
     std::memset(o.iV, 0, sizeof(o.iV));
     return true;

@@ -111,12 +111,12 @@ int fr_interp_face_dev(const double *d_f, long long nrow, int n, int nbatch, con
 int fr_poly_derivative_dev(const double *d_f, long long nrow, int n, int nbatch, const double *pdm, double *d_df,
                            void *stream);
 
-/* ------------------------------------------------------------------ 2D Euler on uniform quads */
+/* ------------------------------------------------------------------ 2D Euler on quads */
 typedef struct fr2d_ctx fr2d_ctx;
 
 typedef struct {
   int nx, ny;       /* interior cells; the state carries one ghost layer (ngx = ngy = 1) */
-  int deg;          /* polynomial degree p, 1 <= p <= 4 */
+  int deg;          /* polynomial degree p, 1 <= p <= 5 */
   double gamma;     /* ratio of specific heats */
   int flux;         /* FR_FLUX_* */
   int bc;           /* FR_BC_* */

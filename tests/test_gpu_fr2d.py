@@ -35,7 +35,7 @@ def setup(nx, ny, deg, flux, bc, L=10.0, seed=11, amp=0.01):
     return ps, u, solver, orc
 
 
-@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("flux", ["hll", "rusanov", "roe"])
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_residual_parity(deg, flux, bc):
@@ -283,7 +283,7 @@ def test_state_alternation(scheme, bc):
         assert np.array_equal(states[0][0], u[0]) and np.array_equal(states[0][:, -1], u[:, -1])
 
 
-@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 def test_sensor_filter_parity(deg):
     ps = frhip.FRPSpace2D(0.0, 2.0, 40, 0.0, 1.0, 20, deg, 1, 1)
     u = ics.shock_vortex(ps, 1.4)
@@ -298,7 +298,7 @@ def test_sensor_filter_parity(deg):
     assert rel_inf(s.get_state(), uc) <= 1e-13
 
 
-@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 def test_positivity_parity(deg):
     ps = frhip.FRPSpace2D(0.0, 1.0, 12, 0.0, 1.0, 10, deg, 1, 1)
     u = ics.isentropic_vortex(ps, 1.4, strength=5.0, center=(0.5, 0.5))
@@ -400,7 +400,7 @@ def test_fill_ghost_rules(rule):
     assert np.array_equal(s.get_state(), _ghost_rule_numpy(u, rule))
 
 
-@pytest.mark.parametrize("deg", [1, 3, 4])
+@pytest.mark.parametrize("deg", [1, 3, 4, 5])
 def test_primitives_parity(deg):
     """Post-processing of example/shock-vortex.jl:333-352: coordinates and (rho, U, V, 1/lambda)
     of every solution point against the oracle's loop restatement (host copy and device output)."""
@@ -418,6 +418,27 @@ def test_primitives_parity(deg):
     s.primitives(out=d)
     torch.cuda.synchronize()
     assert np.array_equal(d.cpu().numpy().reshape(sol.shape, order="F"), sol)
+
+
+@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint"])
+@pytest.mark.parametrize("bc", ["ghost", "periodic"])
+def test_degree5_stepping(scheme, bc):
+    """Degree 5 (nsp = 6, one workgroup of six wavefronts per CU): steps against the oracle, graph
+    replay bitwise equal to eager; the space of test/runtests.jl:16-20 (FRPSpace2D(0, 1, 20, 0, 1,
+    20, 5, 1, 1)) builds a solver."""
+    ps, u, s, orc = setup(21, 9, 5, "hll", bc, seed=21)
+    dt, _ = ics.cfl_steps(u, 1.4, ps.dx0, 0.05, 1.0)
+    s.set_state(u)
+    s.step(dt, 3, scheme)
+    got = s.get_state()
+    uc = u.copy(order="F")
+    getattr(orc, scheme)(uc, dt, 3)
+    assert rel_l2(got, uc) <= STATE_TOL
+    s.set_state(u)
+    s.step_graph(dt, 3, 1, scheme)
+    assert np.array_equal(s.get_state(), got)
+    rt = frhip.FRPSpace2D(0.0, 1.0, 20, 0.0, 1.0, 20, 5, 1, 1)
+    frhip.FREuler2D(rt, 1.4, "hll", "ghost").close()
 
 
 # ----------------------------------------------------------------- parallelogram cells

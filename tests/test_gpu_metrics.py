@@ -25,7 +25,7 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
 
 
-@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("flux", ["hll", "rusanov", "roe"])
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_polar_residual_parity(deg, flux, bc):

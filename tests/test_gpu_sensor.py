@@ -137,7 +137,7 @@ def test_fused_filter_refresh_limit_equals_host_loop(mode, bc, iters):
     assert np.array_equal(out[0], out[2])  # fr2d_run_host_loop
 
 
-@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("mode", ["off", "modal"])
 @pytest.mark.parametrize("rule", ["periodic", "outflow", "inflow_left", "wave"])
 def test_fused_pass_with_active_limiter(deg, mode, rule):

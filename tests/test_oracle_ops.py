@@ -85,7 +85,7 @@ def test_sd_huynh_closed_form(kind):
     assert np.allclose(dgl, (-1) ** deg * 0.5 * (dP(deg) - y), atol=1e-12)
 
 
-@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 def test_vandermonde_orthonormal_and_reference_identity(deg):
     ops = O.Ops(deg)
     n = deg + 1
@@ -148,7 +148,7 @@ def test_modal_filter_l2_coefficients():
 
 
 # ---------------------------------------------------------------- product host operators
-@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 def test_library_host_operators_match_oracle(deg):
     ops = O.Ops(deg)
     ps = frhip.FRPSpace2D(0.0, 1.0, 4, 0.0, 2.0, 3, deg, 1, 1)

@@ -54,7 +54,7 @@ def test_prim_conserve_roundtrip():
 
 
 # ---------------------------------------------------------------- 2D residual
-@pytest.mark.parametrize("deg", [1, 2, 3, 4])
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("flux", ["hll", "rusanov"])
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_residual_vs_numpy_restatement(deg, flux, bc):
