@@ -1,0 +1,363 @@
+// fr_bgk1d.hip -- the 1D discrete-velocity BGK model of example/bgk_wave.jl:69-129 (mol!) as the
+// script writes it, and batched ensembles of such problems that differ in the relaxation time,
+// for gfx950.
+//
+// State u(i, j, k, member): nx periodic cells, nu velocities, nsp solution points, column-major,
+// the cell fastest, so a wavefront of consecutive cells reads contiguous memory for a fixed
+// (velocity, point).  Two kernels per stage:
+// - moments: one thread per (member, cell, point), the sums over the velocities of w f, w u f and
+//   w u^2 f / 2 (KitBase moments_conserve), then the primitive variables at gamma (conserve_prim:
+//   rho, U, lambda) of the point;
+// - transport + relaxation + RK stage: one thread per (member, cell, velocity).  f = u_j u / J, the
+//   traces f . ll and f . lr, the upwind interface value (the Heaviside blend of the script picks
+//   the left cell's right trace for u_j > 0, the right cell's left trace otherwise; the other term
+//   of the blend is the other trace times zero), the derivative, the Radau corrections and
+//   (M - u) / tau with M = rho sqrt(lambda / pi) exp(-lambda (u_j - U)^2) of the point.
+#include <algorithm>
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "fr_common.h"
+
+namespace fr {
+
+constexpr int kBgk1dMaxNsp = 9;  // degrees 1..8, as the 1D Euler kernels
+
+struct Bgk1dGeo {
+  int nx, nu, nsp, nmem;
+  long long npts;     // nx * nsp: points of one member
+  long long mstride;  // nx * nu * nsp: doubles per member
+  double iJ, gamma;   // 2 / dx; gamma of conserve_prim
+};
+
+struct Bgk1dOps {
+  double ll[kBgk1dMaxNsp], lr[kBgk1dMaxNsp], dl[kBgk1dMaxNsp * kBgk1dMaxNsp], dhl[kBgk1dMaxNsp],
+      dhr[kBgk1dMaxNsp];
+};
+
+struct Bgk1dStage {
+  const double *u, *u0;
+  double *out;
+  double a0, a1, b;
+  int residual;
+};
+
+// prim[member][c][p], p = i + nx k: rho, U, lambda of every point
+__global__ __launch_bounds__(256) void bgk1d_moments_kernel(Bgk1dGeo g, const double *__restrict__ f,
+                                                            const double *__restrict__ uv,
+                                                            const double *__restrict__ wv, double *prim) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (t >= g.npts * g.nmem) return;
+  const long long mem = t / g.npts, p = t % g.npts;
+  const long long i = p % g.nx, k = p / g.nx;
+  const double *fb = f + mem * g.mstride + i + (long long)g.nx * g.nu * k;
+  double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+  for (int j = 0; j < g.nu; ++j) {
+    const double fv = fb[(long long)g.nx * j];
+    const double wj = wv[j], uj = uv[j];
+    m0 += wj * fv;
+    m1 += wj * uj * fv;
+    m2 += wj * uj * uj * fv;
+  }
+  const double w2 = 0.5 * m2;
+  double *pr = prim + mem * 3 * g.npts + p;
+  pr[0] = m0;
+  pr[g.npts] = m1 / m0;
+  pr[2 * g.npts] = 0.5 * m0 / (g.gamma - 1.0) / (w2 - 0.5 * m1 * m1 / m0);
+}
+
+template <int NSP>
+__global__ __launch_bounds__(256) void bgk1d_rhs_kernel(Bgk1dGeo g, Bgk1dOps ops, const double *__restrict__ uv,
+                                                        const double *__restrict__ tau,
+                                                        const double *__restrict__ prim, Bgk1dStage sa) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const long long per = (long long)g.nx * g.nu;
+  if (t >= per * g.nmem) return;
+  const long long mem = t / per, r = t % per;
+  const int i = (int)(r % g.nx), j = (int)(r / g.nx);
+  const double vel = uv[j];
+  const bool pos = vel > 0.0;  // heaviside(u_j)
+  const long long base = mem * g.mstride + (long long)i + (long long)g.nx * j;
+  const long long kst = (long long)g.nx * g.nu;  // stride of the point index
+  const double *__restrict__ u = sa.u;
+  double w[NSP], f[NSP];
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) {
+    w[k] = u[base + kst * k];
+    f[k] = vel * w[k] * g.iJ;
+  }
+  double tl = 0.0, tr = 0.0;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) {
+    tl += f[k] * ops.ll[k];
+    tr += f[k] * ops.lr[k];
+  }
+  // the upwind neighbour's trace: the left cell's right trace (u_j > 0) or the right cell's left
+  // trace, periodic
+  const int in = pos ? (i == 0 ? g.nx - 1 : i - 1) : (i == g.nx - 1 ? 0 : i + 1);
+  const long long nbase = base + (long long)(in - i);
+  double nb = 0.0;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) nb += vel * u[nbase + kst * k] * g.iJ * (pos ? ops.lr[k] : ops.ll[k]);
+  const double iL = pos ? nb : tl, iR = pos ? tr : nb;
+  const double tv = tau[mem];
+  const double *pr = prim + mem * 3 * g.npts + i;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) {
+    double r1 = 0.0;
+#pragma unroll
+    for (int q = 0; q < NSP; ++q) r1 += f[q] * ops.dl[k + NSP * q];
+    const double rho = pr[(long long)g.nx * k], U = pr[g.npts + (long long)g.nx * k],
+                 lam = pr[2 * g.npts + (long long)g.nx * k];
+    const double c = vel - U;
+    const double M = rho * sqrt(lam / M_PI) * exp(-lam * (c * c));
+    const double res = -(r1 + (iL - tl) * ops.dhl[k] + (iR - tr) * ops.dhr[k]) + (M - w[k]) / tv;
+    const long long o = base + kst * k;
+    if (sa.residual) {
+      sa.out[o] = res;
+    } else {
+      double v = sa.a1 * w[k];
+      if (sa.a0 != 0.0) v = sa.a0 * sa.u0[o] + v;
+      sa.out[o] = v + sa.b * res;
+    }
+  }
+}
+
+struct Bgk1dCtx {
+  int nx = 0, nsp = 0, nu = 0, nmem = 1;
+  double dx = 1.0, gamma = 3.0;
+  HostOps ops;
+  size_t len = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev = nullptr;  // orders work on a caller stream against the context's stream
+  double *d_vel = nullptr;  // u | w
+  double *d_tau = nullptr, *d_prim = nullptr;
+  double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_in = nullptr, *d_out = nullptr;
+
+  Bgk1dGeo geo() const {
+    Bgk1dGeo g;
+    g.nx = nx;
+    g.nu = nu;
+    g.nsp = nsp;
+    g.nmem = nmem;
+    g.npts = (long long)nx * nsp;
+    g.mstride = g.npts * nu;
+    g.iJ = 2.0 / dx;
+    g.gamma = gamma;
+    return g;
+  }
+
+  template <int NSP>
+  int launch_rhs(const Bgk1dStage &sa, hipStream_t st) const {
+    Bgk1dOps o{};
+    for (int q = 0; q < NSP; ++q) {
+      o.ll[q] = ops.ll[q];
+      o.lr[q] = ops.lr[q];
+      o.dhl[q] = ops.dhl[q];
+      o.dhr[q] = ops.dhr[q];
+    }
+    for (int q = 0; q < NSP * NSP; ++q) o.dl[q] = ops.dl[q];
+    const Bgk1dGeo g = geo();
+    const long long n = (long long)nx * nu * nmem;
+    hipLaunchKernelGGL(bgk1d_rhs_kernel<NSP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, o, d_vel,
+                       d_tau, d_prim, sa);
+    FR_LAUNCHED("bgk1d_rhs_kernel");
+    return FR_OK;
+  }
+
+  int launch_moments(const double *d_f, hipStream_t st) const {
+    const Bgk1dGeo g = geo();
+    const long long n = g.npts * nmem;
+    hipLaunchKernelGGL(bgk1d_moments_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, d_f, d_vel,
+                       d_vel + nu, d_prim);
+    FR_LAUNCHED("bgk1d_moments_kernel");
+    return FR_OK;
+  }
+
+  int stage(const Bgk1dStage &sa, hipStream_t st) const {
+    int rc = launch_moments(sa.u, st);
+    if (rc) return rc;
+    switch (nsp) {
+      case 2: return launch_rhs<2>(sa, st);
+      case 3: return launch_rhs<3>(sa, st);
+      case 4: return launch_rhs<4>(sa, st);
+      case 5: return launch_rhs<5>(sa, st);
+      case 6: return launch_rhs<6>(sa, st);
+      case 7: return launch_rhs<7>(sa, st);
+      case 8: return launch_rhs<8>(sa, st);
+      default: return launch_rhs<9>(sa, st);
+    }
+  }
+
+  void release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    double *bufs[] = {d_vel, d_tau, d_prim, d_u, d_s1, d_s2, d_in, d_out};
+    for (double *b : bufs)
+      if (b) (void)hipFree(b);
+    if (ev) (void)hipEventDestroy(ev);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+}  // namespace fr
+
+using namespace fr;
+
+struct fr_bgk1d_ctx : Bgk1dCtx {};
+
+extern "C" {
+
+int fr_bgk1d_create(fr_bgk1d_ctx **out, const fr_bgk1d_desc *d) {
+  FR_REQUIRE(out && d, FR_E_INVALID, "fr_bgk1d_create: null argument");
+  *out = nullptr;
+  FR_REQUIRE(d->nx >= 1 && d->nu >= 1 && d->nmem >= 1, FR_E_INVALID, "fr_bgk1d_create: sizes must be >= 1");
+  FR_REQUIRE(d->deg >= 1 && d->deg <= kBgk1dMaxNsp - 1, FR_E_UNSUPPORTED, "fr_bgk1d_create: degree must be in 1..8");
+  FR_REQUIRE(d->dx > 0.0 && std::isfinite(d->dx) && d->gamma > 1.0, FR_E_INVALID,
+             "fr_bgk1d_create: dx > 0, gamma > 1");
+  FR_REQUIRE(d->u && d->w && d->tau, FR_E_INVALID, "fr_bgk1d_create: null velocity grid or tau");
+  for (int m = 0; m < d->nmem; ++m) FR_REQUIRE(d->tau[m] > 0.0, FR_E_INVALID, "fr_bgk1d_create: tau must be > 0");
+  FR_REQUIRE((double)d->nx * d->nu * d->nmem < 2147483647.0 * 256.0, FR_E_INVALID, "fr_bgk1d_create: too large");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(FR_E_HIP, "fr_bgk1d_create: no HIP device visible");
+  fr_bgk1d_ctx *c = new (std::nothrow) fr_bgk1d_ctx();
+  FR_REQUIRE(c, FR_E_NOMEM, "fr_bgk1d_create: out of host memory");
+  c->nx = d->nx;
+  c->nsp = d->deg + 1;
+  c->nu = d->nu;
+  c->nmem = d->nmem;
+  c->dx = d->dx;
+  c->gamma = d->gamma;
+  if (!build_ops(d->deg, FR_CORR_RADAU, c->ops)) {
+    delete c;
+    return fail(FR_E_INVALID, "fr_bgk1d_create: operator construction failed");
+  }
+  const Bgk1dGeo g = c->geo();
+  c->len = (size_t)g.mstride * d->nmem;
+  std::vector<double> vel(d->u, d->u + d->nu);
+  vel.insert(vel.end(), d->w, d->w + d->nu);
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&c->ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipMalloc(&c->d_vel, vel.size() * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(c->d_vel, vel.data(), vel.size() * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&c->d_tau, d->nmem * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(c->d_tau, d->tau, d->nmem * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&c->d_prim, (size_t)g.npts * 3 * d->nmem * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->d_u, c->len * sizeof(double));
+  if (e != hipSuccess) {
+    int rc = hip_fail(e, "fr_bgk1d_create", __FILE__, __LINE__);
+    c->release();
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return FR_OK;
+}
+
+int fr_bgk1d_destroy(fr_bgk1d_ctx *c) {
+  if (!c) return FR_OK;
+  c->release();
+  delete c;
+  return FR_OK;
+}
+
+size_t fr_bgk1d_state_len(const fr_bgk1d_ctx *c) { return c ? c->len : 0; }
+
+int fr_bgk1d_rhs(fr_bgk1d_ctx *c, const double *f, double *df) {
+  FR_REQUIRE(c && f && df, FR_E_INVALID, "fr_bgk1d_rhs: null argument");
+  const size_t bytes = c->len * sizeof(double);
+  if (!c->d_in) FR_HIP(hipMalloc(&c->d_in, bytes));
+  if (!c->d_out) FR_HIP(hipMalloc(&c->d_out, bytes));
+  FR_HIP(hipMemcpyAsync(c->d_in, f, bytes, hipMemcpyHostToDevice, c->stream));
+  int rc = c->stage(Bgk1dStage{c->d_in, nullptr, c->d_out, 0.0, 0.0, 0.0, 1}, c->stream);
+  if (rc) return rc;
+  FR_HIP(hipMemcpyAsync(df, c->d_out, bytes, hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr_bgk1d_rhs_dev(fr_bgk1d_ctx *c, const double *d_f, double *d_df, void *stream) {
+  FR_REQUIRE(c && d_f && d_df, FR_E_INVALID, "fr_bgk1d_rhs_dev: null argument");
+  FR_REQUIRE(d_f != d_df, FR_E_INVALID, "fr_bgk1d_rhs_dev: u and du must not alias");
+  // the moments buffer belongs to the context: the caller's stream waits for the context's
+  // queued work, and the context's later work waits for this call
+  hipStream_t st = (hipStream_t)stream;
+  FR_HIP(hipEventRecord(c->ev, c->stream));
+  FR_HIP(hipStreamWaitEvent(st, c->ev, 0));
+  int rc = c->stage(Bgk1dStage{d_f, nullptr, d_df, 0.0, 0.0, 0.0, 1}, st);
+  if (rc) return rc;
+  FR_HIP(hipEventRecord(c->ev, st));
+  FR_HIP(hipStreamWaitEvent(c->stream, c->ev, 0));
+  return FR_OK;
+}
+
+int fr_bgk1d_set_state(fr_bgk1d_ctx *c, const double *f) {
+  FR_REQUIRE(c && f, FR_E_INVALID, "fr_bgk1d_set_state: null argument");
+  FR_HIP(hipMemcpyAsync(c->d_u, f, c->len * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr_bgk1d_get_state(fr_bgk1d_ctx *c, double *f) {
+  FR_REQUIRE(c && f, FR_E_INVALID, "fr_bgk1d_get_state: null argument");
+  FR_HIP(hipMemcpyAsync(f, c->d_u, c->len * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+// nsteps explicit steps of every member: FR_RK_SSPRK3 (Shu-Osher), FR_RK_MIDPOINT (OrdinaryDiffEq
+// Midpoint, bgk_wave.jl:136) or FR_RK_EULER; each stage = moments + transport/relaxation.
+// Consecutive steps alternate the state between d_u and d_s1 (every stage writes every value), so
+// no stage writes an array whose neighbours it reads: the last stage of a step reads u^n only at
+// its own value.  The call's last step writes d_u (Euler: copied back when it ends in d_s1).
+int fr_bgk1d_step(fr_bgk1d_ctx *c, int scheme, double dt, int nsteps) {
+  FR_REQUIRE(c && nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr_bgk1d_step: bad arguments");
+  FR_REQUIRE(scheme == FR_RK_SSPRK3 || scheme == FR_RK_MIDPOINT || scheme == FR_RK_EULER, FR_E_UNSUPPORTED,
+             "fr_bgk1d_step: schemes are SSP-RK3, midpoint and Euler");
+  if (!c->d_s1) FR_HIP(hipMalloc(&c->d_s1, c->len * sizeof(double)));
+  if (!c->d_s2) FR_HIP(hipMalloc(&c->d_s2, c->len * sizeof(double)));
+  hipStream_t st = c->stream;
+  for (int n = 0; n < nsteps; ++n) {
+    int rc = FR_OK;
+    double *const X = (n & 1) ? c->d_s1 : c->d_u, *const Y = (n & 1) ? c->d_u : c->d_s1;
+    double *const out = n == nsteps - 1 ? c->d_u : Y;
+    if (scheme == FR_RK_SSPRK3) {
+      if ((rc = c->stage(Bgk1dStage{X, X, Y, 0.0, 1.0, dt, 0}, st))) return rc;
+      if ((rc = c->stage(Bgk1dStage{Y, X, c->d_s2, 0.75, 0.25, 0.25 * dt, 0}, st))) return rc;
+      if ((rc = c->stage(Bgk1dStage{c->d_s2, X, out, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, 0}, st))) return rc;
+    } else if (scheme == FR_RK_MIDPOINT) {
+      if ((rc = c->stage(Bgk1dStage{X, X, c->d_s2, 0.0, 1.0, 0.5 * dt, 0}, st))) return rc;
+      if ((rc = c->stage(Bgk1dStage{c->d_s2, X, out, 1.0, 0.0, dt, 0}, st))) return rc;
+    } else {
+      if ((rc = c->stage(Bgk1dStage{X, X, Y, 0.0, 1.0, dt, 0}, st))) return rc;
+      if (n == nsteps - 1 && Y != c->d_u)
+        FR_HIP(hipMemcpyAsync(c->d_u, Y, c->len * sizeof(double), hipMemcpyDeviceToDevice, st));
+    }
+  }
+  FR_HIP(hipStreamSynchronize(st));
+  return FR_OK;
+}
+
+// conserved moments (rho, rho U, E) of the device state at every point, host output
+// w[member][c][p], p = i + nx k
+int fr_bgk1d_moments(fr_bgk1d_ctx *c, double *w) {
+  FR_REQUIRE(c && w, FR_E_INVALID, "fr_bgk1d_moments: null argument");
+  const Bgk1dGeo g = c->geo();
+  int rc = c->launch_moments(c->d_u, c->stream);
+  if (rc) return rc;
+  std::vector<double> prim((size_t)g.npts * 3 * c->nmem);
+  FR_HIP(hipMemcpyAsync(prim.data(), c->d_prim, prim.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  FR_HIP(hipStreamSynchronize(c->stream));
+  for (long long mem = 0; mem < c->nmem; ++mem)
+    for (long long p = 0; p < g.npts; ++p) {
+      const double *pr = prim.data() + mem * 3 * g.npts + p;
+      const double rho = pr[0], U = pr[g.npts], lam = pr[2 * g.npts];
+      double *o = w + mem * 3 * g.npts + p;
+      o[0] = rho;
+      o[g.npts] = rho * U;
+      o[2 * g.npts] = 0.5 * rho / lam / (c->gamma - 1.0) + 0.5 * rho * U * U;
+    }
+  return FR_OK;
+}
+
+}  // extern "C"

@@ -36,7 +36,7 @@ from .ops import interp_face, poly_derivative, filter_exp  # noqa: F401
 from .euler2d import FREuler2D  # noqa: F401
 from .fr1d import FR1D, FREulerProblem, FRAdvectionProblem, Ensemble1D  # noqa: F401
 from . import ics  # noqa: F401
-from .bgk import BGK2D  # noqa: F401
+from .bgk import BGK2D, BGK1D, VSpace1D  # noqa: F401
 from . import sensor  # noqa: F401
 from . import io  # noqa: F401
 

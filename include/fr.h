@@ -354,6 +354,40 @@ int fr_ens_autotune(fr_ens_ctx *ctx, double dt, int *chosen);
 int fr_ens_sensor_filter(fr_ens_ctx *ctx, double dt, int *n_flagged);
 int fr_ens_positivity(fr_ens_ctx *ctx, int *n_limited);
 
+/* ------------------------------------------------------------------ 1D discrete-velocity BGK
+ * example/bgk_wave.jl:69-129 (mol!) as the script writes it: nx periodic cells of size dx, nu
+ * velocities u with quadrature weights w (KitBase VSpace1D u, weights), Radau FR of degree deg;
+ * per point the moments (moments_conserve), conserve_prim at gamma (the script: 3.0) and the
+ * Maxwellian rho sqrt(lambda / pi) exp(-lambda (u - U)^2); f = u_j u / J, the upwind interface value
+ * of the Heaviside blend (:103-107), derivative and corrections, plus (M - u) / tau (the script:
+ * tau = 1e-2).  State u(i, j, k) column-major (cell, velocity, point) per member, members
+ * concatenated (nmem > 1: a batched ensemble whose members differ in tau). */
+typedef struct fr_bgk1d_ctx fr_bgk1d_ctx;
+typedef struct {
+  int nx;            /* periodic cells */
+  int deg;           /* 1..8 */
+  int nu;            /* velocities */
+  int nmem;          /* ensemble members */
+  double dx;         /* cell size (J = dx/2) */
+  double gamma;      /* of conserve_prim (3 in bgk_wave.jl:79) */
+  const double *u, *w;  /* velocities and quadrature weights */
+  const double *tau;    /* nmem relaxation times */
+} fr_bgk1d_desc;
+int fr_bgk1d_create(fr_bgk1d_ctx **out, const fr_bgk1d_desc *desc);
+int fr_bgk1d_destroy(fr_bgk1d_ctx *ctx);
+size_t fr_bgk1d_state_len(const fr_bgk1d_ctx *ctx);
+/* mol!(du, u, p, t) of every member (bgk_wave.jl:69-129): host pointers, or device pointers on a
+ * caller stream (NULL: HIP's null stream) */
+int fr_bgk1d_rhs(fr_bgk1d_ctx *ctx, const double *f, double *df);
+int fr_bgk1d_rhs_dev(fr_bgk1d_ctx *ctx, const double *d_f, double *d_df, void *stream);
+int fr_bgk1d_set_state(fr_bgk1d_ctx *ctx, const double *f);
+int fr_bgk1d_get_state(fr_bgk1d_ctx *ctx, double *f);
+/* nsteps steps of the device state: FR_RK_SSPRK3, FR_RK_MIDPOINT (the script's Midpoint()) or
+ * FR_RK_EULER */
+int fr_bgk1d_step(fr_bgk1d_ctx *ctx, int scheme, double dt, int nsteps);
+/* (rho, rho U, E) of the device state at every point: w[member][c][i + nx k] */
+int fr_bgk1d_moments(fr_bgk1d_ctx *ctx, double *w);
+
 /* ------------------------------------------------------------------ 2D discrete-velocity BGK
  * example/bgk_wave.jl:69-129 tensorised to periodic uniform quads (Radau FR, upwind interface flux
  * per velocity, relaxation (M - f)/tau to the 2D Maxwellian of the local moments).  State

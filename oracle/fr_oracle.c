@@ -1421,6 +1421,97 @@ void or_bgk2d_rhs(const or_bgk2d *d, const double *f, double *df) {
 }
 
 /* =====================================================================
+ * 1D discrete-velocity BGK -- example/bgk_wave.jl:69-129 as the script writes it: per cell and
+ * point the moments over the velocities (KitBase moments_conserve: sum w f, sum w u f,
+ * 1/2 sum w u^2 f), the primitive variables at gamma (conserve_prim: rho, U,
+ * lambda = rho / (2 (gamma - 1) (E - rho U^2 / 2))) and the Maxwellian
+ * rho sqrt(lambda / pi) exp(-lambda (u - U)^2) (:76-81); f = u_j u / J (:84-88); the face traces
+ * f_face[., ., 1] = f . ll, f_face[., ., 2] = f . lr (interp_face!, :99-101); the interface value
+ * of face f between its left cell L and right cell R, f_face[R, 1] (1 - delta) + f_face[L, 2]
+ * delta with delta = heaviside(u_j) (:103-107, faces 1 and nface both join cell nx to cell 1);
+ * the derivative (:114-116) and the correction plus (M - u) / tau (:122-128).
+ * ===================================================================== */
+void or_bgk1d_moments(const or_bgk1d *d, const double *f, double *w) {
+  const size_t nx = d->nx, nu = d->nu, npts = nx * d->nsp;
+  for (size_t k = 0; k < (size_t)d->nsp; ++k)
+    for (size_t i = 0; i < nx; ++i) {
+      double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+      for (size_t j = 0; j < nu; ++j) {
+        const double fv = f[i + nx * (j + nu * k)];
+        m0 += d->wv[j] * fv;
+        m1 += d->wv[j] * d->uv[j] * fv;
+        m2 += d->wv[j] * d->uv[j] * d->uv[j] * fv;
+      }
+      w[i + nx * k] = m0;
+      w[i + nx * k + npts] = m1;
+      w[i + nx * k + 2 * npts] = 0.5 * m2;
+    }
+}
+
+void or_bgk1d_rhs(const or_bgk1d *d, const double *f, double *df) {
+  const int nx = d->nx, nu = d->nu, n = d->nsp;
+  const size_t npts = (size_t)nx * n, len = npts * nu;
+  double *w = (double *)malloc(sizeof(double) * npts * 3);
+  double *M = (double *)malloc(sizeof(double) * len);
+  double *fl = (double *)malloc(sizeof(double) * len);   /* f = u_j u / J */
+  double *face = (double *)malloc(sizeof(double) * (size_t)nx * nu * 2);
+  double *fint = (double *)malloc(sizeof(double) * (size_t)(nx + 1) * nu);
+#define A3(a, i, j, k) (a)[(size_t)(i) + (size_t)nx * ((size_t)(j) + (size_t)nu * (size_t)(k))]
+  or_bgk1d_moments(d, f, w);
+  for (int k = 0; k < n; ++k)
+    for (int i = 0; i < nx; ++i) {
+      const size_t p = (size_t)i + (size_t)nx * k;
+      const double rho = w[p], U = w[p + npts] / w[p];
+      const double lam = 0.5 * w[p] / (d->gamma - 1.0) / (w[p + 2 * npts] - 0.5 * w[p + npts] * w[p + npts] / w[p]);
+      for (int j = 0; j < nu; ++j) {
+        const double c = d->uv[j] - U;
+        A3(M, i, j, k) = rho * sqrt(lam / PI) * exp(-lam * (c * c));
+      }
+    }
+  const double J = 0.5 * d->dx;
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < nu; ++j)
+      for (int i = 0; i < nx; ++i) A3(fl, i, j, k) = d->uv[j] * A3(f, i, j, k) / J;
+  for (int j = 0; j < nu; ++j)
+    for (int i = 0; i < nx; ++i) {
+      double a = 0.0, b = 0.0;
+      for (int k = 0; k < n; ++k) {
+        a += A3(fl, i, j, k) * d->ll[k];
+        b += A3(fl, i, j, k) * d->lr[k];
+      }
+      face[(size_t)i + (size_t)nx * ((size_t)j + (size_t)nu * 0)] = a;
+      face[(size_t)i + (size_t)nx * ((size_t)j + (size_t)nu * 1)] = b;
+    }
+  /* faces 0..nx (0-based): right cell R = fi (fi < nx) or 0, left cell L = fi - 1 (fi > 0) or nx - 1 */
+  for (int fi = 0; fi <= nx; ++fi) {
+    const int R = fi < nx ? fi : 0, L = fi > 0 ? fi - 1 : nx - 1;
+    for (int j = 0; j < nu; ++j) {
+      const double delta = d->uv[j] > 0.0 ? 1.0 : 0.0;
+      fint[(size_t)fi + (size_t)(nx + 1) * j] = face[(size_t)R + (size_t)nx * (size_t)j] * (1.0 - delta) +
+                                                face[(size_t)L + (size_t)nx * ((size_t)j + (size_t)nu)] * delta;
+    }
+  }
+  for (int k = 0; k < n; ++k)
+    for (int j = 0; j < nu; ++j)
+      for (int i = 0; i < nx; ++i) {
+        double r1 = 0.0;
+        for (int q = 0; q < n; ++q) r1 += A3(fl, i, j, q) * d->dl[k + n * q];
+        /* left face of cell i: face i (cell 0: face nx); right face: i + 1 (cell nx - 1: face 0) */
+        const int fL = i > 0 ? i : nx, fR = i < nx - 1 ? i + 1 : 0;
+        const double iL = fint[(size_t)fL + (size_t)(nx + 1) * j], iR = fint[(size_t)fR + (size_t)(nx + 1) * j];
+        const double tl = face[(size_t)i + (size_t)nx * (size_t)j];
+        const double tr = face[(size_t)i + (size_t)nx * ((size_t)j + (size_t)nu)];
+        A3(df, i, j, k) = -(r1 + (iL - tl) * d->dhl[k] + (iR - tr) * d->dhr[k]) + (A3(M, i, j, k) - A3(f, i, j, k)) / d->tau;
+      }
+#undef A3
+  free(w);
+  free(M);
+  free(fl);
+  free(face);
+  free(fint);
+}
+
+/* =====================================================================
  * Metrics: src/tools.jl:43-53
  * ===================================================================== */
 double or_L1_error(const double *u, const double *ue, size_t n, double dx) {

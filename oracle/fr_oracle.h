@@ -139,6 +139,18 @@ size_t or_bgk2d_len(const or_bgk2d *d);
 void or_bgk2d_moments(const or_bgk2d *d, const double *f, double *w);
 void or_bgk2d_rhs(const or_bgk2d *d, const double *f, double *df);
 
+/* ---------------- 1D discrete-velocity BGK (example/bgk_wave.jl:69-129 as written) ----------
+ * u(i, j, k): periodic nx cells, nu velocities, nsp points; u[i + nx*(j + nu*k)]. */
+typedef struct {
+  int nx, nu, nsp;
+  double dx, tau, gamma;                  /* cell size (J = dx/2); relaxation time; gamma (3 there) */
+  const double *uv, *wv;                  /* velocities and weights (KitBase VSpace1D u, weights) */
+  const double *ll, *lr, *dl, *dhl, *dhr; /* standard_lagrange, dradau */
+} or_bgk1d;
+/* conserved moments (rho, rho U, E) at every point p = i + nx*k: w[p + nx*nsp*c] */
+void or_bgk1d_moments(const or_bgk1d *d, const double *f, double *w);
+void or_bgk1d_rhs(const or_bgk1d *d, const double *f, double *df);
+
 /* ---------------- metrics (src/tools.jl:43-53) ---------------- */
 double or_L1_error(const double *u, const double *ue, size_t n, double dx);
 double or_L2_error(const double *u, const double *ue, size_t n, double dx);

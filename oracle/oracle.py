@@ -476,3 +476,59 @@ def bgk2d_rhs(f, nx, ny, deg, u, wu, v, wv, tau, Lx=1.0, Ly=1.0, gamma=2.0):
     df = np.zeros_like(f)
     lib().or_bgk2d_rhs(C.byref(d), _p(f), _p(df))
     return df
+
+
+# ----------------------------------------------------------------- 1D BGK (bgk_wave.jl as written)
+class OrBgk1d(C.Structure):
+    _fields_ = [("nx", C.c_int), ("nu", C.c_int), ("nsp", C.c_int),
+                ("dx", C.c_double), ("tau", C.c_double), ("gamma", C.c_double),
+                ("uv", _dp), ("wv", _dp), ("ll", _dp), ("lr", _dp), ("dl", _dp), ("dhl", _dp), ("dhr", _dp)]
+
+
+class Bgk1d:
+    """Oracle of example/bgk_wave.jl:69-129 (mol!): state u(i, j, k) of nx periodic cells, the
+    velocities u / weights w of the velocity space and nsp = deg + 1 points, column-major; tau the
+    relaxation time (1e-2 in the script, inf for free transport), gamma = 3 as the script passes."""
+
+    def __init__(self, nx, deg, uv, wv, dx, tau=1e-2, gamma=3.0):
+        self.ops = Ops(deg)
+        self.nx, self.nu, self.nsp = nx, len(uv), deg + 1
+        self._keep = [_f64(a) for a in (uv, wv, self.ops.ll, self.ops.lr, self.ops.dl, self.ops.dhl, self.ops.dhr)]
+        d = OrBgk1d()
+        d.nx, d.nu, d.nsp = nx, len(uv), deg + 1
+        d.dx, d.tau, d.gamma = dx, tau, gamma
+        d.uv, d.wv, d.ll, d.lr, d.dl, d.dhl, d.dhr = [_p(a) for a in self._keep]
+        self.d = d
+        self.shape = (nx, len(uv), deg + 1)
+
+    def rhs(self, f):
+        f = np.asfortranarray(f, dtype=np.float64).reshape(self.shape, order="F")
+        df = np.zeros(self.shape, order="F")
+        lib().or_bgk1d_rhs(C.byref(self.d), _p(f), _p(df))
+        return df
+
+    def moments(self, f):
+        f = np.asfortranarray(f, dtype=np.float64).reshape(self.shape, order="F")
+        w = np.zeros((self.nx, self.nsp, 3), order="F")
+        lib().or_bgk1d_moments(C.byref(self.d), _p(f), _p(w))
+        return w
+
+    def stage(self, u0, us, a0, a1, b):
+        """out = a0 u0 + a1 us + b L(us), the stage convention of the device steppers."""
+        return a0 * u0 + a1 * us + b * self.rhs(us)
+
+    def step(self, f, dt, nsteps, scheme="midpoint"):
+        u = np.asfortranarray(f, dtype=np.float64).reshape(self.shape, order="F").copy(order="F")
+        for _ in range(nsteps):
+            if scheme == "midpoint":  # OrdinaryDiffEq Midpoint (bgk_wave.jl:136)
+                u1 = self.stage(u, u, 0.0, 1.0, 0.5 * dt)
+                u = self.stage(u, u1, 1.0, 0.0, dt)
+            elif scheme == "ssprk3":
+                u1 = self.stage(u, u, 0.0, 1.0, dt)
+                u2 = self.stage(u, u1, 0.75, 0.25, 0.25 * dt)
+                u = self.stage(u, u2, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt)
+            elif scheme == "euler":
+                u = self.stage(u, u, 0.0, 1.0, dt)
+            else:
+                raise ValueError(scheme)
+        return u

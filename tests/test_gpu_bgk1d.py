@@ -1,0 +1,124 @@
+"""GPU parity of the 1D discrete-velocity BGK model (fr_bgk1d_*, example/bgk_wave.jl:69-129 as the
+script writes it) against the CPU oracle on the same inputs.
+
+Tolerances: residual ||du_gpu - du_cpu||_inf / ||du_cpu||_inf <= 1e-12; states after stepping
+||u_gpu - u_cpu||_2 / ||u_cpu||_2 <= 1e-10 (FP64)."""
+import numpy as np
+import pytest
+
+import frhip
+from frhip import bgk
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RES_TOL = 1e-12
+STATE_TOL = 1e-10
+
+
+def rel_inf(a, b):
+    return float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel()))
+
+
+def script_case(nx=20, deg=2, nu=100):
+    ps = frhip.FRPSpace1D(0.0, 1.0, nx, deg)
+    vs = bgk.VSpace1D(-5.0, 5.0, nu)
+    return ps, vs, bgk.bgk_wave_initial(ps, vs)
+
+
+@pytest.mark.parametrize("deg", [1, 2, 3, 4, 5, 8])
+def test_residual_parity(deg):
+    ps, vs, f0 = script_case(nx=37, deg=deg, nu=64)
+    f = frhip.ics.perturbed(f0, seed=deg, amplitude=0.05)
+    g = bgk.BGK1D(ps, vs)
+    orc = O.Bgk1d(37, deg, vs.u, vs.weights, 1.0 / 37)
+    assert rel_inf(g.rhs(f), orc.rhs(f)) <= RES_TOL
+    g.close()
+
+
+def test_bgk_wave_to_t_end():
+    """example/bgk_wave.jl as written: 20 cells, degree 2, 100 velocities on [-5, 5], tau = 1e-2,
+    Midpoint with dt = cfl dx / u1 = 1e-3 to t = 1 (1000 steps), against the oracle's run."""
+    ps, vs, f0 = script_case()
+    dt = 0.1 * (1.0 / 20) / 5.0
+    g = bgk.BGK1D(ps, vs)
+    g.set_state(f0)
+    g.step(dt, 1000, "midpoint")
+    got = g.get_state()
+    orc = O.Bgk1d(20, 2, vs.u, vs.weights, 1.0 / 20)
+    ref = orc.step(f0, dt, 1000, "midpoint")
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got, ref) <= STATE_TOL
+    # the script's post-processing: density of every point against its own t = 0 profile
+    w = g.moments()[0]
+    wo = orc.moments(ref)
+    assert rel_inf(w[0], wo[..., 0]) <= 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("scheme", ["ssprk3", "euler", "midpoint"])
+def test_schemes(scheme):
+    ps, vs, f0 = script_case(nx=16, deg=3, nu=48)
+    dt = 2e-4
+    g = bgk.BGK1D(ps, vs)
+    g.set_state(f0)
+    g.step(dt, 7, scheme)
+    ref = O.Bgk1d(16, 3, vs.u, vs.weights, 1.0 / 16).step(f0, dt, 7, scheme)
+    assert rel_l2(g.get_state(), ref) <= STATE_TOL
+    g.close()
+
+
+def test_tau_ensemble_equals_single_problems():
+    ps, vs, f0 = script_case(nx=24, deg=2, nu=40)
+    taus = [1e-2, 3e-3, 1e-1]
+    ens = bgk.BGK1D(ps, vs, tau=taus)
+    ens.set_state(np.concatenate([f0.ravel(order="F")] * 3))
+    ens.step(5e-4, 6, "ssprk3")
+    got = ens.get_state()
+    for m, t in enumerate(taus):
+        one = bgk.BGK1D(ps, vs, tau=t)
+        one.set_state(f0)
+        one.step(5e-4, 6, "ssprk3")
+        assert np.array_equal(got[..., m], one.get_state())
+        one.close()
+    ens.close()
+
+
+def test_mol_on_device_arrays_side_stream():
+    import torch
+
+    ps, vs, f0 = script_case(nx=33, deg=2, nu=50)
+    f = frhip.ics.perturbed(f0, seed=3, amplitude=0.05)
+    g = bgk.BGK1D(ps, vs)
+    side = torch.cuda.Stream()
+    with torch.cuda.stream(side):
+        fd = torch.from_numpy(f.ravel(order="F").copy()).cuda()
+        dd = torch.full_like(fd, float("nan"))
+        g.mol(dd, fd)
+        out = dd.clone()
+    torch.cuda.synchronize()
+    ref = O.Bgk1d(33, 2, vs.u, vs.weights, 1.0 / 33).rhs(f)
+    assert rel_inf(out.cpu().numpy().reshape(f.shape, order="F"), ref) <= RES_TOL
+    # the context's own stream is ordered after that call: a step right after sees its state
+    g.set_state(f)
+    g.step(1e-4, 1, "euler")
+    assert rel_l2(g.get_state(), f + 1e-4 * ref) <= STATE_TOL
+    g.close()
+
+
+def test_create_rejects_bad_input():
+    ps, vs, _ = script_case()
+    with pytest.raises(frhip.FRError):
+        bgk.BGK1D(ps, vs, tau=0.0)
+    with pytest.raises(frhip.FRError):
+        bgk.BGK1D(ps, vs, gamma=1.0)
+    g = bgk.BGK1D(ps, vs)
+    with pytest.raises(frhip.FRError):
+        g.step(1e-3, 1, "rk4")
+    with pytest.raises(ValueError):
+        g.set_state(np.zeros(7))
+    g.close()
