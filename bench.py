@@ -103,6 +103,22 @@ def bench_bgk(steps=3, ens_steps=1):
             "single": workloads.bgk(steps), "ensemble64": workloads.bgk(ens_steps, members=64)}
 
 
+def bench_bgk1d():
+    """The 1D BGK model of example/bgk_wave.jl as written, 8192 cells x 256 velocities p = 2, one
+    member and 64 members (frhip.workloads)."""
+    from frhip import workloads
+
+    return {"workload": "1D BGK (bgk_wave.jl) 8192 cells x 256 velocities, p=2, SSP-RK3", "unit": "values·RK-stage/s",
+            "single": workloads.bgk1d(20), "ensemble64": workloads.bgk1d(4, members=64)}
+
+
+def bench_polar():
+    """The p = 3 stage on a 256 x 256 polar mesh with point-varying metrics (frhip.workloads)."""
+    from frhip import workloads
+
+    return workloads.polar_stage(256, 20)
+
+
 def bench_grids(sizes=(1024, 2048), steps=60):
     """The same vortex stage on larger periodic grids (frhip.workloads.vortex_grid)."""
     from frhip import workloads
@@ -289,7 +305,8 @@ def main():
         out["gpu_over_cpu"] = value / cpu["value"]
     if not args.no_extras:
         for key, fn in (("vortex_larger_grids", bench_grids), ("ensemble_riemann512", bench_ensemble),
-                        ("bgk_128sq_p2_v24", bench_bgk), ("sensor_shock_vortex", bench_sensor)):
+                        ("bgk_128sq_p2_v24", bench_bgk), ("bgk1d_8192_v256", bench_bgk1d),
+                        ("polar_256sq_p3", bench_polar), ("sensor_shock_vortex", bench_sensor)):
             try:
                 out[key] = fn()
             except Exception as exc:  # an extra leg must not hide the headline line

@@ -7,8 +7,9 @@
 // (velocity, point).  Two kernels per stage:
 // - moments: one thread per (member, cell, point), the sums over the velocities of w f, w u f and
 //   w u^2 f / 2 (KitBase moments_conserve), then the primitive variables at gamma (conserve_prim:
-//   rho, U, lambda) of the point;
-// - transport + relaxation + RK stage: one thread per (member, cell, velocity).  f = u_j u / J, the
+//   rho, U, lambda) of the point and the Maxwellian's prefactor rho sqrt(lambda / pi), so that the
+//   transport kernel spends one exp per value on M;
+// - transport + relaxation + RK stage: one thread per (member, cell, 8 velocities).  f = u_j u / J, the
 //   traces f . ll and f . lr, the upwind interface value (the Heaviside blend of the script picks
 //   the left cell's right trace for u_j > 0, the right cell's left trace otherwise; the other term
 //   of the blend is the other trace times zero), the derivative, the Radau corrections and
@@ -43,7 +44,7 @@ struct Bgk1dStage {
   int residual;
 };
 
-// prim[member][c][p], p = i + nx k: rho, U, lambda of every point
+// prim[member][c][p], p = i + nx k: rho, U, lambda and rho sqrt(lambda / pi) of every point
 __global__ __launch_bounds__(256) void bgk1d_moments_kernel(Bgk1dGeo g, const double *__restrict__ f,
                                                             const double *__restrict__ uv,
                                                             const double *__restrict__ wv, double *prim) {
@@ -61,65 +62,80 @@ __global__ __launch_bounds__(256) void bgk1d_moments_kernel(Bgk1dGeo g, const do
     m2 += wj * uj * uj * fv;
   }
   const double w2 = 0.5 * m2;
-  double *pr = prim + mem * 3 * g.npts + p;
+  double *pr = prim + mem * 4 * g.npts + p;
+  const double lam = 0.5 * m0 / (g.gamma - 1.0) / (w2 - 0.5 * m1 * m1 / m0);
   pr[0] = m0;
   pr[g.npts] = m1 / m0;
-  pr[2 * g.npts] = 0.5 * m0 / (g.gamma - 1.0) / (w2 - 0.5 * m1 * m1 / m0);
+  pr[2 * g.npts] = lam;
+  pr[3 * g.npts] = m0 * sqrt(lam / M_PI);
 }
+
+// A thread takes one cell and kVelPerThread consecutive velocities: the point data of the cell
+// (U, lambda, rho sqrt(lambda / pi)) is loaded once for all of them.
+constexpr int kVelPerThread = 8;
 
 template <int NSP>
 __global__ __launch_bounds__(256) void bgk1d_rhs_kernel(Bgk1dGeo g, Bgk1dOps ops, const double *__restrict__ uv,
                                                         const double *__restrict__ tau,
                                                         const double *__restrict__ prim, Bgk1dStage sa) {
   const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const long long per = (long long)g.nx * g.nu;
+  const int nvg = (g.nu + kVelPerThread - 1) / kVelPerThread;  // velocity groups
+  const long long per = (long long)g.nx * nvg;
   if (t >= per * g.nmem) return;
   const long long mem = t / per, r = t % per;
-  const int i = (int)(r % g.nx), j = (int)(r / g.nx);
-  const double vel = uv[j];
-  const bool pos = vel > 0.0;  // heaviside(u_j)
-  const long long base = mem * g.mstride + (long long)i + (long long)g.nx * j;
+  const int i = (int)(r % g.nx), j0 = (int)(r / g.nx) * kVelPerThread;
   const long long kst = (long long)g.nx * g.nu;  // stride of the point index
   const double *__restrict__ u = sa.u;
-  double w[NSP], f[NSP];
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) {
-    w[k] = u[base + kst * k];
-    f[k] = vel * w[k] * g.iJ;
-  }
-  double tl = 0.0, tr = 0.0;
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) {
-    tl += f[k] * ops.ll[k];
-    tr += f[k] * ops.lr[k];
-  }
-  // the upwind neighbour's trace: the left cell's right trace (u_j > 0) or the right cell's left
-  // trace, periodic
-  const int in = pos ? (i == 0 ? g.nx - 1 : i - 1) : (i == g.nx - 1 ? 0 : i + 1);
-  const long long nbase = base + (long long)(in - i);
-  double nb = 0.0;
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) nb += vel * u[nbase + kst * k] * g.iJ * (pos ? ops.lr[k] : ops.ll[k]);
-  const double iL = pos ? nb : tl, iR = pos ? tr : nb;
   const double tv = tau[mem];
-  const double *pr = prim + mem * 3 * g.npts + i;
+  const double *pr = prim + mem * 4 * g.npts + i;
+  double U[NSP], lam[NSP], c0[NSP];
 #pragma unroll
   for (int k = 0; k < NSP; ++k) {
-    double r1 = 0.0;
+    U[k] = pr[g.npts + (long long)g.nx * k];
+    lam[k] = pr[2 * g.npts + (long long)g.nx * k];
+    c0[k] = pr[3 * g.npts + (long long)g.nx * k];
+  }
+  const int jend = min(g.nu, j0 + kVelPerThread);
+  for (int j = j0; j < jend; ++j) {
+    const double vel = uv[j];
+    const bool pos = vel > 0.0;  // heaviside(u_j)
+    const long long base = mem * g.mstride + (long long)i + (long long)g.nx * j;
+    double w[NSP], f[NSP];
 #pragma unroll
-    for (int q = 0; q < NSP; ++q) r1 += f[q] * ops.dl[k + NSP * q];
-    const double rho = pr[(long long)g.nx * k], U = pr[g.npts + (long long)g.nx * k],
-                 lam = pr[2 * g.npts + (long long)g.nx * k];
-    const double c = vel - U;
-    const double M = rho * sqrt(lam / M_PI) * exp(-lam * (c * c));
-    const double res = -(r1 + (iL - tl) * ops.dhl[k] + (iR - tr) * ops.dhr[k]) + (M - w[k]) / tv;
-    const long long o = base + kst * k;
-    if (sa.residual) {
-      sa.out[o] = res;
-    } else {
-      double v = sa.a1 * w[k];
-      if (sa.a0 != 0.0) v = sa.a0 * sa.u0[o] + v;
-      sa.out[o] = v + sa.b * res;
+    for (int k = 0; k < NSP; ++k) {
+      w[k] = u[base + kst * k];
+      f[k] = vel * w[k] * g.iJ;
+    }
+    double tl = 0.0, tr = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+      tl += f[k] * ops.ll[k];
+      tr += f[k] * ops.lr[k];
+    }
+    // the upwind neighbour's trace: the left cell's right trace (u_j > 0) or the right cell's
+    // left trace, periodic
+    const int in = pos ? (i == 0 ? g.nx - 1 : i - 1) : (i == g.nx - 1 ? 0 : i + 1);
+    const long long nbase = base + (long long)(in - i);
+    double nb = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) nb += vel * u[nbase + kst * k] * g.iJ * (pos ? ops.lr[k] : ops.ll[k]);
+    const double iL = pos ? nb : tl, iR = pos ? tr : nb;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+      double r1 = 0.0;
+#pragma unroll
+      for (int q = 0; q < NSP; ++q) r1 += f[q] * ops.dl[k + NSP * q];
+      const double c = vel - U[k];
+      const double M = c0[k] * exp(-lam[k] * (c * c));
+      const double res = -(r1 + (iL - tl) * ops.dhl[k] + (iR - tr) * ops.dhr[k]) + (M - w[k]) / tv;
+      const long long o = base + kst * k;
+      if (sa.residual) {
+        sa.out[o] = res;
+      } else {
+        double v = sa.a1 * w[k];
+        if (sa.a0 != 0.0) v = sa.a0 * sa.u0[o] + v;
+        sa.out[o] = v + sa.b * res;
+      }
     }
   }
 }
@@ -159,7 +175,7 @@ struct Bgk1dCtx {
     }
     for (int q = 0; q < NSP * NSP; ++q) o.dl[q] = ops.dl[q];
     const Bgk1dGeo g = geo();
-    const long long n = (long long)nx * nu * nmem;
+    const long long n = (long long)nx * ((nu + kVelPerThread - 1) / kVelPerThread) * nmem;
     hipLaunchKernelGGL(bgk1d_rhs_kernel<NSP>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, o, d_vel,
                        d_tau, d_prim, sa);
     FR_LAUNCHED("bgk1d_rhs_kernel");
@@ -242,7 +258,7 @@ int fr_bgk1d_create(fr_bgk1d_ctx **out, const fr_bgk1d_desc *d) {
   if (e == hipSuccess) e = hipMemcpy(c->d_vel, vel.data(), vel.size() * sizeof(double), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMalloc(&c->d_tau, d->nmem * sizeof(double));
   if (e == hipSuccess) e = hipMemcpy(c->d_tau, d->tau, d->nmem * sizeof(double), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMalloc(&c->d_prim, (size_t)g.npts * 3 * d->nmem * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->d_prim, (size_t)g.npts * 4 * d->nmem * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->d_u, c->len * sizeof(double));
   if (e != hipSuccess) {
     int rc = hip_fail(e, "fr_bgk1d_create", __FILE__, __LINE__);
@@ -345,12 +361,12 @@ int fr_bgk1d_moments(fr_bgk1d_ctx *c, double *w) {
   const Bgk1dGeo g = c->geo();
   int rc = c->launch_moments(c->d_u, c->stream);
   if (rc) return rc;
-  std::vector<double> prim((size_t)g.npts * 3 * c->nmem);
+  std::vector<double> prim((size_t)g.npts * 4 * c->nmem);
   FR_HIP(hipMemcpyAsync(prim.data(), c->d_prim, prim.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   FR_HIP(hipStreamSynchronize(c->stream));
   for (long long mem = 0; mem < c->nmem; ++mem)
     for (long long p = 0; p < g.npts; ++p) {
-      const double *pr = prim.data() + mem * 3 * g.npts + p;
+      const double *pr = prim.data() + mem * 4 * g.npts + p;
       const double rho = pr[0], U = pr[g.npts], lam = pr[2 * g.npts];
       double *o = w + mem * 3 * g.npts + p;
       o[0] = rho;

@@ -20,10 +20,10 @@ import time
 import numpy as np
 
 from . import ics, sensor
-from .bgk import BGK2D, maxwellian_state, velocity_grid
+from .bgk import BGK1D, BGK2D, VSpace1D, bgk_wave_initial, maxwellian_state, velocity_grid
 from .euler2d import FREuler2D
 from .fr1d import Ensemble1D
-from .space import FRPSpace2D
+from .space import CSpace2D, FRPSpace1D, FRPSpace2D
 
 
 def riemann_ensemble(steps=10, nprob=512, seed=2024):
@@ -89,6 +89,64 @@ def bgk(steps=3, members=1, seed=5):
             "alg_GBs": vals * 3 * steps * (8 + 64 / 3) / el / 1e9, "steps": steps}
 
 
+def bgk1d(steps=20, members=1, nx=8192, nu=256):
+    """The 1D BGK model of example/bgk_wave.jl as written (Radau FR p = 2, VSpace1D on [-5, 5],
+    gamma = 3, tau = 1e-2, the script's initial Maxwellians) on nx cells x nu velocities, one member
+    or `members` copies (one launch per stage for all); SSP-RK3, wall time over eager steps.
+    Algorithmic bytes per value and stage: 8 (moments) + 21.33 (transport: 16 / 24 / 24)."""
+    ps = FRPSpace1D(0.0, 1.0, nx, 2)
+    vs = VSpace1D(-5.0, 5.0, nu)
+    f0 = bgk_wave_initial(ps, vs)
+    g = BGK1D(ps, vs, tau=[1e-2] * members)
+    g.set_state(np.tile(f0.ravel(order="F"), members))
+    dt = 0.1 * (1.0 / nx) / 5.0
+    g.step(dt, 2, "ssprk3")
+    t0 = time.perf_counter()
+    g.step(dt, steps, "ssprk3")
+    el = time.perf_counter() - t0
+    vals = g.len
+    g.close()
+    stage = el / (3 * steps)
+    return {"members": members, "values": int(vals), "stage_us": stage * 1e6, "value": vals / stage,
+            "alg_GBs": vals * (8 + 64 / 3) / stage / 1e9, "steps": steps}
+
+
+def polar_stage(n=256, steps=20):
+    """Cells with point-varying metrics: the p = 3 SSP-RK3 stage on the polar mesh
+    CSpace2D(1, 6, n, 0, pi, n, 1, 1) (fr2d_set_metrics), Rusanov, ghost layer as data, a smooth
+    positive state, 20 graph-replayed steps after 300 eager ones (dt = 1e-7 keeps the run finite),
+    HIP events on the launch stream.  Algorithmic bytes: 21.33 per DOF and stage + 8 for the metric."""
+    import torch
+
+    ps = FRPSpace2D(CSpace2D(1.0, 6.0, n, 0.0, np.pi, n, 1, 1), 3)
+    x, y = ps.xpg[..., 0], ps.xpg[..., 1]
+    rho = 1.0 + 0.2 * np.sin(x) * np.cos(y)
+    uu, vv, p = 0.3 * np.cos(y), -0.2 * np.sin(x), 1.0 + 0.1 * np.cos(x + y)
+    u0 = np.asfortranarray(np.stack([rho, rho * uu, rho * vv, p / 0.4 + 0.5 * rho * (uu * uu + vv * vv)], axis=-1))
+    s = FREuler2D(ps, 1.4, "rusanov", "ghost")
+    st = torch.cuda.Stream()
+    s.use_torch_stream(st)
+    s.set_state(u0)
+    dt = 1e-7
+    with torch.cuda.stream(st):
+        s.step_graph(dt, steps, 1)
+        s.set_state(u0)
+        s.step(dt, 300)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        s.step_graph(dt, steps, 1)
+        e1.record(st)
+        torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / (3 * steps)
+    dof = n * n * 16 * 4
+    s.check_state()
+    s.close()
+    return {"grid": f"polar {n}x{n}", "dof": dof, "stage_us": us, "dof_stage_per_s": dof / (us * 1e-6),
+            "alg_GBs": (64.0 / 3.0 + 8.0) * dof / (us * 1e-6) / 1e9}
+
+
 def shock_vortex_loop(mode="modal", steps=600):
     """`steps` iterations of the shock-vortex host loop (example/shock-vortex.jl:296-326) on the
     512 x 256, p = 3 case: positivity, SSP-RK3, sensor + filter, ghost refresh."""
@@ -145,8 +203,9 @@ def vortex_grid(n=1024, steps=60, flux="rusanov"):
 
 def main():
     ap = argparse.ArgumentParser(description="run one measured workload and print one JSON line")
-    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "shock_vortex", "vortex_grid"])
-    ap.add_argument("--n", type=int, default=1024, help="vortex_grid: cells per side")
+    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "bgk1d", "bgk1d64", "polar", "shock_vortex",
+                                         "vortex_grid"])
+    ap.add_argument("--n", type=int, default=None, help="vortex_grid (1024) / polar (256): cells per side")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--mode", default="modal", choices=["modal", "learned", "off"])
     a = ap.parse_args()
@@ -156,8 +215,14 @@ def main():
         r = bgk(a.steps or 3)
     elif a.workload == "bgk64":
         r = bgk(a.steps or 1, members=64)
+    elif a.workload == "bgk1d":
+        r = bgk1d(a.steps or 20)
+    elif a.workload == "bgk1d64":
+        r = bgk1d(a.steps or 4, members=64)
+    elif a.workload == "polar":
+        r = polar_stage(a.n or 256, a.steps or 20)
     elif a.workload == "vortex_grid":
-        r = vortex_grid(a.n, a.steps or 60)
+        r = vortex_grid(a.n or 1024, a.steps or 60)
     else:
         r = shock_vortex_loop(a.mode, a.steps or 600)
     print(json.dumps(r), flush=True)
