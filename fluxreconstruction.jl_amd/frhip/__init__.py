@@ -27,6 +27,7 @@ from .space import (  # noqa: F401
     dvandermonde_matrix,
     rs_jacobi,
     rs_jacobi_mesh,
+    basis_norm,
     shock_detector,
     L1_error,
     L2_error,

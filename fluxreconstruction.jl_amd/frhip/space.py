@@ -140,11 +140,54 @@ def inv(A) -> np.ndarray:
     return out.reshape((n, n), order="F")
 
 
-def rs_jacobi(r: float, s: float, vertices) -> np.ndarray:
-    """Bilinear quadrilateral Jacobian [xr xs; yr ys] (src/Geometry/geo_jacobi.jl:77-88)."""
+def _rs_jacobi_point(r: float, s: float, vertices) -> np.ndarray:
     J = np.zeros(4)
     check(_lib.lib().fr_rs_jacobi_quad(float(r), float(s), dptr(_f(vertices)), dptr(J)))
     return J.reshape((2, 2), order="F")
+
+
+def rs_jacobi(*args) -> np.ndarray:
+    """Bilinear quadrilateral Jacobians [xr xs; yr ys] (src/Geometry/geo_jacobi.jl:77-108), in the
+    reference's method forms, with a 4 x 2 vertex matrix (one cell) or a (ncx, ncy, 4, 2) vertex
+    array (every cell, leading axes (ncx, ncy)); results end in the 2 x 2 matrix:
+    - rs_jacobi(r, s, vertices), scalars r, s: one matrix (:77-88);
+    - rs_jacobi(r, s, vertices), vectors r, s: one matrix per pair (r[i], s[i]) (:90-91);
+    - rs_jacobi(r, s, vertices), matrices r (A x B) and s (C x D): the reference's comprehension
+      over i in axes(r, 1), j in axes(s, 2) at (r[i], s[i]) with Julia's linear (column-major)
+      index i, an A x D array (:93-94);
+    - rs_jacobi(r, vertices), a vector r: the tensor form J[i, j] at (r[i], r[j]) (:102-103)."""
+    if len(args) == 2:
+        r, vertices = np.asarray(args[0], dtype=np.float64), np.asarray(args[1], dtype=np.float64)
+        if vertices.shape == (4, 2):
+            J, _ = rs_jacobi_mesh(vertices[None, None], r)
+            return J[0, 0]
+        J, _ = rs_jacobi_mesh(vertices, r)
+        return J
+    if len(args) != 3:
+        raise TypeError("rs_jacobi(r, s, vertices) or rs_jacobi(r, vertices)")
+    r, s, vertices = args
+    vertices = np.asarray(vertices, dtype=np.float64)
+    if vertices.ndim == 4:
+        return np.stack([np.stack([rs_jacobi(r, s, vertices[i, j]) for j in range(vertices.shape[1])])
+                         for i in range(vertices.shape[0])])
+    if np.ndim(r) == 0:
+        return _rs_jacobi_point(r, s, vertices)
+    r, s = np.asarray(r, dtype=np.float64), np.asarray(s, dtype=np.float64)
+    if r.ndim == 1:
+        return np.stack([_rs_jacobi_point(r[i], s[i], vertices) for i in range(r.size)])
+    rl, sl = r.ravel(order="F"), s.ravel(order="F")
+    out = np.zeros((r.shape[0], s.shape[1], 2, 2))
+    for i in range(r.shape[0]):
+        out[i, :] = _rs_jacobi_point(rl[i], sl[i], vertices)[None]
+    return out
+
+
+def basis_norm(deg: int) -> np.ndarray:
+    """Discrete L1 norms of the orthonormal Legendre basis P_0 .. P_deg on 100 uniform points of
+    [-1, 1], dx sum |P_i| (src/Polynomial/poly_filter.jl:54-66)."""
+    x = np.linspace(-1.0, 1.0, 100)
+    dx = x[1] - x[0]
+    return np.array([dx * np.sum(np.abs(JacobiP(x, 0.0, 0.0, i))) for i in range(deg + 1)])
 
 
 def shock_detector(Se: float, deg: int, S0: float | None = None, kappa: float = 4.0) -> bool:

@@ -1020,14 +1020,11 @@ int or_fr2d_positivity(const or_fr2d *d, const double *wp4, double *u) {
 }
 
 /* src/dissipation.jl:38-59 (scalar vector version). */
-int or_positive_limiter1d_scalar(double *u, int n, const double *weights, const double *ll,
+int or_positive_limiter1d_scalar(double *u, int n, const double *weights, double wsum, const double *ll,
                                  const double *lr) {
-  double s = 0.0, sw = 0.0, b0 = 0.0, b1 = 0.0, mn = INFINITY;
-  for (int i = 0; i < n; ++i) {
-    s += u[i] * weights[i];
-    sw += weights[i];
-  }
-  double um = s / sw;
+  double s = 0.0, b0 = 0.0, b1 = 0.0, mn = INFINITY;
+  for (int i = 0; i < n; ++i) s += u[i] * weights[i];
+  double um = s / wsum;
   for (int i = 0; i < n; ++i) {
     b0 += u[i] * ll[i];
     b1 += u[i] * lr[i];

@@ -113,7 +113,9 @@ int or_positive_limiter1d_euler(double *ue, int nsp, double g, const double *wei
 int or_fr1d_sensor_filter(int ncell, int nsp, const double *V, const double *iV, double dt, double *u);
 int or_fr1d_positivity(int ncell, int nsp, double g, const double *weights, const double *ll, const double *lr,
                        double *u);
-int or_positive_limiter1d_scalar(double *u, int n, const double *weights, const double *ll,
+/* src/dissipation.jl:38-59 on one scalar element; wsum = sum(weights) as Julia forms it (the sum of
+ * the array, or the weight itself when the caller passes one scalar weight) */
+int or_positive_limiter1d_scalar(double *u, int n, const double *weights, double wsum, const double *ll,
                                  const double *lr);
 
 /* ---------------- 1D residuals (src/Equation/eq_euler.jl, eq_advection.jl, eq_scalar.jl) ------ */

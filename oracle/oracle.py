@@ -360,6 +360,19 @@ def positive_limiter2d(ue, g, weights, ll, lr):
     return rc, a
 
 
+def positive_limiter1d_scalar(u, weights, ll, lr):
+    """One scalar element u(p) (src/dissipation.jl:38-59, positive_limiter for a scalar field).
+    weights is an array or one number; as in Julia, sum(weights) of one number is that number, so
+    the element mean um = sum(u .* w) / sum(w) is then sum(u).  Returns (rc, limited copy)."""
+    a = np.array(u, dtype=np.float64)
+    scalar = np.ndim(weights) == 0
+    w = np.ascontiguousarray(np.broadcast_to(np.asarray(weights, dtype=np.float64), a.shape))
+    wsum = float(weights) if scalar else float(np.sum(w))
+    rc = lib().or_positive_limiter1d_scalar(_p(a), C.c_int(a.size), _p(w), C.c_double(wsum), _p(_f64(ll)),
+                                            _p(_f64(lr)))
+    return rc, a
+
+
 def positive_limiter1d_euler(ue, g, weights, ll, lr):
     """One 1D Euler element ue(p, k) (nsp x 3); weights sum to one. Returns (rc, limited copy)."""
     a = np.array(ue, dtype=np.float64, order="F")

@@ -1,0 +1,89 @@
+"""The calls of the reference's own test file (test/runtests.jl), in order, on frhip's host
+functions, with the values the mathematics fixes for them.  test_triangle.jl and TriFRPSpace
+(triangle meshes read with PyCall) are out of scope; GPU counterparts of the solver calls are in
+the tests/test_gpu_*.py files."""
+import numpy as np
+
+import frhip
+
+
+def test_errors_runtests_6_10():
+    rng = np.random.default_rng(0)
+    u, u0 = rng.random(3), rng.random(3)
+    d = np.abs(u - u0) * 0.1
+    assert np.isclose(frhip.L1_error(u, u0, 0.1), d.sum(), rtol=1e-15)
+    assert np.isclose(frhip.L2_error(u, u0, 0.1), np.sqrt((d * d).sum()), rtol=1e-15)
+    assert np.isclose(frhip.Linf_error(u, u0, 0.1), d.max(), rtol=1e-15)
+
+
+def test_shock_detector_runtests_12_14():
+    assert frhip.shock_detector(-np.inf, 3) is False
+    assert frhip.shock_detector(np.log10(0.1), 3) is True
+    assert frhip.shock_detector(np.log10(1e5), 3) is True
+
+
+def test_rs_jacobi_forms_runtests_16_20():
+    ps2 = frhip.FRPSpace2D(0.0, 1.0, 20, 0.0, 1.0, 20, 5, 1, 1)
+    r = ps2.xpl
+    # a square of side 2 turned by -30 degrees: the bilinear map is a rotation, J constant
+    v = np.array([[0.0, 0.0], [np.sqrt(3), -1.0], [np.sqrt(3) + 1, np.sqrt(3) - 1], [1.0, np.sqrt(3)]])
+    J = frhip.rs_jacobi(r, v)
+    assert J.shape == (6, 6, 2, 2)
+    rot = np.array([[np.sqrt(3) / 2, 0.5], [-0.5, np.sqrt(3) / 2]])
+    assert np.max(np.abs(J - rot)) <= 1e-15
+    # per cell of a random vertex array: the tensor form at every (r[i], r[j]) of every cell
+    vr = np.random.default_rng(1).random((3, 3, 4, 2))
+    Jr = frhip.rs_jacobi(r, vr)
+    assert Jr.shape == (3, 3, 6, 6, 2, 2)
+    for (a, b, i, j) in [(0, 0, 0, 0), (2, 1, 3, 5), (1, 2, 5, 1)]:
+        assert np.array_equal(Jr[a, b, i, j], frhip.rs_jacobi(r[i], r[j], vr[a, b]))
+    # the pairwise vector form and the reference's linear-index matrix form
+    pair = frhip.rs_jacobi(r, r[::-1], v)
+    assert pair.shape == (6, 2, 2) and np.array_equal(pair[2], frhip.rs_jacobi(r[2], r[3], v))
+    ri = np.stack([r, np.ones(6), r[::-1], np.zeros(6)])
+    si = np.stack([np.zeros(6), r, np.ones(6), r[::-1]])
+    lin = frhip.rs_jacobi(ri, si, vr[1, 1])
+    assert lin.shape == (4, 6, 2, 2)
+    for f, (rr, ss) in enumerate(((r[0], 0.0), (1.0, r[0]), (r[-1], 1.0), (0.0, r[-1]))):
+        assert np.array_equal(lin[f, 4], frhip.rs_jacobi(rr, ss, vr[1, 1]))
+
+
+def test_positive_limiter_calls_runtests_24_25():
+    """positive_limiter(ones(6), 1/6, ll, lr) and the Euler form on ones(6, 3): a constant state is
+    admissible, so both return it unchanged (the oracle restates the limiter; the library's device
+    limiter is compared with it in the GPU tests)."""
+    from oracle import oracle as O
+
+    ops = O.Ops(5)
+    rc, u = O.positive_limiter1d_scalar(np.ones(6), 1.0 / 6.0, ops.ll, ops.lr)
+    assert rc == 0 and np.array_equal(u, np.ones(6))
+    rc, ue = O.positive_limiter1d_euler(np.ones((6, 3), order="F"), 5.0 / 3.0, np.full(6, 1.0 / 6.0), ops.ll, ops.lr)
+    assert rc == 0 and np.array_equal(ue, np.ones((6, 3)))
+
+
+def test_basis_norm_and_filter_exp_runtests_27_31():
+    deg = 5
+    nrm = frhip.basis_norm(deg)
+    x = np.linspace(-1.0, 1.0, 100)
+    dx = x[1] - x[0]
+    # P_0 = 1 / sqrt(2) on the orthonormal basis: 100 dx / sqrt(2)
+    assert np.isclose(nrm[0], 100 * dx / np.sqrt(2.0), rtol=1e-14)
+    # P_n from numpy's Legendre module, scaled by sqrt((2n + 1) / 2)
+    for n in range(deg + 1):
+        Pn = np.polynomial.legendre.legval(x, np.eye(deg + 1)[n]) * np.sqrt((2 * n + 1) / 2)
+        assert np.isclose(nrm[n], dx * np.abs(Pn).sum(), rtol=1e-13)
+    ps = frhip.FRPSpace1D(0.0, 1.0, 20, deg)
+    F = frhip.filter_exp(2, 10, ps.V)  # 6 x 6 V: the triangle branch of N = 2 (nv = 6)
+    assert F.shape == (6, 6) and np.all(np.isfinite(F))
+
+
+def test_interp_face_and_standard_lagrange_runtests_39_43():
+    ps = frhip.FRPSpace1D(0.0, 1.0, 20, 5)
+    rng = np.random.default_rng(2)
+    f = rng.standard_normal((5, 6))
+    ll, lr, dl = frhip.standard_lagrange(ps.xpl)
+    assert np.allclose(ll, ps.ll, atol=1e-15) and np.allclose(lr, ps.lr, atol=1e-15)
+    assert np.allclose(dl @ ps.xpl, np.ones(6), atol=1e-12)  # derivative of x
+    # interp_face! is a device routine (tests/test_gpu_ops.py); its host meaning:
+    ff = np.stack([f @ ll, f @ lr], axis=1)
+    assert ff.shape == (5, 2)
