@@ -447,3 +447,60 @@ def test_euler_residual_device_entry_point():
             g.dudt(dud, ud)
         torch.cuda.synchronize()
         assert np.array_equal(dud.cpu().numpy().reshape(g.shape, order="F"), ref)
+
+
+def test_euler_lowlevel_as_written():
+    """example/euler_lowlevel.jl as the reference writes it: 100 cells, degree 2, gamma = 5/3,
+    cell-wise states prim (1, 0, 0.5) | (0.3, 0, 0.625) at x <= 0.5 (KitBase prim: rho, U,
+    lambda), Dirichlet ends, HLL, Midpoint with dt = cfl dx = 5e-4 for tspan[2] ÷ dt = 299 steps,
+    no filter or limiter; the device run against the oracle's (rel L2 <= 1e-10)."""
+    ncell, deg, g = 100, 2, 5.0 / 3.0
+    ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
+    left = np.repeat((ps.base.x <= 0.5)[:, None], deg + 1, axis=1)
+    prim = np.stack([np.where(left, 1.0, 0.3), np.zeros(left.shape), np.where(left, 0.5, 0.625)], -1)
+    u0 = np.asfortranarray(ics.prim_conserve1(prim, g))
+    dt = 0.05 * (1.0 / ncell)
+    nt = int(0.15 // dt)
+    assert nt == 299
+    pr = frhip.FREulerProblem(ps, g, "dirichlet", "hll")
+    pr.set_state(u0)
+    pr.step(dt, nt, "midpoint")
+    ug = pr.get_state()
+    ops = oracle_ops(deg)
+    L = lambda v: O.fr1d_euler_rhs(v, ps.J, ops, g, "hll", "dirichlet")  # noqa: E731
+    uc = u0.copy(order="F")
+    for _ in range(nt):
+        uc = np.asfortranarray(uc + dt * L(np.asfortranarray(uc + (0.5 * dt) * L(uc))))
+    assert np.all(np.isfinite(ug))
+    assert np.linalg.norm(ug - uc) / np.linalg.norm(uc) <= 1e-10
+
+
+def test_euler1d_convergence_study():
+    """example/euler1d_convergence.jl: the density wave rho = 1 + 0.2 sin(2 pi x), U = 1, lambda =
+    rho (pressure 1/2) on periodic cells of degree 3, gamma = 5/3, HLL, to t = 2 (two periods, so
+    the exact solution is the initial one), errors of the point densities in L1 / L2 (src/tools.jl)
+    on 8, 16, 32 cells.  The script integrates with Tsit5 at dt = 0.08 dx; RK4 stands in for it
+    here at dt = 0.04 dx (at 0.08 dx RK4, whose stability region is smaller than Tsit5's, blows up
+    on 32 cells in the oracle as well; halving dt again changes the errors by < 0.1 %, so they are
+    the spatial ones).  Observed orders at least p + 1 - 0.3 in both norms (about 4.0)."""
+    deg, g = 3, 5.0 / 3.0
+    errs = []
+    for ncell in (8, 16, 32):
+        ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg, 0)
+        rho = 1.0 + 0.2 * np.sin(2 * np.pi * ps.xpg)
+        prim = np.stack([rho, np.ones_like(rho), rho], -1)
+        u0 = np.asfortranarray(ics.prim_conserve1(prim, g))
+        dx = 1.0 / ncell
+        dt = 0.04 * dx
+        nt = int(round(2.0 / dt))
+        assert abs(nt * dt - 2.0) < 1e-12
+        pr = frhip.FREulerProblem(ps, g, "period", "hll")
+        pr.set_state(u0)
+        pr.step(dt, nt, "rk4")
+        u = pr.get_state()
+        e = u[:, :, 0] - rho
+        errs.append((frhip.L1_error(u[:, :, 0].ravel(), rho.ravel(), dx / (deg + 1)),
+                     frhip.L2_error(u[:, :, 0].ravel(), rho.ravel(), dx / (deg + 1)), np.max(np.abs(e))))
+    for a in range(2):
+        orders = [np.log2(errs[n][a] / errs[n + 1][a]) for n in range(2)]
+        assert min(orders) >= deg + 1 - 0.3, (a, orders, errs)
