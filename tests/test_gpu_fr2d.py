@@ -659,3 +659,48 @@ def test_euler2d_wave_host_loop(direction):
     assert np.all(np.isfinite(got))
     assert rel_l2(got, ref) <= STATE_TOL
     s.close()
+
+
+def test_parallelogram_script_host_loop():
+    """dev/parallelogram.jl as written: its sheared vertices (30 x 15 cells, dx = 1/30, dy = 1/30,
+    each row shifted by 0.5 / ny), FRPSpace2D over them (the inverse Jacobian [60 -60; 0 60] in
+    every cell), p = 1, HLL, x faces in the frame of n1 = (cos(-pi/4), sin(-pi/4)) and y faces in
+    that of n2 = (0, 1) (:175-186), a density wave per cell index rho = 1 + 0.1 sin(2 pi i / nx)
+    with prim (rho, 1, 0, rho), and per iteration the script's periodic ghost copy then one
+    forward-Euler step of dt = 0.001, nt / 2 = 500 iterations; against the oracle's loop of the same
+    calls (rel L2 <= 1e-10).  gamma 5/3 (KitBase's Gas default, outside the reference tree)."""
+    nx, ny, g, dt = 30, 15, 5.0 / 3.0, 0.001
+    Dx, Dy = 1.0 / nx, 0.5 / ny
+    V = np.zeros((nx + 2, ny + 2, 4, 2), order="F")
+    for j in range(ny + 2):
+        for i in range(nx + 2):  # the script's OffsetArray indices 0..nx+1
+            V[i, j, 0, 0] = 0.5 * (j - 1) / ny + (i - 1) * Dx
+            V[i, j, 1, 0] = V[i, j, 0, 0] + Dx
+            V[i, j, 2, 0] = V[i, j, 1, 0] + Dy
+            V[i, j, 3, 0] = V[i, j, 2, 0] - Dx
+            V[i, j, 0, 1] = 0.5 * (j - 1) * Dy
+            V[i, j, 1, 1] = V[i, j, 0, 1]
+            V[i, j, 2, 1] = V[i, j, 0, 1] + Dy
+            V[i, j, 3, 1] = V[i, j, 2, 1]
+    _, iJp = frhip.rs_jacobi_mesh(V, frhip.legendre_point(1))
+    iJ = iJp[1, 1, 0, 0]
+    assert np.allclose(iJ, [[60.0, -60.0], [0.0, 60.0]], rtol=1e-12, atol=1e-10)
+    u0 = np.zeros((nx + 2, ny + 2, 2, 2, 4), order="F")
+    for i in range(nx + 2):
+        rho = 1.0 + 0.1 * np.sin(2 * np.pi * i / nx)
+        u0[i] = O.prim_conserve2([rho, 1.0, 0.0, rho], g)
+    n1, n2 = (np.cos(-np.pi / 4), np.sin(-np.pi / 4)), (0.0, 1.0)
+    ps = frhip.FRPSpace2D(0.0, 1.0, nx, 0.0, 0.5, ny, 1, 1, 1)
+    s = frhip.FREuler2D(ps, g, "hll", "ghost")
+    s.set_geometry(iJ, n1, n2)
+    orc = O.Fr2d(nx, ny, 1, Dx, Dy, g, "hll", "ghost", iJ=iJ, n1=n1, n2=n2)
+    s.set_state(u0)
+    ref = u0.copy(order="F")
+    for _ in range(500):
+        s.fill_ghost("periodic")
+        s.step(dt, 1, "euler")
+        ref = orc.stage(_ghost_rule_numpy(ref, "periodic"), _ghost_rule_numpy(ref, "periodic"), 0.0, 1.0, dt)
+    got = s.get_state()
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got, ref) <= STATE_TOL
+    s.close()
