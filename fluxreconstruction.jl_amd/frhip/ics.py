@@ -77,6 +77,37 @@ def rankine_hugoniot(mach, gamma):
     return (1.0, u1, 1.0), (rr, u1 / rr, pr)
 
 
+def shock_vortex_script(ps, gamma=5.0 / 3.0, mach=1.12):
+    """The initial condition of example/shock-vortex.jl:160-190 as the script writes it: cells whose
+    centre lies at x <= x1 / 4 take the post-shock state moving at u1 - u2, the others the quiescent
+    pre-shock state with the vortex (kappa 0.2, mu 0.204, rc 0.05, centre (0.8, 0.5), within 8 rc)
+    added per point, its density and temperature by the script's own expressions.  The shock states
+    come from the Rankine-Hugoniot relations of `rankine_hugoniot` (pre-shock rho = 1, p = 1):
+    KitBase's ib_rh, which the script calls, is not part of the reference tree, so its
+    normalisation is not pinned."""
+    (r1, u1, p1), (r2, u2, p2) = rankine_hugoniot(mach, gamma)
+    lam1, lam2 = r1 / (2.0 * p1), r2 / (2.0 * p2)
+    x = ps.xpg[..., 0]
+    y = ps.xpg[..., 1]
+    xc = np.broadcast_to(ps.base.x[:, :, None, None], x.shape)
+    kap, mu, rc, x0, y0 = 0.2, 0.204, 0.05, 0.8, 0.5
+    r = np.sqrt((x - x0) ** 2 + (y - y0) ** 2)
+    eta = r / rc
+    du = kap * eta * np.exp(mu * (1 - eta ** 2)) * (y - y0) / r
+    dv = -kap * eta * np.exp(mu * (1 - eta ** 2)) * (x - x0) / r
+    dT = -(gamma - 1) * kap ** 2 / (8 * mu * gamma) * np.exp(2 * mu * (1 - eta ** 2))
+    T0 = 1.0 / lam1
+    rho_v = r1 ** (gamma - 1) * (T0 + dT) / T0 ** (1 / (gamma - 1))
+    vort = r <= rc * 8
+    up = xc <= ps.x1 * 0.25
+    rho = np.where(up, r2, np.where(vort, rho_v, r1))
+    U = np.where(up, u1 - u2, np.where(vort, du, 0.0))
+    V = np.where(up, 0.0, np.where(vort, dv, 0.0))
+    lam = np.where(up, lam2, np.where(vort, 1.0 / (1.0 / lam1 + dT), lam1))
+    prim = np.stack([rho, U, V, lam], axis=-1)
+    return _to_state(prim_conserve(prim, gamma))
+
+
 def shock_vortex(ps, gamma=1.4, shock_mach=1.1, shock_x=0.5, vortex_mach=0.9, vortex_r=0.075,
                  vortex_x=0.25, vortex_y=0.5):
     """Stationary normal shock at x = shock_x with Rankine-Hugoniot states and a Lamb-Oseen

@@ -704,3 +704,45 @@ def test_parallelogram_script_host_loop():
     assert np.all(np.isfinite(got))
     assert rel_l2(got, ref) <= STATE_TOL
     s.close()
+
+
+def test_shock_vortex_script_host_loop():
+    """example/shock-vortex.jl's run as written: FRPSpace2D(0, 2, 100, 0, 1, 50, 2, 1, 1), gamma 5/3,
+    Ma 1.12, the script's initial condition (frhip.ics.shock_vortex_script), HLL, and 100 iterations
+    of its loop (:296-326): positivity limiter with weights wp / 4, one Midpoint step of dt =
+    0.001, the modal sensor (S0 = -3 log10(deg), kappa 9) + l2 filter of strength 5e-4 on every
+    cell, the ghost copy of bottom / top / right.  The device loop call by call against the
+    oracle's (rel L2 <= 1e-10, the same filtered-cell count, no limiter failure), and the native
+    loop (fr2d_run_host_loop) bitwise equal to the call-by-call one."""
+    g, dt, nit = 5.0 / 3.0, 0.001, 100
+    ps = frhip.FRPSpace2D(0.0, 2.0, 100, 0.0, 1.0, 50, 2, 1, 1)
+    u0 = ics.shock_vortex_script(ps, g)
+    S0 = -3.0 * np.log10(2)
+    s = frhip.FREuler2D(ps, g, "hll", "ghost")
+    s.set_state(u0)
+    s.positivity()
+    nf = 0
+    for n in range(nit):
+        s.step(dt, 1, "midpoint")
+        nf += s.sensor_filter("modal", 5e-4, S0, 9.0)
+        s.fill_ghost("inflow_left")
+        if n + 1 < nit:
+            s.positivity()
+    got = s.get_state()
+    orc = O.Fr2d(100, 50, 2, 0.02, 0.02, g, "hll", "ghost")
+    ref = u0.copy(order="F")
+    bad = orc.positivity(ref)
+    nref = 0
+    for n in range(nit):
+        orc.midpoint(ref, dt, 1)
+        nref += orc.sensor_filter(ref, 5e-4, S0, 9.0)[0]
+        ref[...] = _ghost_rule_numpy(ref, "inflow_left")
+        if n + 1 < nit:
+            bad += orc.positivity(ref)
+    assert bad == 0 and nf == nref and nf > 0
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got, ref) <= STATE_TOL
+    s.set_state(u0)
+    s.run_host_loop(dt, nit, "modal", 5e-4, S0, 9.0, "inflow_left", scheme="midpoint")
+    assert np.array_equal(s.get_state(), got)
+    s.close()
