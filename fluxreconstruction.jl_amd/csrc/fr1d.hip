@@ -359,6 +359,38 @@ __global__ __launch_bounds__(256) void fr1d_diss_kernel(const Prob1D *__restrict
   }
 }
 
+// ------------------------------------------------------------------ Tsit5 stage combinations
+// Tsitouras' 5(4) pair (Ch. Tsitouras, Comput. Math. Appl. 62 (2011) 770-775), the Tsit5 of
+// OrdinaryDiffEq that example/advection_highlevel.jl:26 and example/euler1d_convergence.jl:133
+// step with a fixed dt (adaptive = false).  Row s (0..4) holds a_{s+2, 1..s+1}, the weights of
+// k_1..k_{s+1} in the input of stage s+2; row 5 holds b_1..b_6 of the new state (b_7 = 0: the
+// seventh evaluation is the next step's first, so a fixed-step run needs six per step).
+constexpr double kTsit5[6][6] = {
+    {0.161, 0, 0, 0, 0, 0},
+    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
+    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
+    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
+    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0},
+    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774},
+};
+
+struct Coef6 {
+  double c[6];
+};
+
+// out = u + dt * (c_1 k_1 + ... + c_nk k_nk), k_j at k + (j-1)*stride: one thread per value
+// (coalesced, HBM-bound).  out may be u (the last combination updates the state in place: every
+// thread reads its own u[i] before writing out[i]).
+__global__ __launch_bounds__(256) void fr1d_tsit5_comb_kernel(double *out, const double *u,
+                                                              const double *__restrict__ k, size_t stride,
+                                                              int nk, Coef6 c, double dt, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    double s = c.c[0] * k[i];
+    for (int j = 1; j < nk; ++j) s += c.c[j] * k[j * stride + i];
+    out[i] = u[i] + dt * s;
+  }
+}
+
 // ------------------------------------------------------------------ batched problem set
 struct Batch1D {
   int eq = FR_EQ_EULER, flux = FR_FLUX_HLL, bc = FR_BC1_DIRICHLET, correction = FR_CORR_RADAU;
@@ -371,6 +403,7 @@ struct Batch1D {
   Prob1D *d_probs = nullptr;
   double *d_ops = nullptr;
   double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_acc = nullptr;
+  double *d_k = nullptr;  // Tsit5: the six stage derivatives, len doubles each
   double *d_in = nullptr, *d_out = nullptr;
   int *d_count = nullptr;
   // work lists per thread count (64, 128, 256): concatenated list and per-member sub-lists
@@ -438,7 +471,7 @@ struct Batch1D {
 
   void release() {
     if (stream) (void)hipStreamSynchronize(stream);
-    double *bufs[] = {d_ops, d_u, d_s1, d_s2, d_acc, d_in, d_out};
+    double *bufs[] = {d_ops, d_u, d_s1, d_s2, d_acc, d_k, d_in, d_out};
     for (double *b : bufs)
       if (b) (void)hipFree(b);
     if (d_probs) (void)hipFree(d_probs);
@@ -493,10 +526,22 @@ struct Batch1D {
     return FR_OK;
   }
 
-  int ensure_stage_buffers(bool acc) {
+  int ensure_stage_buffers(bool acc, bool tsit = false) {
     if (!d_s1) FR_HIP(hipMalloc(&d_s1, len * sizeof(double)));
     if (!d_s2) FR_HIP(hipMalloc(&d_s2, len * sizeof(double)));
     if (acc && !d_acc) FR_HIP(hipMalloc(&d_acc, len * sizeof(double)));
+    if (tsit && !d_k) FR_HIP(hipMalloc(&d_k, 6 * len * sizeof(double)));
+    return FR_OK;
+  }
+
+  // out = u + dt * sum_{j < nk} row[j] k_{j+1} over the whole state
+  int tsit5_comb(double *out, const double *u, const double *row, int nk, double dt, hipStream_t st) const {
+    Coef6 c{};
+    for (int j = 0; j < nk; ++j) c.c[j] = row[j];
+    const size_t blocks = std::min<size_t>((len + 255) / 256, 8192);
+    hipLaunchKernelGGL(fr1d_tsit5_comb_kernel, dim3((unsigned)blocks), dim3(256), 0, st, out, u, d_k, len, nk, c, dt,
+                       len);
+    FR_LAUNCHED("fr1d_tsit5_comb_kernel");
     return FR_OK;
   }
 
@@ -572,6 +617,19 @@ struct Batch1D {
           double *const X = (n & 1) ? S1 : U, *const Y = (n & 1) ? U : S1;
           if ((rc = stg(X, X, Y, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
           if (n == nsteps - 1 && Y != U) FR_HIP(hipMemcpyAsync(U, Y, len * sizeof(double), hipMemcpyDeviceToDevice, st));
+          break;
+        }
+        case FR_RK_TSIT5: {
+          // k_s = L(Y_s) (residual launch), Y_1 = U, Y_{s+1} = U + dt sum_j a_{s+1,j} k_j, and
+          // U <- U + dt sum_j b_j k_j (the last combination, in place)
+          for (int sg = 0; sg < 6; ++sg) {
+            Stage1D r{};
+            r.u = sg == 0 ? U : S1;
+            r.out = d_k + (size_t)sg * len;
+            r.residual = 1;
+            if ((rc = stage(r, mode, st))) return rc;
+            if ((rc = tsit5_comb(sg == 5 ? U : S1, U, kTsit5[sg], sg + 1, dt, st))) return rc;
+          }
           break;
         }
         default:
@@ -698,8 +756,8 @@ int fr1d_get_state(fr1d_ctx *c, double *u) {
 
 int fr1d_step(fr1d_ctx *c, int scheme, double dt, int nsteps) {
   FR_REQUIRE(c && nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr1d_step: bad arguments");
-  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr1d_step: unknown scheme");
-  int rc = c->ensure_stage_buffers(scheme == FR_RK_RK4);
+  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_TSIT5, FR_E_INVALID, "fr1d_step: unknown scheme");
+  int rc = c->ensure_stage_buffers(scheme == FR_RK_RK4, scheme == FR_RK_TSIT5);
   if (rc) return rc;
   return c->enqueue_steps(scheme, 0, dt, nsteps, c->stream);
 }

@@ -361,6 +361,7 @@ int fr2d_state_pitch(const fr2d_ctx *c, int *pitch, long long *plane) {
 int fr2d_step(fr2d_ctx *c, int scheme, double dt, int nsteps) {
   FR_REQUIRE(c, FR_E_INVALID, "fr2d_step: null context");
   FR_REQUIRE(nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_step: bad dt / nsteps");
+  FR_REQUIRE(scheme != FR_RK_TSIT5, FR_E_UNSUPPORTED, "fr2d_step: Tsit5 is built for the 1D problems only");
   FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_step: unknown scheme");
   hipStream_t st = stream_of(*c);
   int rc = prepare_step(*c, scheme, st);
@@ -372,6 +373,7 @@ int fr2d_step(fr2d_ctx *c, int scheme, double dt, int nsteps) {
 int fr2d_step_graph(fr2d_ctx *c, int scheme, double dt, int nsteps, int nrep) {
   FR_REQUIRE(c, FR_E_INVALID, "fr2d_step_graph: null context");
   FR_REQUIRE(nsteps >= 1 && nrep >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_step_graph: bad arguments");
+  FR_REQUIRE(scheme != FR_RK_TSIT5, FR_E_UNSUPPORTED, "fr2d_step_graph: Tsit5 is built for the 1D problems only");
   FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_step_graph: unknown scheme");
   hipStream_t st = stream_of(*c);
   int rc = prepare_step(*c, scheme, st);
@@ -513,6 +515,7 @@ int fr2d_run_host_loop(fr2d_ctx *c, int scheme, double dt, int nsteps, int mode,
                        double kappa, int rule, int *n_filtered) {
   FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_run_host_loop: no state");
   FR_REQUIRE(nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_run_host_loop: bad dt / nsteps");
+  FR_REQUIRE(scheme != FR_RK_TSIT5, FR_E_UNSUPPORTED, "fr2d_run_host_loop: Tsit5 is built for the 1D problems only");
   FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_run_host_loop: unknown scheme");
   FR_REQUIRE(mode >= FR_SENSOR_OFF && mode <= FR_SENSOR_LEARNED, FR_E_INVALID, "fr2d_run_host_loop: unknown mode");
   FR_REQUIRE(lambda >= 0.0 && kappa > 0.0, FR_E_INVALID, "fr2d_run_host_loop: lambda >= 0 and kappa > 0 required");

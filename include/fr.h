@@ -40,7 +40,7 @@ enum { FR_FLUX_HLL = 0, FR_FLUX_RUSANOV = 1, FR_FLUX_ROE = 2 };
  * example/shock-vortex.jl:324-326).  FR_BC_PERIODIC: neighbours wrap inside the kernels. */
 enum { FR_BC_GHOST = 0, FR_BC_PERIODIC = 1 };
 enum { FR_CORR_RADAU = 0, FR_CORR_SD = 1, FR_CORR_HUYNH = 2 };
-enum { FR_RK_SSPRK3 = 0, FR_RK_RK4 = 1, FR_RK_MIDPOINT = 2, FR_RK_EULER = 3 };
+enum { FR_RK_SSPRK3 = 0, FR_RK_RK4 = 1, FR_RK_MIDPOINT = 2, FR_RK_EULER = 3, FR_RK_TSIT5 = 4 };
 /* ghost refresh rules for fr2d_fill_ghost (host loops of the reference scripts) */
 enum {
   FR_GHOST_PERIODIC = 0, /* copy the opposite interior row/column (euler2d_wave.jl:127-131, no sign flips) */
@@ -306,7 +306,10 @@ int fr1d_rhs_dev(fr1d_ctx *ctx, const double *d_u, double *d_du, double t);
  * null stream, FR_OWN_STREAM the context's own non-blocking stream, the default) */
 int fr1d_set_stream(fr1d_ctx *ctx, void *hip_stream);
 /* device-resident state and nsteps steps of FR_RK_*, replacing init / step!(itg) on the
- * ODEProblem (example/euler_highlevel.jl:34,54) */
+ * ODEProblem (example/euler_highlevel.jl:34,54).  FR_RK_TSIT5 is OrdinaryDiffEq's Tsit5 with a
+ * fixed dt (adaptive = false: example/advection_highlevel.jl:26, euler1d_convergence.jl:133),
+ * six residual evaluations per step; it is built for these 1D problems only (fr2d_step and
+ * fr_bgk1d_step return FR_E_UNSUPPORTED for it). */
 int fr1d_set_state(fr1d_ctx *ctx, const double *u);
 int fr1d_get_state(fr1d_ctx *ctx, double *u);
 int fr1d_step(fr1d_ctx *ctx, int scheme, double dt, int nsteps);

@@ -452,6 +452,43 @@ def fr1d_advection_rhs(u, J, ops: Ops, a=1.0, bc="period"):
     return du
 
 
+# ----------------------------------------------------------------- Tsit5 (fixed step)
+# Tsitouras' 5(4) pair (Ch. Tsitouras, "Runge-Kutta pairs of order 5(4) satisfying only the first
+# column simplifying assumption", Comput. Math. Appl. 62 (2011) 770-775, Table 1, to double
+# precision): OrdinaryDiffEq's Tsit5, which example/advection_highlevel.jl:26 and
+# example/euler1d_convergence.jl:133 step with adaptive = false.  TSIT5_C: nodes c_1..c_7;
+# TSIT5_A[i]: a_{i+2, 1..i+1}; TSIT5_B: b_1..b_6 (b_7 = 0, the seventh stage is the next step's
+# first).  Pinned by the order conditions through order 5 (tests/test_oracle_tsit5.py).
+TSIT5_C = (0.0, 0.161, 0.327, 0.9, 0.9800255409045097, 1.0, 1.0)
+TSIT5_A = (
+    (0.161,),
+    (-0.008480655492356989, 0.335480655492357),
+    (2.897153057105493, -6.359448489975075, 4.3622954328695815),
+    (5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525),
+    (5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383),
+)
+TSIT5_B = (0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081,
+           2.324710524099774)
+
+
+def tsit5(L, u, dt, nsteps):
+    """nsteps fixed steps of Tsit5 on du/dt = L(u): k_1 = L(u), k_s = L(u + dt sum_j a_sj k_j),
+    u <- u + dt sum_j b_j k_j (the stage sums in the order j = 1, 2, ..)."""
+    u = np.array(u, dtype=np.float64, order="F")
+    for _ in range(nsteps):
+        k = [L(u)]
+        for row in TSIT5_A:
+            s = row[0] * k[0]
+            for a, kj in zip(row[1:], k[1:]):
+                s = s + a * kj
+            k.append(L(u + dt * s))
+        s = TSIT5_B[0] * k[0]
+        for b, kj in zip(TSIT5_B[1:], k[1:]):
+            s = s + b * kj
+        u = u + dt * s
+    return u
+
+
 def L1_error(u, ue, dx):
     a = _f64(u)
     b = _f64(ue)
