@@ -5,8 +5,8 @@
 // State u(i, j, k, member): nx periodic cells, nu velocities, nsp solution points, column-major,
 // the cell fastest, so a wavefront of consecutive cells reads contiguous memory for a fixed
 // (velocity, point).  Two kernels per stage:
-// - moments: one thread per (member, cell, point), the sums over the velocities of w f, w u f and
-//   w u^2 f / 2 (KitBase moments_conserve), then the primitive variables at gamma (conserve_prim:
+// - moments: one or kMomSlices threads per (member, cell, point), the sums over the velocities of
+//   w f, w u f and w u^2 f / 2 (KitBase moments_conserve), then the primitive variables at gamma (conserve_prim:
 //   rho, U, lambda) of the point and the Maxwellian's prefactor rho sqrt(lambda / pi), so that the
 //   transport kernel spends one exp per value on M;
 // - transport + relaxation + RK stage: one thread per (member, cell, 8 velocities).  f = u_j u / J, the
@@ -44,23 +44,88 @@ struct Bgk1dStage {
   int residual;
 };
 
-// prim[member][c][p], p = i + nx k: rho, U, lambda and rho sqrt(lambda / pi) of every point
-__global__ __launch_bounds__(256) void bgk1d_moments_kernel(Bgk1dGeo g, const double *__restrict__ f,
-                                                            const double *__restrict__ uv,
-                                                            const double *__restrict__ wv, double *prim) {
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (t >= g.npts * g.nmem) return;
-  const long long mem = t / g.npts, p = t % g.npts;
-  const long long i = p % g.nx, k = p / g.nx;
-  const double *fb = f + mem * g.mstride + i + (long long)g.nx * g.nu * k;
+// prim[member][c][p], p = i + nx k: rho, U, lambda and rho sqrt(lambda / pi) of every point.
+// The velocity sums run in kMomSlices interleaved slices (slice s: j = s, s + kMomSlices, ..),
+// added in slice order at the end, so every launch shape gives the same bits.  A workgroup takes
+// P consecutive points (of the member-major point list), lanes on consecutive cells, and S
+// threads per point:
+// - S = kMomSlices (P = kMomSplitPts): thread group s sums slice s, the partial sums meet in LDS
+//   and group 0 adds them; for small states (a one-member state of a few ten thousand points:
+//   one thread per point left three quarters of the chip idle, each with a serial chain of nu
+//   loads);
+// - S = 1 (P = 64): each thread sums all slices of its point itself; for large states, where one
+//   thread per point already fills the chip.
+constexpr int kMomSlices = 8;
+constexpr int kMomSplitPts = 32;
+constexpr long long kMomSplitBelow = 262144;  // points: below, S = kMomSlices
+
+template <int S, int P>
+__global__ __launch_bounds__(S * P) void bgk1d_moments_kernel(Bgk1dGeo g, const double *__restrict__ f,
+                                                              const double *__restrict__ uv,
+                                                              const double *__restrict__ wv, double *prim) {
+  static_assert(S == 1 || S == kMomSlices, "one thread per point or one per slice");
+  __shared__ double part[S > 1 ? S - 1 : 1][3][P];
+  const int lane = threadIdx.x % P, s = threadIdx.x / P;
+  const long long t = blockIdx.x * (long long)P + lane;
+  const bool act = t < g.npts * g.nmem;
+  const long long mem = act ? t / g.npts : 0, p = act ? t % g.npts : 0;
   double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-  for (int j = 0; j < g.nu; ++j) {
-    const double fv = fb[(long long)g.nx * j];
-    const double wj = wv[j], uj = uv[j];
-    m0 += wj * fv;
-    m1 += wj * uj * fv;
-    m2 += wj * uj * uj * fv;
+  if (act) {
+    const long long i = p % g.nx, k = p / g.nx;
+    const double *fb = f + mem * g.mstride + i + (long long)g.nx * g.nu * k;
+    if constexpr (S == 1) {
+      double a0[kMomSlices], a1[kMomSlices], a2[kMomSlices];
+#pragma unroll
+      for (int q = 0; q < kMomSlices; ++q) a0[q] = a1[q] = a2[q] = 0.0;
+      for (int j0 = 0; j0 < g.nu; j0 += kMomSlices) {
+#pragma unroll
+        for (int q = 0; q < kMomSlices; ++q) {
+          const int j = j0 + q;
+          if (j < g.nu) {
+            const double fv = fb[(long long)g.nx * j];
+            const double wj = wv[j], uj = uv[j];
+            a0[q] += wj * fv;
+            a1[q] += wj * uj * fv;
+            a2[q] += wj * uj * uj * fv;
+          }
+        }
+      }
+      m0 = a0[0];
+      m1 = a1[0];
+      m2 = a2[0];
+#pragma unroll
+      for (int q = 1; q < kMomSlices; ++q) {
+        m0 += a0[q];
+        m1 += a1[q];
+        m2 += a2[q];
+      }
+    } else {
+#pragma unroll 8
+      for (int j = s; j < g.nu; j += kMomSlices) {
+        const double fv = fb[(long long)g.nx * j];
+        const double wj = wv[j], uj = uv[j];
+        m0 += wj * fv;
+        m1 += wj * uj * fv;
+        m2 += wj * uj * uj * fv;
+      }
+    }
   }
+  if constexpr (S > 1) {
+    if (s > 0) {
+      part[s - 1][0][lane] = m0;
+      part[s - 1][1][lane] = m1;
+      part[s - 1][2][lane] = m2;
+    }
+    __syncthreads();
+    if (s != 0) return;
+#pragma unroll
+    for (int q = 0; q < S - 1; ++q) {
+      m0 += part[q][0][lane];
+      m1 += part[q][1][lane];
+      m2 += part[q][2][lane];
+    }
+  }
+  if (!act) return;
   const double w2 = 0.5 * m2;
   double *pr = prim + mem * 4 * g.npts + p;
   const double lam = 0.5 * m0 / (g.gamma - 1.0) / (w2 - 0.5 * m1 * m1 / m0);
@@ -185,8 +250,13 @@ struct Bgk1dCtx {
   int launch_moments(const double *d_f, hipStream_t st) const {
     const Bgk1dGeo g = geo();
     const long long n = g.npts * nmem;
-    hipLaunchKernelGGL(bgk1d_moments_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, d_f, d_vel,
-                       d_vel + nu, d_prim);
+    if (n < kMomSplitBelow)
+      hipLaunchKernelGGL((bgk1d_moments_kernel<kMomSlices, kMomSplitPts>),
+                         dim3((unsigned)((n + kMomSplitPts - 1) / kMomSplitPts)), dim3(kMomSlices * kMomSplitPts), 0,
+                         st, g, d_f, d_vel, d_vel + nu, d_prim);
+    else
+      hipLaunchKernelGGL((bgk1d_moments_kernel<1, 64>), dim3((unsigned)((n + 63) / 64)), dim3(64), 0, st, g, d_f,
+                         d_vel, d_vel + nu, d_prim);
     FR_LAUNCHED("bgk1d_moments_kernel");
     return FR_OK;
   }

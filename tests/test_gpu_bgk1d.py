@@ -88,6 +88,33 @@ def test_tau_ensemble_equals_single_problems():
     ens.close()
 
 
+def test_moments_launch_shapes_agree_bitwise():
+    """The moments kernel sums the velocities in eight interleaved slices added in slice order, with
+    one wavefront per slice on states below 262,144 points and one thread per point above: a
+    43-member ensemble of 6,144-point members (264,192 points, one thread per point) gives every
+    member the bits of the single problem (one wavefront per slice), residual and one SSP-RK3 step."""
+    ps, vs, f0 = script_case(nx=2048, deg=2, nu=40)
+    f = frhip.ics.perturbed(f0, seed=11, amplitude=0.05)
+    nm = 43
+    assert ps.nx * 3 * nm >= 262144 > ps.nx * 3
+    ens = bgk.BGK1D(ps, vs, tau=[1e-2] * nm)
+    one = bgk.BGK1D(ps, vs)
+    flat = np.tile(f.ravel(order="F"), nm)
+    de = ens.rhs(flat)
+    d1 = one.rhs(f)
+    for m in (0, 17, nm - 1):
+        assert np.array_equal(de[..., m], d1)
+    ens.set_state(flat)
+    one.set_state(f)
+    ens.step(2e-5, 1, "ssprk3")
+    one.step(2e-5, 1, "ssprk3")
+    assert np.array_equal(ens.get_state()[..., nm - 1], one.get_state())
+    ref = O.Bgk1d(2048, 2, vs.u, vs.weights, 1.0 / 2048).rhs(f)
+    assert rel_inf(d1, ref) <= RES_TOL
+    ens.close()
+    one.close()
+
+
 def test_mol_on_device_arrays_side_stream():
     import torch
 
