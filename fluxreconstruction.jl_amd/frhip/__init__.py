@@ -27,6 +27,8 @@ from .space import (  # noqa: F401
     dvandermonde_matrix,
     rs_jacobi,
     rs_jacobi_mesh,
+    r_x,
+    global_sp,
     basis_norm,
     shock_detector,
     L1_error,

@@ -286,6 +286,39 @@ class CSpace2D:
     ngy = property(lambda self: self.ngθ)
 
 
+def r_x(r, vl, vr):
+    """The point r of the standard element [-1, 1] on the real element [vl, vr]
+    (src/Geometry/geo_transform.jl:7)."""
+    return ((1.0 - r) / 2.0) * vl + ((1.0 + r) / 2.0) * vr
+
+
+def global_sp(*args):
+    """Global coordinates of the solution points (src/Geometry/geo_points.jl:8-41).
+
+    global_sp(xi, r): line elements between the n + 1 faces xi, shape (n, len(r)).
+    global_sp(xi, yi, r): quadrilaterals of the vertex arrays xi, yi of shape (nx + 1, ny + 1);
+    returns (xp, yp) of shape (nx, ny, len(r), len(r)): xp[i, j, k, l] = r_x(r[k], xi[i, j],
+    xi[i+1, j]) and yp[i, j, k, l] = r_x(r[l], yi[i, j], yi[i, j+1]).  The reference's loop runs j
+    over all ny + 1 vertex columns, whose last one reads yi[i, ny + 2] past the end of the array;
+    here j covers the ny cells."""
+    if len(args) == 2:
+        xi = np.asarray(args[0], dtype=np.float64)
+        r = np.asarray(args[1], dtype=np.float64)
+        return np.asfortranarray(r_x(r[None, :], xi[:-1, None], xi[1:, None]))
+    if len(args) == 3:
+        xi = np.asarray(args[0], dtype=np.float64)
+        yi = np.asarray(args[1], dtype=np.float64)
+        r = np.asarray(args[2], dtype=np.float64)
+        if xi.ndim != 2 or xi.shape != yi.shape:
+            raise ValueError("xi and yi must be vertex arrays of one shape (nx + 1, ny + 1)")
+        xp = r_x(r[None, None, :, None], xi[:-1, :-1, None, None], xi[1:, :-1, None, None])
+        yp = r_x(r[None, None, None, :], yi[:-1, :-1, None, None], yi[:-1, 1:, None, None])
+        n = r.size
+        shape = (xi.shape[0] - 1, xi.shape[1] - 1, n, n)
+        return np.asfortranarray(np.broadcast_to(xp, shape)), np.asfortranarray(np.broadcast_to(yp, shape))
+    raise TypeError("global_sp(xi, r) or global_sp(xi, yi, r)")
+
+
 def rs_jacobi_mesh(vertices, r):
     """J[i, j][k, l] = rs_jacobi(r[k], r[l], vertices[i, j]) of every cell and solution point and
     its inverse (src/struct.jl:134-142, geo_jacobi.jl:90-98), as (ncx, ncy, n, n, 2, 2) arrays."""

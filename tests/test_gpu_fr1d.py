@@ -536,6 +536,36 @@ def test_advection_highlevel_as_written():
     assert frhip.Linf_error(u.ravel(), u0.ravel(), dx) < 8e-7
 
 
+def test_advection_cuda_as_written():
+    """example/advection_cuda.jl, the reference's GPU example, as written but in FP64 (the script
+    keeps a Float32 CuArray; this library computes in FP64 only): 100 periodic cells of degree 2 on
+    [-1, 1], u0 = exp(-20 x^2), a = 1, Midpoint at dt = 0.1 dx (dx = 2f0 / 100, so dt =
+    0.1 * 0.02f0) for nt = floor(1f0 / dt) = 500 steps.  Against the oracle's Midpoint run: rel L2
+    <= 1e-6, because the regularised flux (f_R - f_L) / (u_R - u_L + 1e-8) is ill-conditioned on
+    this smooth run (one ulp of one input value moves the oracle's own final state by 3.5e-7; the
+    Float32 rounding of u0 alone by 2.2e-7, which is what the script's Float32 run carries).  Against
+    the pulse shifted by nt dt (periodic): L1 error < 4e-4 (the oracle's: 3.06e-4)."""
+    ncell, deg = 100, 2
+    ps = frhip.FRPSpace1D(-1.0, 1.0, ncell, deg)
+    u0 = np.asfortranarray(np.exp(-20.0 * ps.xpg**2))
+    dt = 0.1 * float(np.float32(2.0) / np.float32(100))
+    nt = int(np.floor(1.0 / dt))
+    assert nt == 500
+    pr = frhip.FRAdvectionProblem(ps, 1.0, "period")
+    pr.set_state(u0)
+    pr.step(dt, nt, "midpoint")
+    u = pr.get_state()
+    ops = oracle_ops(deg)
+    L = lambda v: O.fr1d_advection_rhs(v, ps.J, ops, 1.0, "period")  # noqa: E731
+    ref = u0.copy(order="F")
+    for _ in range(nt):
+        y = ref + (0.5 * dt) * L(ref)
+        ref = ref + dt * L(y)
+    assert np.linalg.norm(u - ref) / np.linalg.norm(ref) <= 1e-6
+    xs = (ps.xpg - nt * dt + 1.0) % 2.0 - 1.0
+    assert frhip.L1_error(u.ravel(), np.exp(-20.0 * xs**2).ravel(), 2.0 / ncell) < 4e-4
+
+
 @pytest.mark.parametrize("deg", [2, 4, 7])
 def test_tsit5_parity_dirichlet(deg):
     """Tsit5 steps with the Dirichlet ends (du = 0 in the end cells) and Rusanov, split into two

@@ -87,3 +87,19 @@ def test_interp_face_and_standard_lagrange_runtests_39_43():
     # interp_face! is a device routine (tests/test_gpu_ops.py); its host meaning:
     ff = np.stack([f @ ll, f @ lr], axis=1)
     assert ff.shape == (5, 2)
+
+
+def test_global_sp_and_r_x():
+    """global_sp / r_x (geo_points.jl:8-41, geo_transform.jl:7): the line form as bgk_wave.jl:29
+    calls it gives FRPSpace1D's xpg; the quadrilateral form on a rectangular vertex grid gives the
+    interior of FRPSpace2D's xpg; r_x maps -1, 0, 1 to the ends and the midpoint."""
+    assert frhip.r_x(-1.0, 2.0, 5.0) == 2.0 and frhip.r_x(1.0, 2.0, 5.0) == 5.0 and frhip.r_x(0.0, 2.0, 5.0) == 3.5
+    ps = frhip.FRPSpace1D(0.0, 1.0, 20, 2)
+    xface = np.linspace(0.0, 1.0, 21)
+    assert np.allclose(frhip.global_sp(xface, ps.xpl), ps.xpg, rtol=0, atol=1e-15)
+    ps2 = frhip.FRPSpace2D(0.0, 2.0, 5, -1.0, 1.0, 4, 3, 1, 1)
+    xi, yi = np.meshgrid(np.linspace(0.0, 2.0, 6), np.linspace(-1.0, 1.0, 5), indexing="ij")
+    xp, yp = frhip.global_sp(xi, yi, ps2.xpl)
+    assert xp.shape == (5, 4, 4, 4)
+    assert np.allclose(xp, ps2.xpg[1:-1, 1:-1, :, :, 0], rtol=0, atol=1e-14)
+    assert np.allclose(yp, ps2.xpg[1:-1, 1:-1, :, :, 1], rtol=0, atol=1e-14)
