@@ -4,7 +4,7 @@
  * TEST INFRASTRUCTURE ONLY.  This is a plain-C restatement of the reference
  * Julia package (vavrines/FluxReconstruction.jl) and of the KitBase routines it
  * calls.  Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
- * may load it, and only as the checker / the timed CPU baseline.  The product
+ * may load it, and only as the reference results / the timed CPU baseline.  The product
  * library (libfrhip.so) never links, loads or calls anything in this folder.
  *
  * Array conventions follow the reference: every multi-dimensional array is

@@ -1,6 +1,6 @@
 """ctypes view of the CPU oracle (oracle/liboracle.so).
 
-TEST INFRASTRUCTURE ONLY: the checker for tests/, __graft_entry__.smoke() and the
+TEST INFRASTRUCTURE ONLY: the reference results for tests/ and __graft_entry__.smoke(), and the
 cpu_baseline leg of bench.py.  The product package never imports this module.
 Arrays follow the reference's Julia column-major layout (see fr_oracle.h).
 """
