@@ -452,7 +452,7 @@ int fr2d_step_dev_noalloc(fr2d_ctx *c, double *d_u, int scheme, double dt, int n
 
 int fr2d_fill_ghost(fr2d_ctx *c, int rule) {
   FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_fill_ghost: no state");
-  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_WAVE, FR_E_INVALID, "fr2d_fill_ghost: unknown rule");
+  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_WAVE_Y, FR_E_INVALID, "fr2d_fill_ghost: unknown rule");
   return fr2d_launch_fill_ghost(*c, c->d_u, rule, stream_of(*c));
 }
 
@@ -484,7 +484,7 @@ int fr2d_filter_refresh_limit(fr2d_ctx *c, int mode, double lambda, double S0, d
   FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_filter_refresh_limit: no state");
   FR_REQUIRE(mode >= FR_SENSOR_OFF && mode <= FR_SENSOR_LEARNED, FR_E_INVALID, "fr2d_filter_refresh_limit: unknown mode");
   FR_REQUIRE(lambda >= 0.0 && kappa > 0.0, FR_E_INVALID, "fr2d_filter_refresh_limit: lambda >= 0 and kappa > 0 required");
-  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_WAVE, FR_E_INVALID, "fr2d_filter_refresh_limit: unknown rule");
+  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_WAVE_Y, FR_E_INVALID, "fr2d_filter_refresh_limit: unknown rule");
   hipStream_t st = stream_of(*c);
   if (mode == FR_SENSOR_LEARNED && !c->mlp_loaded) mode = FR_SENSOR_MODAL;  // plain sensor is the fallback
   FR_HIP(hipMemsetAsync(c->d_count, 0, 2 * sizeof(int), st));
@@ -519,7 +519,7 @@ int fr2d_run_host_loop(fr2d_ctx *c, int scheme, double dt, int nsteps, int mode,
   FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_run_host_loop: unknown scheme");
   FR_REQUIRE(mode >= FR_SENSOR_OFF && mode <= FR_SENSOR_LEARNED, FR_E_INVALID, "fr2d_run_host_loop: unknown mode");
   FR_REQUIRE(lambda >= 0.0 && kappa > 0.0, FR_E_INVALID, "fr2d_run_host_loop: lambda >= 0 and kappa > 0 required");
-  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_WAVE, FR_E_INVALID, "fr2d_run_host_loop: unknown rule");
+  FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_WAVE_Y, FR_E_INVALID, "fr2d_run_host_loop: unknown rule");
   hipStream_t st = stream_of(*c);
   if (mode == FR_SENSOR_LEARNED && !c->mlp_loaded) mode = FR_SENSOR_MODAL;  // plain sensor is the fallback
   // d_count[0]: filtered cells, d_count[1]: limiter failures, both summed over the loop

@@ -90,16 +90,18 @@ __global__ __launch_bounds__(256) void fr2d_sensor_filter_kernel(double *u, int 
 
 
 // Source cell of ghost cell (gi, gj) under a ghost refresh rule (as fr2d_fill_ghost_kernel);
-// keep = the rule leaves the cell as it is (the inflow column of FR_GHOST_INFLOW_LEFT); negy =
-// the y-momentum is copied negated (the y ghost rows of FR_GHOST_WAVE).
+// keep = the rule leaves the cell as it is (the inflow column of FR_GHOST_INFLOW_LEFT); negv =
+// the variable copied negated (2: the y-momentum of the y ghost rows of FR_GHOST_WAVE, 1: the
+// x-momentum of the x ghost columns of FR_GHOST_WAVE_Y), -1 none.
 __device__ __forceinline__ void ghost_source(int gi, int gj, int nx, int ny, int rule, int &si, int &sj, bool &keep,
-                                             bool &negy) {
+                                             int &negv) {
   keep = false;
-  negy = false;
-  if (rule == FR_GHOST_PERIODIC || rule == FR_GHOST_WAVE) {
+  negv = -1;
+  if (rule == FR_GHOST_PERIODIC || rule == FR_GHOST_WAVE || rule == FR_GHOST_WAVE_Y) {
     si = (gi == 0) ? nx : (gi == nx + 1 ? 1 : gi);
     sj = (gj == 0) ? ny : (gj == ny + 1 ? 1 : gj);
-    negy = rule == FR_GHOST_WAVE && (gj == 0 || gj == ny + 1);
+    if (rule == FR_GHOST_WAVE && (gj == 0 || gj == ny + 1)) negv = 2;
+    if (rule == FR_GHOST_WAVE_Y && (gi == 0 || gi == nx + 1)) negv = 1;
   } else if (rule == FR_GHOST_OUTFLOW) {
     si = (gi == 0) ? 1 : (gi == nx + 1 ? nx : gi);
     sj = (gj == 0) ? 1 : (gj == ny + 1 ? ny : gj);
@@ -171,8 +173,9 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
       // a kept ghost stays: it may be the source of other ghosts (the corners of
       // FR_GHOST_INFLOW_LEFT copy the inflow column)
       int si, sj;
-      bool keep, negy;
-      ghost_source(i, j, nx, ny, rule, si, sj, keep, negy);
+      bool keep;
+      int negv;
+      ghost_source(i, j, nx, ny, rule, si, sj, keep, negv);
       if (!keep) return;  // refreshed by the threads of its source cell
     }
   }
@@ -247,11 +250,12 @@ __global__ __launch_bounds__(256, 4) void fr2d_filter_refresh_limit_kernel(
         if (dupj || (gi == i && gj == j)) continue;
         if (!(gi == 0 || gi == nx + 1 || gj == 0 || gj == ny + 1)) continue;
         int si, sj;
-        bool keep, negy;
-        ghost_source(gi, gj, nx, ny, rule, si, sj, keep, negy);
+        bool keep;
+        int negv;
+        ghost_source(gi, gj, nx, ny, rule, si, sj, keep, negv);
         if (keep || si != i || sj != j) continue;
         double *const gr = u + (long long)gi + (long long)pitch * gj + ldp * (NP * r);
-        const double sg = (negy && r == 2) ? -1.0 : 1.0;
+        const double sg = r == negv ? -1.0 : 1.0;
 #pragma unroll
         for (int q = 0; q < NP; ++q) gr[ldp * q] = sg * x[q];
       }

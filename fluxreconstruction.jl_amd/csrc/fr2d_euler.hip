@@ -599,7 +599,7 @@ __global__ __launch_bounds__(NSP * 64, (fr2d_min_blocks<NSP, FLUX, EPI, GEO>()))
 }
 
 // ------------------------------------------------------------------ ghost-layer helpers
-// rule FR_GHOST_PERIODIC / _OUTFLOW / _INFLOW_LEFT / _WAVE over every ghost cell and every
+// rule FR_GHOST_PERIODIC / _OUTFLOW / _INFLOW_LEFT / _WAVE / _WAVE_Y over every ghost cell and every
 // (k,l,m) plane (planes = 4 nsp^2, variable m = plane / nsp^2).
 __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, int pitch, long long ldp, int planes, int rule) {
   const int nring = 2 * (nx + 2) + 2 * ny;
@@ -625,13 +625,16 @@ __global__ void fr2d_fill_ghost_kernel(double *u, int nx, int ny, int pitch, lon
     }
     int si = i, sj = j;
     bool neg = false;
-    if (rule == FR_GHOST_PERIODIC || rule == FR_GHOST_WAVE) {
+    if (rule == FR_GHOST_PERIODIC || rule == FR_GHOST_WAVE || rule == FR_GHOST_WAVE_Y) {
       // euler2d_wave.jl:127-134 fills the x ghosts first and then the whole y ghost rows from the
-      // filled columns, so a corner takes the diagonally opposite interior cell; the wave rule
-      // negates the y-momentum of the y ghost rows
+      // filled columns (:153-160 the other way round), so a corner takes the diagonally opposite
+      // interior cell; the wave rules negate the y-momentum of the y ghost rows (along x) or the
+      // x-momentum of the x ghost columns (along y)
       si = (i == 0) ? nx : (i == nx + 1 ? 1 : i);
       sj = (j == 0) ? ny : (j == ny + 1 ? 1 : j);
-      neg = rule == FR_GHOST_WAVE && (j == 0 || j == ny + 1) && pl / (planes / 4) == 2;
+      const long long var = pl / (planes / 4);
+      neg = (rule == FR_GHOST_WAVE && (j == 0 || j == ny + 1) && var == 2) ||
+            (rule == FR_GHOST_WAVE_Y && (i == 0 || i == nx + 1) && var == 1);
     } else if (rule == FR_GHOST_OUTFLOW) {
       si = (i == 0) ? 1 : (i == nx + 1 ? nx : i);
       sj = (j == 0) ? 1 : (j == ny + 1 ? ny : j);

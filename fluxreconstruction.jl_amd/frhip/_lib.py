@@ -25,7 +25,7 @@ FLUX = {"hll": 0, "rusanov": 1, "roe": 2}
 BC = {"ghost": 0, "periodic": 1}
 CORR = {"radau": 0, "sd": 1, "huynh": 2}
 RK = {"ssprk3": 0, "rk4": 1, "midpoint": 2, "euler": 3, "tsit5": 4}
-GHOST = {"periodic": 0, "outflow": 1, "inflow_left": 2, "wave": 3}
+GHOST = {"periodic": 0, "outflow": 1, "inflow_left": 2, "wave": 3, "wave_y": 4}
 SENSOR = {"off": 0, "modal": 1, "learned": 2}
 EQ = {"euler": 0, "advection": 1}
 BC1 = {"dirichlet": 0, "period": 1}

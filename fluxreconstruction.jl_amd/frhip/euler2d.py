@@ -206,6 +206,8 @@ class FREuler2D:
         check(_lib.lib().fr2d_step_graph(self.h, _lib.RK[scheme], float(dt), int(nsteps), int(nrep)))
 
     def fill_ghost(self, rule: str = "periodic"):
+        """Ghost refresh of the device state: "periodic", "outflow", "inflow_left"
+        (shock-vortex.jl:322-325), "wave" / "wave_y" (euler2d_wave.jl:127-132 / :153-158)."""
         check(_lib.lib().fr2d_fill_ghost(self.h, _lib.GHOST[rule]))
 
     def sensor_filter(self, mode: str = "modal", lam: float = 5e-4, S0: float | None = None, kappa: float = 9.0):

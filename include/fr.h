@@ -47,8 +47,10 @@ enum {
   FR_GHOST_OUTFLOW = 1,  /* copy the adjacent interior row/column on every side */
   FR_GHOST_INFLOW_LEFT = 2, /* shock-vortex.jl:322-325: bottom, top and right copy the adjacent row/column,
                                the left (supersonic inflow) column keeps its values */
-  FR_GHOST_WAVE = 3 /* euler2d_wave.jl:127-134: FR_GHOST_PERIODIC with the y-momentum (variable 3) of the
-                       bottom and top ghost rows, corners included, negated */
+  FR_GHOST_WAVE = 3, /* euler2d_wave.jl:127-134 (wave along x): FR_GHOST_PERIODIC with the y-momentum
+                        (variable 3) of the bottom and top ghost rows, corners included, negated */
+  FR_GHOST_WAVE_Y = 4 /* euler2d_wave.jl:153-160 (wave along y): FR_GHOST_PERIODIC with the x-momentum
+                         (variable 2) of the left and right ghost columns, corners included, negated */
 };
 /* sensor modes for fr2d_sensor_filter */
 enum { FR_SENSOR_OFF = 0, FR_SENSOR_MODAL = 1, FR_SENSOR_LEARNED = 2 };

@@ -139,7 +139,7 @@ def test_fused_filter_refresh_limit_equals_host_loop(mode, bc, iters):
 
 @pytest.mark.parametrize("deg", [1, 2, 3, 4, 5])
 @pytest.mark.parametrize("mode", ["off", "modal"])
-@pytest.mark.parametrize("rule", ["periodic", "outflow", "inflow_left", "wave"])
+@pytest.mark.parametrize("rule", ["periodic", "outflow", "inflow_left", "wave", "wave_y"])
 def test_fused_pass_with_active_limiter(deg, mode, rule):
     """The fused pass on a state whose limiter corrects cells (density and energy pushed down at a
     few points) reproduces the unfused sensor_filter -> fill_ghost -> positivity sequence bitwise,
