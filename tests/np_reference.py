@@ -153,3 +153,65 @@ def fr2d_rhs_metric(u, ops, g, iJ, flux="hll", periodic=False):
     du = np.zeros_like(u)
     du[I, J] = -(d1 + d2 + corr)
     return np.asfortranarray(du)
+
+
+def exact_riemann(wl, wr, g, xi):
+    """Exact solution of the 1D Euler Riemann problem (Toro, Riemann Solvers and Numerical Methods
+    for Fluid Dynamics, ch. 4) at the similarity coordinates xi = (x - x0) / t.  wl, wr =
+    (rho, u, p) of the two states; returns rho, u, p arrays shaped like xi.  The star pressure is
+    the root of f_L(p) + f_R(p) + u_R - u_L (shock branch for p above the side's pressure,
+    rarefaction branch below), found by bisection to round-off."""
+    rl, ul, pl = wl
+    rr, ur, pr = wr
+    cl, cr = np.sqrt(g * pl / rl), np.sqrt(g * pr / rr)
+    gm, gp = g - 1.0, g + 1.0
+
+    def fk(p, rk, pk, ck):
+        if p > pk:
+            a, b = 2.0 / (gp * rk), gm / gp * pk
+            return (p - pk) * np.sqrt(a / (p + b))
+        return 2.0 * ck / gm * ((p / pk) ** (gm / (2 * g)) - 1.0)
+
+    f = lambda p: fk(p, rl, pl, cl) + fk(p, rr, pr, cr) + ur - ul  # noqa: E731
+    lo, hi = 1e-14, max(pl, pr)
+    while f(hi) < 0:
+        hi *= 2.0
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if f(mid) > 0:
+            hi = mid
+        else:
+            lo = mid
+    ps = 0.5 * (lo + hi)
+    us = 0.5 * (ul + ur) + 0.5 * (fk(ps, rr, pr, cr) - fk(ps, rl, pl, cl))
+    xi = np.asarray(xi, dtype=np.float64)
+    rho, u, p = np.empty_like(xi), np.empty_like(xi), np.empty_like(xi)
+    for n, s in np.ndenumerate(xi):
+        if s <= us:
+            if ps > pl:
+                sl = ul - cl * np.sqrt(gp / (2 * g) * ps / pl + gm / (2 * g))
+                w = (rl, ul, pl) if s <= sl else (rl * (ps / pl + gm / gp) / (gm / gp * ps / pl + 1.0), us, ps)
+            else:
+                csl = cl * (ps / pl) ** (gm / (2 * g))
+                if s <= ul - cl:
+                    w = (rl, ul, pl)
+                elif s > us - csl:
+                    w = (rl * (ps / pl) ** (1.0 / g), us, ps)
+                else:
+                    c = 2.0 / gp * (cl + gm / 2 * (ul - s))
+                    w = (rl * (c / cl) ** (2.0 / gm), 2.0 / gp * (cl + gm / 2 * ul + s), pl * (c / cl) ** (2 * g / gm))
+        else:
+            if ps > pr:
+                sr = ur + cr * np.sqrt(gp / (2 * g) * ps / pr + gm / (2 * g))
+                w = (rr, ur, pr) if s >= sr else (rr * (ps / pr + gm / gp) / (gm / gp * ps / pr + 1.0), us, ps)
+            else:
+                csr = cr * (ps / pr) ** (gm / (2 * g))
+                if s >= ur + cr:
+                    w = (rr, ur, pr)
+                elif s <= us + csr:
+                    w = (rr * (ps / pr) ** (1.0 / g), us, ps)
+                else:
+                    c = 2.0 / gp * (cr - gm / 2 * (ur - s))
+                    w = (rr * (c / cr) ** (2.0 / gm), 2.0 / gp * (-cr + gm / 2 * ur + s), pr * (c / cr) ** (2 * g / gm))
+        rho[n], u[n], p[n] = w
+    return rho, u, p, ps, us

@@ -7,6 +7,7 @@ import pytest
 import frhip
 from frhip import ics
 from oracle import oracle as O
+from np_reference import exact_riemann
 
 pytestmark = pytest.mark.gpu
 
@@ -59,20 +60,24 @@ def test_euler_residual_parity(deg, flux, bc):
 def test_euler_highlevel_sod_degree7():
     """example/euler_highlevel.jl as the reference writes it: Sod's problem on 100 cells at degree 7
     (gamma = 5/3, cell-wise left / right states, Dirichlet ends), the sensor + filter of every cell
-    before every step, Midpoint steps of dt = 0.01 dx; 150 steps on the device against the
-    oracle's loop (rel L2 <= 1e-10, same filtered-cell count), and a mixed ensemble of degrees
-    2 and 7 equal to the single problems bitwise."""
+    before every step, Midpoint steps of dt = 0.01 dx, nt = 0.15 / dt = 1499 steps (the script's
+    floor division); on the device against the oracle's loop (rel L2 <= 1e-10, same filtered-cell
+    count), the density against the exact Riemann solution at t = nt dt (mean error of the
+    points below 7e-3; the oracle's 4.8e-3), and a mixed ensemble of degrees 2 and 7 equal to the
+    single problems bitwise."""
     ncell, deg, g = 100, 7, 5.0 / 3.0
     ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
     xc = (np.arange(ncell) + 0.5) / ncell
     left = np.repeat((xc <= 0.5)[:, None], deg + 1, axis=1)
     prim = np.stack([np.where(left, 1.0, 0.125), np.zeros(left.shape), np.where(left, 0.5, 0.625)], -1)
     u0 = np.asfortranarray(ics.prim_conserve1(prim, g))
-    dt = 0.01 / ncell
+    dt = 0.01 * (1.0 / ncell)
+    nt = int(np.floor(0.15 / dt))
+    assert nt == 1499
     pr = frhip.FREulerProblem(ps, g, "dirichlet", "hll")
     pr.set_state(u0)
     nf = 0
-    for _ in range(150):
+    for _ in range(nt):
         nf += pr.sensor_filter(dt)
         pr.step(dt, 1, "midpoint")
     ug = pr.get_state()
@@ -80,11 +85,14 @@ def test_euler_highlevel_sod_degree7():
     L = lambda v: O.fr1d_euler_rhs(v, ps.J, ops, g, "hll", "dirichlet")  # noqa: E731
     uc = u0.copy(order="F")
     nref = 0
-    for _ in range(150):
+    for _ in range(nt):
         nref += O.fr1d_sensor_filter(uc, ops, dt)
         uc = np.asfortranarray(uc + dt * L(np.asfortranarray(uc + (0.5 * dt) * L(uc))))
     assert np.all(np.isfinite(ug)) and nf == nref and nf > 0
     assert np.linalg.norm(ug - uc) / np.linalg.norm(uc) <= 1e-10
+    # lambda = rho / (2 p): the states are (1, 0, 1) and (0.125, 0, 0.1) in (rho, u, p)
+    rex, _, _, _, _ = exact_riemann((1.0, 0.0, 1.0), (0.125, 0.0, 0.1), g, (ps.xpg - 0.5) / (nt * dt))
+    assert np.mean(np.abs(ug[:, :, 0] - rex)) < 7e-3
     # an ensemble mixing degree 7 with degree 2 takes the kernel instances of every degree
     ps2 = frhip.FRPSpace1D(0.0, 1.0, 64, 2)
     u2 = np.asfortranarray(euler_state(64, 2, seed=3, gamma=g))
