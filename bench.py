@@ -51,14 +51,15 @@ def measured_traffic():
     return None
 
 
-def hold_stream(stream, torch):
-    """Keeps the stream busy for about a millisecond (a bounded spin kernel of 2e6 clock cycles)
-    so that the launches of the timed region are queued behind it while the device works, and the
-    region's start event is taken when the device, not the host's submission, reaches it (the
-    blocking-kernel method of NVBench).  The spin kernel runs before the start event and is not
-    timed."""
+def hold_stream(stream, torch, cycles=2_000_000):
+    """Keeps the stream busy (a bounded spin kernel of `cycles` clock cycles, 2e6 about a
+    millisecond) so that the launches of the timed region are queued behind it while the device
+    works, and the region's start event is taken when the device, not the host's submission,
+    reaches it (the blocking-kernel method of NVBench).  The spin kernel runs before the start
+    event and is not timed.  Regions of eager calls from Python (about 15 us of host time each)
+    take a longer hold, so that all of them are queued before the device reaches the first."""
     with torch.cuda.stream(stream):
-        torch.cuda._sleep(2_000_000)
+        torch.cuda._sleep(cycles)
 
 
 def cpu_baseline(u0, dt, budget_s=12.0):
@@ -220,7 +221,7 @@ def main():
         e0 = torch.cuda.Event(enable_timing=True)
         e1 = torch.cuda.Event(enable_timing=True)
         nres = 200
-        hold_stream(stream, torch)
+        hold_stream(stream, torch, 20_000_000)
         e0.record(stream)
         for _ in range(nres):
             solver.dudt(dud, ud)
@@ -236,7 +237,7 @@ def main():
         for _ in range(5):
             torch.add(sa, sb, out=sc)
         torch.cuda.synchronize()
-        hold_stream(stream, torch)
+        hold_stream(stream, torch, 20_000_000)
         e0.record(stream)
         for _ in range(nres):
             torch.add(sa, sb, out=sc)
