@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "fr_common.h"
+#include "fr_tsit5.h"
 
 namespace fr {
 
@@ -359,38 +360,6 @@ __global__ __launch_bounds__(256) void fr1d_diss_kernel(const Prob1D *__restrict
   }
 }
 
-// ------------------------------------------------------------------ Tsit5 stage combinations
-// Tsitouras' 5(4) pair (Ch. Tsitouras, Comput. Math. Appl. 62 (2011) 770-775), the Tsit5 of
-// OrdinaryDiffEq that example/advection_highlevel.jl:26 and example/euler1d_convergence.jl:133
-// step with a fixed dt (adaptive = false).  Row s (0..4) holds a_{s+2, 1..s+1}, the weights of
-// k_1..k_{s+1} in the input of stage s+2; row 5 holds b_1..b_6 of the new state (b_7 = 0: the
-// seventh evaluation is the next step's first, so a fixed-step run needs six per step).
-constexpr double kTsit5[6][6] = {
-    {0.161, 0, 0, 0, 0, 0},
-    {-0.008480655492356989, 0.335480655492357, 0, 0, 0, 0},
-    {2.897153057105493, -6.359448489975075, 4.3622954328695815, 0, 0, 0},
-    {5.325864828439257, -11.748883564062828, 7.4955393428898365, -0.09249506636175525, 0, 0},
-    {5.86145544294642, -12.92096931784711, 8.159367898576159, -0.071584973281401, -0.028269050394068383, 0},
-    {0.09646076681806523, 0.01, 0.4798896504144996, 1.379008574103742, -3.290069515436081, 2.324710524099774},
-};
-
-struct Coef6 {
-  double c[6];
-};
-
-// out = u + dt * (c_1 k_1 + ... + c_nk k_nk), k_j at k + (j-1)*stride: one thread per value
-// (coalesced, HBM-bound).  out may be u (the last combination updates the state in place: every
-// thread reads its own u[i] before writing out[i]).
-__global__ __launch_bounds__(256) void fr1d_tsit5_comb_kernel(double *out, const double *u,
-                                                              const double *__restrict__ k, size_t stride,
-                                                              int nk, Coef6 c, double dt, size_t n) {
-  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-    double s = c.c[0] * k[i];
-    for (int j = 1; j < nk; ++j) s += c.c[j] * k[j * stride + i];
-    out[i] = u[i] + dt * s;
-  }
-}
-
 // ------------------------------------------------------------------ batched problem set
 struct Batch1D {
   int eq = FR_EQ_EULER, flux = FR_FLUX_HLL, bc = FR_BC1_DIRICHLET, correction = FR_CORR_RADAU;
@@ -536,12 +505,7 @@ struct Batch1D {
 
   // out = u + dt * sum_{j < nk} row[j] k_{j+1} over the whole state
   int tsit5_comb(double *out, const double *u, const double *row, int nk, double dt, hipStream_t st) const {
-    Coef6 c{};
-    for (int j = 0; j < nk; ++j) c.c[j] = row[j];
-    const size_t blocks = std::min<size_t>((len + 255) / 256, 8192);
-    hipLaunchKernelGGL(fr1d_tsit5_comb_kernel, dim3((unsigned)blocks), dim3(256), 0, st, out, u, d_k, len, nk, c, dt,
-                       len);
-    FR_LAUNCHED("fr1d_tsit5_comb_kernel");
+    FR_HIP(fr::tsit5_comb(out, u, d_k, len, row, nk, dt, len, st));
     return FR_OK;
   }
 

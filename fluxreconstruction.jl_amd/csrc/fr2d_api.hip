@@ -9,6 +9,7 @@
 #include <string>
 
 #include "fr2d_internal.h"
+#include "fr_tsit5.h"
 
 namespace fr {
 
@@ -68,11 +69,22 @@ static int ensure_state(Fr2dCtx &c) {
   return FR_OK;
 }
 
-static int ensure_stage_buffers(Fr2dCtx &c, bool acc) {
+static int ensure_stage_buffers(Fr2dCtx &c, bool acc, bool tsit = false) {
   int rc = FR_OK;
   if (!c.d_s1 && (rc = alloc_padded(c, c.raw_s1, c.d_s1))) return rc;
   if (!c.d_s2 && (rc = alloc_padded(c, c.raw_s2, c.d_s2))) return rc;
   if (acc && !c.d_acc && (rc = alloc_padded(c, c.raw_acc, c.d_acc))) return rc;
+  if (tsit && !c.d_k) {
+    // six arrays with the state's alignment: slots of a multiple of 256 bytes, each array
+    // kPadFront doubles into its slot as the state is into its allocation
+    const size_t slot = (c.lenp + Fr2dCtx::kPadFront + 1 + 31) / 32 * 32;
+    FR_HIP(hipMalloc(&c.raw_k, 6 * slot * sizeof(double)));
+    hipStream_t st = fr2d_stream(c);
+    FR_HIP(hipMemsetAsync(c.raw_k, 0, 6 * slot * sizeof(double), st));
+    FR_HIP(hipStreamSynchronize(st));
+    c.d_k = c.raw_k + Fr2dCtx::kPadFront;
+    c.kstride = slot;
+  }
   return FR_OK;
 }
 
@@ -142,6 +154,19 @@ static int enqueue_steps(const Fr2dCtx &c, int scheme, double dt, int nsteps, hi
           FR_HIP(hipMemcpyAsync(U, Y, c.lenp * sizeof(double), hipMemcpyDeviceToDevice, st));
         break;
       }
+      case FR_RK_TSIT5: {
+        // k_s = L(Y_s) into the padded k arrays (a stage launch with a0 = a1 = 0, b = 1, which
+        // writes no ghost cell, so the zeroed ghost layer of every k stays zero), Y_1 = U,
+        // Y_{s+1} = U + dt sum_j a_{s+1,j} k_j and U <- U + dt sum_j b_j k_j over the whole padded
+        // arrays: the ghost layer of every Y and of U stays that of u^n, as the reference holds it
+        for (int sg = 0; sg < 6; ++sg) {
+          const double *Y = sg == 0 ? U : S1;
+          double *K = c.d_k + (size_t)sg * c.kstride;
+          if ((rc = fr2d_launch_stage(c, nullptr, Y, K, 0.0, 0.0, 1.0, nullptr, nullptr, 0.0, st))) return rc;
+          FR_HIP(tsit5_comb(sg == 5 ? U : S1, U, c.d_k, c.kstride, kTsit5[sg], sg + 1, dt, c.lenp, st));
+        }
+        break;
+      }
       default:
         return fail(FR_E_INVALID, "fr2d_step: unknown scheme");
     }
@@ -153,7 +178,7 @@ static int prepare_step(Fr2dCtx &c, int scheme, hipStream_t st) {
   FR_REQUIRE(c.d_u, FR_E_INVALID, "fr2d_step: no state (call fr2d_set_state first)");
   // the stage buffers take the state's ghost layer inside the first stage of every step
   (void)st;
-  int rc = ensure_stage_buffers(c, scheme == FR_RK_RK4);
+  int rc = ensure_stage_buffers(c, scheme == FR_RK_RK4, scheme == FR_RK_TSIT5);
   // the first stepping call of a context picks the staggered-start delay on the device, on the
   // context's own stream (so a caller stream under capture records nothing of it); it reads the
   // state and writes only the first stage buffer, which the steps overwrite
@@ -261,7 +286,8 @@ int fr2d_destroy(fr2d_ctx *c) {
   if (!c) return FR_OK;
   hipStream_t st = stream_of(*c);
   if (st) (void)hipStreamSynchronize(st);
-  double *bufs[] = {c->raw_u, c->raw_s1, c->raw_s2, c->raw_acc, c->d_in, c->d_out, c->d_dops, c->d_mj_ref, c->raw_mj_pad};
+  double *bufs[] = {c->raw_u,  c->raw_s1,   c->raw_s2,   c->raw_acc,  c->raw_k,
+                    c->d_in,   c->d_out,    c->d_dops,   c->d_mj_ref, c->raw_mj_pad};
   for (double *b : bufs)
     if (b) (void)hipFree(b);
   if (c->d_count) (void)hipFree(c->d_count);
@@ -361,8 +387,7 @@ int fr2d_state_pitch(const fr2d_ctx *c, int *pitch, long long *plane) {
 int fr2d_step(fr2d_ctx *c, int scheme, double dt, int nsteps) {
   FR_REQUIRE(c, FR_E_INVALID, "fr2d_step: null context");
   FR_REQUIRE(nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_step: bad dt / nsteps");
-  FR_REQUIRE(scheme != FR_RK_TSIT5, FR_E_UNSUPPORTED, "fr2d_step: Tsit5 is built for the 1D problems only");
-  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_step: unknown scheme");
+  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_TSIT5, FR_E_INVALID, "fr2d_step: unknown scheme");
   hipStream_t st = stream_of(*c);
   int rc = prepare_step(*c, scheme, st);
   if (rc) return rc;
@@ -373,8 +398,7 @@ int fr2d_step(fr2d_ctx *c, int scheme, double dt, int nsteps) {
 int fr2d_step_graph(fr2d_ctx *c, int scheme, double dt, int nsteps, int nrep) {
   FR_REQUIRE(c, FR_E_INVALID, "fr2d_step_graph: null context");
   FR_REQUIRE(nsteps >= 1 && nrep >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_step_graph: bad arguments");
-  FR_REQUIRE(scheme != FR_RK_TSIT5, FR_E_UNSUPPORTED, "fr2d_step_graph: Tsit5 is built for the 1D problems only");
-  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_step_graph: unknown scheme");
+  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_TSIT5, FR_E_INVALID, "fr2d_step_graph: unknown scheme");
   hipStream_t st = stream_of(*c);
   int rc = prepare_step(*c, scheme, st);
   if (rc) return rc;
@@ -515,8 +539,7 @@ int fr2d_run_host_loop(fr2d_ctx *c, int scheme, double dt, int nsteps, int mode,
                        double kappa, int rule, int *n_filtered) {
   FR_REQUIRE(c && c->d_u, FR_E_INVALID, "fr2d_run_host_loop: no state");
   FR_REQUIRE(nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_run_host_loop: bad dt / nsteps");
-  FR_REQUIRE(scheme != FR_RK_TSIT5, FR_E_UNSUPPORTED, "fr2d_run_host_loop: Tsit5 is built for the 1D problems only");
-  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_EULER, FR_E_INVALID, "fr2d_run_host_loop: unknown scheme");
+  FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_TSIT5, FR_E_INVALID, "fr2d_run_host_loop: unknown scheme");
   FR_REQUIRE(mode >= FR_SENSOR_OFF && mode <= FR_SENSOR_LEARNED, FR_E_INVALID, "fr2d_run_host_loop: unknown mode");
   FR_REQUIRE(lambda >= 0.0 && kappa > 0.0, FR_E_INVALID, "fr2d_run_host_loop: lambda >= 0 and kappa > 0 required");
   FR_REQUIRE(rule >= FR_GHOST_PERIODIC && rule <= FR_GHOST_WAVE_Y, FR_E_INVALID, "fr2d_run_host_loop: unknown rule");

@@ -44,6 +44,10 @@ struct Fr2dCtx {
   // allocations, d_* = raw_* + kPadFront)
   double *d_u = nullptr, *d_s1 = nullptr, *d_s2 = nullptr, *d_acc = nullptr;
   double *raw_u = nullptr, *raw_s1 = nullptr, *raw_s2 = nullptr, *raw_acc = nullptr;
+  // Tsit5: six stage derivatives in the padded layout, kstride doubles apart (one zeroed
+  // allocation; the kernels never write their ghost cells or row padding, which stay zero)
+  double *d_k = nullptr, *raw_k = nullptr;
+  size_t kstride = 0;
   // staging buffers of the host-pointer residual
   double *d_in = nullptr, *d_out = nullptr;
   // per-cell flags / counters of the dissipation kernels

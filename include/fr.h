@@ -163,8 +163,9 @@ int fr2d_get_state_dev(fr2d_ctx *ctx, double *d_u);
 double *fr2d_state_dev(fr2d_ctx *ctx);
 int fr2d_state_pitch(const fr2d_ctx *ctx, int *pitch, long long *plane);
 /* nsteps explicit steps of size dt with scheme FR_RK_*, replacing step!(itg) of OrdinaryDiffEq
- * (shock-vortex.jl:294,305; Midpoint there, SSP-RK3 / RK4 / Euler here too); each stage is one
- * fused residual+update launch.  In FR_BC_GHOST mode the ghost layer is held fixed (L = 0 there). */
+ * (shock-vortex.jl:294,305; Midpoint there, SSP-RK3 / RK4 / Euler / fixed-step Tsit5 here too);
+ * each stage is one fused residual+update launch (Tsit5: a residual launch and an elementwise
+ * combination launch).  In FR_BC_GHOST mode the ghost layer is held fixed (L = 0 there). */
 int fr2d_step(fr2d_ctx *ctx, int scheme, double dt, int nsteps);
 /* the launches of nsteps steps captured into a hipGraph and replayed nrep times on the context's
  * stream (asynchronous, like fr2d_step); the context keeps the instantiated graph and reuses it
@@ -310,8 +311,8 @@ int fr1d_set_stream(fr1d_ctx *ctx, void *hip_stream);
 /* device-resident state and nsteps steps of FR_RK_*, replacing init / step!(itg) on the
  * ODEProblem (example/euler_highlevel.jl:34,54).  FR_RK_TSIT5 is OrdinaryDiffEq's Tsit5 with a
  * fixed dt (adaptive = false: example/advection_highlevel.jl:26, euler1d_convergence.jl:133),
- * six residual evaluations per step; it is built for these 1D problems only (fr2d_step and
- * fr_bgk1d_step return FR_E_UNSUPPORTED for it). */
+ * six residual evaluations per step and six more state-sized arrays (fr2d_step takes it too;
+ * fr_bgk1d_step and fr2d_step_dev_noalloc return FR_E_UNSUPPORTED for it). */
 int fr1d_set_state(fr1d_ctx *ctx, const double *u);
 int fr1d_get_state(fr1d_ctx *ctx, double *u);
 int fr1d_step(fr1d_ctx *ctx, int scheme, double dt, int nsteps);

@@ -321,6 +321,14 @@ class Fr2d:
     def midpoint(self, u, dt, nsteps):
         lib().or_fr2d_midpoint(C.byref(self.d), _p(self._u(u)), C.c_double(dt), C.c_int(nsteps))
 
+    def tsit5(self, u, dt, nsteps):
+        """Fixed-step Tsit5 (the module's tsit5 loop over this residual; du = 0 on the ghost layer,
+        which the steps therefore keep), in place; periodic grids refresh their ghost layer from
+        the interior at the end, as or_fr2d_ssprk3 does."""
+        u[...] = tsit5(self.rhs, u, dt, nsteps)
+        if self.d.bc == BC["periodic"]:
+            self.fill_periodic(u)
+
     def stage(self, u0, us, a0, a1, b):
         out = np.zeros(self.shape, order="F")
         lib().or_fr2d_stage(C.byref(self.d), _p(self._u(u0)), _p(self._u(us)), _p(out),

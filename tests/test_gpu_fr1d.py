@@ -635,12 +635,3 @@ def test_tsit5_parity_dirichlet(deg):
     ref = O.tsit5(lambda v: O.fr1d_euler_rhs(v, ps.J, ops, 1.4, "rusanov", "dirichlet"), u0, dt, 9)
     assert np.linalg.norm(one - ref) / np.linalg.norm(ref) <= 1e-10
     assert np.array_equal(one[0], u0[0]) and np.array_equal(one[-1], u0[-1])
-
-
-def test_tsit5_is_1d_only():
-    ps = frhip.FRPSpace2D(0.0, 1.0, 8, 0.0, 1.0, 8, 2, 1, 1)
-    s = frhip.FREuler2D(ps, 1.4, "hll", "periodic")
-    s.set_state(np.ones((10, 10, 3, 3, 4), order="F") * np.array([1.0, 0.1, 0.1, 2.5]))
-    with pytest.raises(frhip.FRError):
-        s.step(1e-3, 1, "tsit5")
-    s.close()

@@ -193,7 +193,7 @@ def test_staggered_start_tuned_and_results_unchanged():
     assert s2.stagger() == 0
 
 
-@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler"])
+@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler", "tsit5"])
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_stepping_parity(scheme, bc):
     ps, u, s, orc = setup(24, 20, 3, "rusanov", bc, amp=0.0)
@@ -208,6 +208,8 @@ def test_stepping_parity(scheme, bc):
         orc.rk4(uc, dt, 10)
     elif scheme == "midpoint":
         orc.midpoint(uc, dt, 10)
+    elif scheme == "tsit5":
+        orc.tsit5(uc, dt, 10)
     else:
         for _ in range(10):
             uc = orc.stage(uc, uc, 0.0, 1.0, dt)
@@ -216,7 +218,7 @@ def test_stepping_parity(scheme, bc):
     assert rel_l2(ug, uc) <= STATE_TOL
 
 
-@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler"])
+@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler", "tsit5"])
 def test_zero_dt_step_keeps_state(scheme):
     """dt = 0 takes the general stage form (b = 0): every scheme returns the state (SSP-RK3 up to
     the rounding of 1/3 + 2/3, the others bitwise)."""
@@ -427,7 +429,7 @@ def test_primitives_parity(deg):
     assert np.array_equal(d.cpu().numpy().reshape(sol.shape, order="F"), sol)
 
 
-@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint"])
+@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "tsit5"])
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_degree5_stepping(scheme, bc):
     """Degree 5 (nsp = 6, one workgroup of six wavefronts per CU): steps against the oracle, graph
@@ -513,7 +515,7 @@ def test_parallelogram_stepping_parity():
     assert rel_l2(s.get_state(), ur) <= STATE_TOL
 
 
-@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint"])
+@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "tsit5"])
 def test_parallelogram_stepping_ghost_layer(scheme):
     """Sheared cells with the ghost layer as data: the first stage of every step copies the ghost
     layer into the stage buffers (parallelogram kernels, buffer addressing), so every scheme

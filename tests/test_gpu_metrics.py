@@ -72,7 +72,7 @@ def test_uniform_point_metric_matches_the_rectangle_kernel():
     pt.close()
 
 
-@pytest.mark.parametrize("scheme,nsteps", [("ssprk3", 6), ("rk4", 3), ("midpoint", 4)])
+@pytest.mark.parametrize("scheme,nsteps", [("ssprk3", 6), ("rk4", 3), ("midpoint", 4), ("tsit5", 2)])
 def test_polar_stepping_parity(scheme, nsteps):
     ps = polar_space(24, 20, 2)
     u = smooth_state(ps, seed=5)
