@@ -184,9 +184,10 @@ int fr2d_get_stagger(const fr2d_ctx *ctx, int *stagger);
  * buffers taken from a fixed scratch array of the library: 2 x 6,291,456 doubles (96 MiB) per
  * GPU, held until the process ends.  No allocation and no host synchronisation; asynchronous on
  * the context's stream.  A state above 6,291,456 doubles, FR_RK_RK4 (a third array) or
- * FR_RK_TSIT5 (six more) returns FR_E_UNSUPPORTED.  Every call on one GPU shares that GPU's scratch (which keeps the last call's
- * contents and needs no initial values): calls that overlap on one GPU race, whatever their
- * stream, thread or context, and must be serialised by the caller. */
+ * FR_RK_TSIT5 (six more) returns FR_E_UNSUPPORTED.  Every call on one GPU shares that GPU's
+ * scratch (which keeps the last call's contents and needs no initial values): calls that overlap
+ * on one GPU race, whatever their stream, thread or context, and must be serialised by the
+ * caller. */
 int fr2d_step_dev_noalloc(fr2d_ctx *ctx, double *d_u, int scheme, double dt, int nsteps);
 /* refresh the ghost layer of the device state (FR_GHOST_*): the ghost copies of the reference
  * host loops (shock-vortex.jl:322-325, euler2d_wave.jl:125-135) */
