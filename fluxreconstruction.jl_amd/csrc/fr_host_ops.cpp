@@ -228,6 +228,99 @@ bool build_ops(int deg, int correction, HostOps &o) {
 // ====================================================================== C ABI: operator entry points
 using namespace fr;
 
+// ------------------------------------------------------------------ one-element positivity limiter
+// The reference's per-element host routine positive_limiter (src/dissipation.jl:37-215), for
+// callers that limit one element at a time; the device passes (fr1d_positivity, fr2d_positivity)
+// run the same steps over whole states.
+namespace {
+
+// largest t in [0, 1] with rho(t) (p(t) - eps) = 0, where the state moves on the segment from the
+// mean um (t = 0) to ut (t = 1): with rho, m, E linear in t, rho p = (g - 1)(rho E - |m|^2 / 2)
+// is a quadratic in t (the root the reference's Newton solve of tj_equation, dissipation.jl:208-215,
+// finds from t = 1; 0 when there is none)
+double squeeze_root(const double *ut, const double *um, int nv, double g, double eps) {
+  const int ne = nv - 1;  // energy slot; momenta 1 .. ne-1
+  const double r0 = um[0], r1 = ut[0] - um[0];
+  const double e0 = um[ne], e1 = ut[ne] - um[ne];
+  double mm00 = 0.0, mm01 = 0.0, mm11 = 0.0;
+  for (int v = 1; v < ne; ++v) {
+    const double a = um[v], b = ut[v] - um[v];
+    mm00 += a * a;
+    mm01 += a * b;
+    mm11 += b * b;
+  }
+  const double A = (g - 1.0) * (r1 * e1 - 0.5 * mm11);
+  const double B = (g - 1.0) * (r0 * e1 + r1 * e0 - mm01) - eps * r1;
+  const double C = (g - 1.0) * (r0 * e0 - 0.5 * mm00) - eps * r0;
+  double best = 0.0;
+  if (std::fabs(A) < 1e-300) {
+    if (B != 0.0) {
+      const double t = -C / B;
+      if (t >= 0.0 && t <= 1.0) best = t;
+    }
+    return best;
+  }
+  const double disc = B * B - 4.0 * A * C;
+  if (disc < 0.0) return 0.0;
+  const double q = -0.5 * (B + (B >= 0.0 ? std::sqrt(disc) : -std::sqrt(disc)));
+  const double ta = q / A, tb = q != 0.0 ? C / q : ta;
+  if (ta >= 0.0 && ta <= 1.0 && ta > best) best = ta;
+  if (tb >= 0.0 && tb <= 1.0 && tb > best) best = tb;
+  return best;
+}
+
+// KitBase conserve_prim's last entry, lambda = rho / (2 p), of a state (rho, m..., E)
+double lambda_of(const double *w, int nv, double g) {
+  double mm = 0.0;
+  for (int v = 1; v < nv - 1; ++v) mm += w[v] * w[v];
+  const double p = (g - 1.0) * (w[nv - 1] - 0.5 * mm / w[0]);
+  return 0.5 * w[0] / p;
+}
+
+// the Euler forms (1D: np points, nv = 3; 2D: np = nsp^2 points, nv = 4) once the traces
+// tr[nt][nv] of the element's faces are formed: density squeeze, then the pressure squeeze
+int limit_euler(double *u, int np, int nv, double g, const double *w, const double *tr, int nt, double t0,
+                const char *who) {
+  double um[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int v = 0; v < nv; ++v)
+    for (int q = 0; q < np; ++q) um[v] += u[q + np * v] * w[q];
+  const double pm = 0.5 * um[0] / lambda_of(um, nv, g);
+  const double eps = std::fmin(1e-13, std::fmin(um[0], pm));
+  double rmin = INFINITY;
+  for (int f = 0; f < nt; ++f) rmin = std::fmin(rmin, tr[f * nv]);
+  for (int q = 0; q < np; ++q) rmin = std::fmin(rmin, u[q]);
+  const double t1 = std::fmin((um[0] - eps) / (um[0] - rmin + 1e-8), 1.0);
+  if (!(t1 > 0.0 && t1 <= 1.0)) return fail(FR_E_LIMITER, std::string(who) + ": density limiter parameter out of (0, 1]");
+  for (int q = 0; q < np; ++q) u[q] = t1 * (u[q] - um[0]) + um[0];
+  // pressure squeeze over the face traces (taken before the density squeeze, as the reference
+  // forms them once) and the solution points
+  bool any = false;
+  double t2 = INFINITY;
+  for (int f = 0; f < nt; ++f)
+    if (lambda_of(tr + f * nv, nv, g) < eps) {
+      any = true;
+      t2 = std::fmin(t2, squeeze_root(tr + f * nv, um, nv, g, eps));
+    }
+  for (int q = 0; q < np; ++q) {
+    double s[4];
+    for (int v = 0; v < nv; ++v) s[v] = u[q + np * v];
+    if (lambda_of(s, nv, g) < eps) {
+      any = true;
+      t2 = std::fmin(t2, squeeze_root(s, um, nv, g, eps));
+    }
+  }
+  if (any) {
+    // min(minimum(tj), t0): the reference's minimum(tj, t0) (dissipation.jl:116,199) is a
+    // MethodError in Julia
+    t2 = std::fmin(t2, t0);
+    for (int v = 0; v < nv; ++v)
+      for (int q = 0; q < np; ++q) u[q + np * v] = t2 * (u[q + np * v] - um[v]) + um[v];
+  }
+  return FR_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *fr_last_error(void) { return g_error.c_str(); }
@@ -351,6 +444,66 @@ int fr_shock_detector(double Se, int deg, double S0, double kappa) {
   else
     sigma = 0.0;
   return sigma < 0.99 ? 1 : 0;
+}
+
+int fr_positive_limiter_scalar(double *u, int n, const double *weights, double wsum, const double *ll,
+                               const double *lr) {
+  FR_REQUIRE(u && weights && ll && lr && n >= 1 && wsum != 0.0, FR_E_INVALID, "fr_positive_limiter_scalar: bad arguments");
+  double um = 0.0, bl = 0.0, br = 0.0, umin = INFINITY;
+  for (int q = 0; q < n; ++q) {
+    um += u[q] * weights[q];
+    bl += u[q] * ll[q];
+    br += u[q] * lr[q];
+    umin = std::fmin(umin, u[q]);
+  }
+  um /= wsum;
+  const double eps = std::fmin(1e-13, um);
+  const double theta = std::fmin(std::fmin(bl, br), umin);
+  const double t = std::fmin((um - eps) / (um - theta + 1e-8), 1.0);
+  FR_REQUIRE(t > 0.0 && t <= 1.0, FR_E_LIMITER, "fr_positive_limiter_scalar: limiter parameter out of (0, 1]");
+  for (int q = 0; q < n; ++q) u[q] = t * (u[q] - um) + um;
+  return FR_OK;
+}
+
+int fr_positive_limiter_euler1d(double *u, int nsp, double gamma, const double *weights, const double *ll,
+                                const double *lr, double t0) {
+  FR_REQUIRE(u && weights && ll && lr && nsp >= 1 && nsp <= kMaxNp2D && gamma > 1.0, FR_E_INVALID,
+             "fr_positive_limiter_euler1d: bad arguments");
+  double tr[2][3];
+  for (int v = 0; v < 3; ++v) {
+    double a = 0.0, b = 0.0;
+    for (int q = 0; q < nsp; ++q) {
+      a += u[q + nsp * v] * ll[q];
+      b += u[q + nsp * v] * lr[q];
+    }
+    tr[0][v] = a;
+    tr[1][v] = b;
+  }
+  return limit_euler(u, nsp, 3, gamma, weights, &tr[0][0], 2, t0, "fr_positive_limiter_euler1d");
+}
+
+int fr_positive_limiter_euler2d(double *u, int nsp, double gamma, const double *weights, const double *ll,
+                                const double *lr, double t0) {
+  FR_REQUIRE(u && weights && ll && lr && nsp >= 1 && nsp <= 6 && gamma > 1.0, FR_E_INVALID,
+             "fr_positive_limiter_euler2d: bad arguments");
+  const int np = nsp * nsp;
+  // traces of face f (1 bottom, 2 right, 3 top, 4 left) at face point j, every variable
+  std::vector<double> tr(4 * nsp * 4);
+  for (int j = 0; j < nsp; ++j)
+    for (int v = 0; v < 4; ++v) {
+      double s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0;
+      for (int q = 0; q < nsp; ++q) {
+        s1 += u[j + nsp * q + np * v] * ll[q];
+        s2 += u[q + nsp * j + np * v] * lr[q];
+        s3 += u[j + nsp * q + np * v] * lr[q];
+        s4 += u[q + nsp * j + np * v] * ll[q];
+      }
+      tr[(0 * nsp + j) * 4 + v] = s1;
+      tr[(1 * nsp + j) * 4 + v] = s2;
+      tr[(2 * nsp + j) * 4 + v] = s3;
+      tr[(3 * nsp + j) * 4 + v] = s4;
+    }
+  return limit_euler(u, np, 4, gamma, weights, tr.data(), 4 * nsp, t0, "fr_positive_limiter_euler2d");
 }
 
 }  // extern "C"

@@ -29,6 +29,7 @@ from .space import (  # noqa: F401
     rs_jacobi_mesh,
     r_x,
     global_sp,
+    positive_limiter,
     basis_norm,
     shock_detector,
     L1_error,

@@ -197,6 +197,44 @@ def shock_detector(Se: float, deg: int, S0: float | None = None, kappa: float = 
     return bool(_lib.lib().fr_shock_detector(float(Se), int(deg), float(S0), float(kappa)))
 
 
+def positive_limiter(u, *args):
+    """positive_limiter of src/dissipation.jl on one element, in place (the reference's three
+    methods, chosen by u's dimension like Julia's dispatch), computed by libfrhip's native host
+    routine; raises FRError (FR_E_LIMITER) where the reference's @assert fails, leaving u as it was.
+
+    positive_limiter(u, weights, ll, lr[, t0]): scalar element u(nsp) (:37-58; weights an array
+    or one number, as the reference accepts);
+    positive_limiter(u, gamma, weights, ll, lr[, t0]): 1D Euler u(nsp, 3) (:61-123) or 2D Euler
+    u(nsp, nsp, 4) (:125-206), weights summing to one.  u must be a float64 numpy array; 2D and
+    matrix elements are taken in column-major order."""
+    if not isinstance(u, np.ndarray) or u.dtype != np.float64:
+        raise ValueError("u must be a float64 numpy array (limited in place)")
+    L = _lib.lib()
+    if u.ndim == 1:
+        weights, ll, lr = args[:3]
+        n = u.size
+        scalar = np.ndim(weights) == 0
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(weights, dtype=np.float64), (n,)))
+        wsum = float(weights) if scalar else float(np.sum(w))
+        a = np.ascontiguousarray(u)
+        check(L.fr_positive_limiter_scalar(dptr(a), n, dptr(w), wsum, dptr(_f(ll)), dptr(_f(lr))))
+    elif u.ndim in (2, 3):
+        g, weights, ll, lr = args[:4]
+        t0 = float(args[4]) if len(args) > 4 else 1.0
+        nsp = u.shape[0]
+        if (u.ndim == 2 and u.shape != (nsp, 3)) or (u.ndim == 3 and u.shape != (nsp, nsp, 4)):
+            raise ValueError("u must be (nsp, 3) (1D Euler) or (nsp, nsp, 4) (2D Euler)")
+        a = np.asfortranarray(u)
+        w = np.asfortranarray(weights, dtype=np.float64)
+        fn = L.fr_positive_limiter_euler1d if u.ndim == 2 else L.fr_positive_limiter_euler2d
+        check(fn(dptr(a), nsp, float(g), dptr(w), dptr(_f(ll)), dptr(_f(lr)), t0))
+    else:
+        raise ValueError("u must be an element of 1, 2 or 3 dimensions")
+    if a is not u:
+        u[...] = a
+    return None
+
+
 # ------------------------------------------------------------------ metrics (src/tools.jl:43-53)
 def L1_error(u, ue, dx):
     return float(np.sum(np.abs(np.asarray(u) - np.asarray(ue)) * dx))

@@ -92,6 +92,21 @@ int fr_rs_jacobi_quad(double r, double s, const double *vert, double *J);
 int fr_rs_jacobi_mesh(int ncx, int ncy, const double *vert, int n, const double *r, double *J, double *iJ);
 /* Persson-Peraire switch; returns 1 (shock) / 0 (src/dissipation.jl:13-23) */
 int fr_shock_detector(double Se, int deg, double S0, double kappa);
+/* positive_limiter of src/dissipation.jl on one element, in place, on the host (the reference's
+ * per-element routine; fr1d_positivity / fr2d_positivity run it over device states).  Returns
+ * FR_E_LIMITER, leaving u unchanged, where the reference's @assert 0 < t <= 1 fails.
+ * scalar (:37-58): u(n), um = sum(u w) / wsum (wsum = sum(weights), or the weight itself when
+ * the reference passes one number);
+ * euler1d (:61-123): u(nsp, 3) column-major, weights summing to one;
+ * euler2d (:125-206): u(nsp, nsp, 4) column-major, weights(nsp, nsp) summing to one (wp / 4);
+ * the pressure squeeze takes min(minimum(tj), t0) (the reference's minimum(tj, t0) raises a
+ * MethodError) and the closed-form root of tj_equation (:208-215) in place of its Newton solve. */
+int fr_positive_limiter_scalar(double *u, int n, const double *weights, double wsum, const double *ll,
+                               const double *lr);
+int fr_positive_limiter_euler1d(double *u, int nsp, double gamma, const double *weights, const double *ll,
+                                const double *lr, double t0);
+int fr_positive_limiter_euler2d(double *u, int nsp, double gamma, const double *weights, const double *ll,
+                                const double *lr, double t0);
 /* filter_exp(N, s, V, Nc, invV) (src/Polynomial/poly_filter.jl:11-24): the nodal exponential
  * filter F = V diag(sigma) invV (nv x nv, column-major) with sigma of KitBase filter_exp1d for a
  * line basis (nv = N+1) or filter_exp2d (poly_filter.jl:32-47) for a triangle basis
