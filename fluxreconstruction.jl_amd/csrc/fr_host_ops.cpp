@@ -353,6 +353,12 @@ int fr_dlagrange(const double *sp, int nsp, double *lpdm) {
   return FR_OK;
 }
 
+int fr_dlegendre(int p, const double *x, int n, double *out) {
+  FR_REQUIRE(p >= 0 && n >= 0 && (n == 0 || (x && out)), FR_E_INVALID, "fr_dlegendre: bad arguments");
+  for (int i = 0; i < n; ++i) out[i] = dlegendre(p, x[i]);
+  return FR_OK;
+}
+
 int fr_dcorrection(int kind, int deg, const double *x, int n, double *dgl, double *dgr) {
   FR_REQUIRE(kind >= FR_CORR_RADAU && kind <= FR_CORR_HUYNH, FR_E_INVALID, "fr_dcorrection: unknown kind");
   FR_REQUIRE(deg >= 1 && x && dgl && dgr && n > 0, FR_E_INVALID, "fr_dcorrection: bad arguments");

@@ -18,6 +18,7 @@ from .space import (  # noqa: F401
     lagrange_point,
     dlagrange,
     standard_lagrange,
+    dlegendre,
     dradau,
     dsd,
     dhuynh,

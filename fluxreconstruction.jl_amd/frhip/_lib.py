@@ -84,6 +84,7 @@ SIGNATURES = {
     "fr_legendre_point": (C.c_int, [C.c_int, _dp, _dp]),
     "fr_lagrange_point": (C.c_int, [_dp, C.c_int, C.c_double, _dp]),
     "fr_dlagrange": (C.c_int, [_dp, C.c_int, _dp]),
+    "fr_dlegendre": (C.c_int, [C.c_int, _dp, C.c_int, _dp]),
     "fr_dcorrection": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp]),
     "fr_jacobi_p": (C.c_int, [_dp, C.c_int, C.c_double, C.c_double, C.c_int, _dp]),
     "fr_djacobi_p": (C.c_int, [_dp, C.c_int, C.c_double, C.c_double, C.c_int, _dp]),

@@ -64,6 +64,15 @@ def standard_lagrange(x):
     return lagrange_point(x, -1.0), lagrange_point(x, 1.0), dlagrange(x)
 
 
+def dlegendre(p: int, x):
+    """∂legendre(p, x) (src/Polynomial/poly_legendre.jl:13-22): P_p'(x) at a point or an array."""
+    scalar = np.ndim(x) == 0
+    xs = np.ascontiguousarray(np.atleast_1d(np.asarray(x, dtype=np.float64)).ravel())
+    out = np.zeros_like(xs)
+    check(_lib.lib().fr_dlegendre(int(p), dptr(xs), xs.size, dptr(out)))
+    return float(out[0]) if scalar else out.reshape(np.shape(x))
+
+
 def _dcorr(kind: str, p: int, x):
     x = _f(np.atleast_1d(x))
     dgl = np.zeros_like(x)

@@ -70,6 +70,9 @@ int fr_legendre_point(int deg, double *r, double *w);
 int fr_lagrange_point(const double *sp, int nsp, double x, double *l);
 /* lpdm(m,k) = l_k'(sp_m) (src/Polynomial/poly_lagrange.jl:38-59) */
 int fr_dlagrange(const double *sp, int nsp, double *lpdm);
+/* the derivative of the Legendre polynomial P_p at each of the n points x (∂legendre,
+ * src/Polynomial/poly_legendre.jl:13-22; the last entry of GSL's sf_legendre_Pl_deriv_array) */
+int fr_dlegendre(int p, const double *x, int n, double *out);
 /* correction-function derivatives dgl, dgr at x (src/Polynomial/poly_legendre.jl:29-71) */
 int fr_dcorrection(int kind, int deg, const double *x, int n, double *dgl, double *dgr);
 /* orthonormal Jacobi P_N^(a,b) at n points (src/Polynomial/poly_jacobi.jl:7-83,88-107) */

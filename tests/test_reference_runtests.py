@@ -103,3 +103,15 @@ def test_global_sp_and_r_x():
     assert xp.shape == (5, 4, 4, 4)
     assert np.allclose(xp, ps2.xpg[1:-1, 1:-1, :, :, 0], rtol=0, atol=1e-14)
     assert np.allclose(yp, ps2.xpg[1:-1, 1:-1, :, :, 1], rtol=0, atol=1e-14)
+
+
+def test_dlegendre_closed_forms():
+    """∂legendre (poly_legendre.jl:13-22) against numpy's Legendre derivative at interior points
+    and the end values P_p'(+-1) = (+-1)^(p+1) p (p + 1) / 2."""
+    x = np.linspace(-1.0, 1.0, 41)
+    for p in range(0, 10):
+        ref = np.polynomial.legendre.legval(x, np.polynomial.legendre.legder(np.eye(p + 1)[p]))
+        got = frhip.dlegendre(p, x)
+        assert np.allclose(got, ref, rtol=1e-13, atol=1e-13), p
+        assert frhip.dlegendre(p, 1.0) == p * (p + 1) / 2
+        assert frhip.dlegendre(p, -1.0) == (-1.0) ** (p + 1) * p * (p + 1) / 2
