@@ -149,6 +149,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default: the full run to t = 2)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--stagger", type=int, default=None,
+                    help="fix the stage launches' staggered-start delay (units of 512 clocks) instead of the "
+                         "library's device timing, e.g. under a profiler, whose dispatch handling skews that timing")
     ap.add_argument("--no-extras", action="store_true", help="only the headline 2D Euler measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 10 plain device copies of 64 MiB (reference kernel for HBM counters)")
@@ -173,6 +176,8 @@ def main():
 
     stream = torch.cuda.Stream()
     solver = frhip.FREuler2D(ps, GAMMA, "rusanov", "periodic")
+    if args.stagger is not None:
+        solver.set_stagger(args.stagger)
     solver.use_torch_stream(stream)
     solver.set_state(u0)
     # The timed steps replay a hipGraph of CHUNK SSP-RK3 steps that the context captured and

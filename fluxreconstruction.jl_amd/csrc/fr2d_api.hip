@@ -441,6 +441,18 @@ int fr2d_get_stagger(const fr2d_ctx *c, int *stagger) {
   return FR_OK;
 }
 
+int fr2d_set_stagger(fr2d_ctx *c, int stagger) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr2d_set_stagger: null context");
+  FR_REQUIRE(stagger >= -1 && stagger <= 16, FR_E_INVALID, "fr2d_set_stagger: stagger must be in -1..16");
+  // a captured step graph holds the delay of its capture
+  if (c->gexec && stagger != c->stagger) {
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+  }
+  c->stagger = stagger;
+  return FR_OK;
+}
+
 int fr2d_step_dev_noalloc(fr2d_ctx *c, double *d_u, int scheme, double dt, int nsteps) {
   FR_REQUIRE(c && d_u, FR_E_INVALID, "fr2d_step_dev_noalloc: null argument");
   FR_REQUIRE(nsteps >= 0 && std::isfinite(dt), FR_E_INVALID, "fr2d_step_dev_noalloc: bad dt / nsteps");

@@ -115,6 +115,7 @@ SIGNATURES = {
     "fr2d_step_graph": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int]),
     "fr2d_step_dev_noalloc": (C.c_int, [_vp, _vp, C.c_int, C.c_double, C.c_int]),
     "fr2d_get_stagger": (C.c_int, [_vp, C.POINTER(C.c_int)]),
+    "fr2d_set_stagger": (C.c_int, [_vp, C.c_int]),
     "fr2d_fill_ghost": (C.c_int, [_vp, C.c_int]),
     "fr2d_sensor_filter": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_double, _ip]),
     "fr2d_positivity": (C.c_int, [_vp]),

@@ -199,6 +199,11 @@ class FREuler2D:
         check(_lib.lib().fr2d_get_stagger(self.h, C.byref(v)))
         return v.value
 
+    def set_stagger(self, stagger: int) -> None:
+        """Fix the staggered-start delay (0..16; 0 off) instead of the first stepping call's device
+        timing, or (-1) have the next stepping call time the candidates again."""
+        check(_lib.lib().fr2d_set_stagger(self.h, int(stagger)))
+
     def step(self, dt: float, nsteps: int = 1, scheme: str = "ssprk3"):
         check(_lib.lib().fr2d_step(self.h, _lib.RK[scheme], float(dt), int(nsteps)))
 

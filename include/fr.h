@@ -195,6 +195,11 @@ int fr2d_step_graph(fr2d_ctx *ctx, int scheme, double dt, int nsteps, int nrep);
  * resident at once start the workgroups sharing a CU one after another, so that their load phases
  * do not coincide; the delay changes the timing only, never the results. */
 int fr2d_get_stagger(const fr2d_ctx *ctx, int *stagger);
+/* No reference counterpart (launch tuning of this library): fixes the staggered-start delay (0..16
+ * units; 0 off) instead of the device timing of the first stepping call, or (-1) asks the next
+ * stepping call to time the candidates again.  For runs whose timing of the candidates is not
+ * representative, such as runs under a profiler; only the timing changes, never the results. */
+int fr2d_set_stagger(fr2d_ctx *ctx, int stagger);
 /* No counterpart in the reference: fr2d_step for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  nsteps steps
  * of FR_RK_SSPRK3, FR_RK_MIDPOINT or FR_RK_EULER on the caller's device state d_u (reference

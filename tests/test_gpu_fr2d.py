@@ -191,6 +191,22 @@ def test_staggered_start_tuned_and_results_unchanged():
     s2.set_state(u2)
     s2.step(dt, 1)
     assert s2.stagger() == 0
+    # a fixed delay replaces the tuning and gives the same bits as no delay, eager and replayed
+    got = {}
+    for d in (0, 5):
+        fx = frhip.FREuler2D(ps, 1.4, "rusanov", "periodic")
+        fx.set_stagger(d)
+        fx.set_state(u)
+        fx.step(dt, 3)
+        assert fx.stagger() == d
+        got[d] = fx.get_state()
+        fx.set_state(u)
+        fx.step_graph(dt, 3, 1)
+        assert np.array_equal(fx.get_state(), got[d])
+        fx.close()
+    assert np.array_equal(got[0], got[5])
+    with pytest.raises(frhip.FRError):
+        s.set_stagger(17)
 
 
 @pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler", "tsit5"])
