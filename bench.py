@@ -120,6 +120,13 @@ def bench_polar():
     return workloads.polar_stage(256, 20)
 
 
+def bench_parallelogram():
+    """The p = 3 stage on 256 x 256 uniform parallelogram cells (frhip.workloads)."""
+    from frhip import workloads
+
+    return workloads.parallelogram_stage(256, 20)
+
+
 def bench_grids(sizes=(1024, 2048), steps=60):
     """The same vortex stage on larger periodic grids (frhip.workloads.vortex_grid)."""
     from frhip import workloads
@@ -312,7 +319,8 @@ def main():
     if not args.no_extras:
         for key, fn in (("vortex_larger_grids", bench_grids), ("ensemble_riemann512", bench_ensemble),
                         ("bgk_128sq_p2_v24", bench_bgk), ("bgk1d_8192_v256", bench_bgk1d),
-                        ("polar_256sq_p3", bench_polar), ("sensor_shock_vortex", bench_sensor)):
+                        ("polar_256sq_p3", bench_polar), ("parallelogram_256sq_p3", bench_parallelogram),
+                        ("sensor_shock_vortex", bench_sensor)):
             try:
                 out[key] = fn()
             except Exception as exc:  # an extra leg must not hide the headline line

@@ -147,6 +147,47 @@ def polar_stage(n=256, steps=20):
             "alg_GBs": (64.0 / 3.0 + 8.0) * dof / (us * 1e-6) / 1e9}
 
 
+def parallelogram_stage(n=256, steps=20, shear=0.4, stagger=None):
+    """Uniform parallelogram cells (fr2d_set_geometry; dev/parallelogram.jl's face frames): the
+    p = 3 SSP-RK3 stage on n x n cells of [0, 10]^2 sheared by `shear` (edges a = (dx, 0),
+    b = (shear dx, dy), x faces in the frame of grad r, y faces in that of grad s), Rusanov,
+    periodic, the isentropic vortex; 20 graph-replayed steps after 300 eager ones, HIP events on the
+    launch stream.  Algorithmic bytes: 21.33 per DOF and stage (the metric is two scalars).
+    stagger: a fixed staggered-start delay instead of the device timing (fr2d_set_stagger)."""
+    import torch
+
+    ps = FRPSpace2D(0.0, 10.0, n, 0.0, 10.0, n, 3, 1, 1)
+    u0 = ics.isentropic_vortex(ps, 1.4, strength=5.0)
+    dx, dy = 10.0 / n, 10.0 / n
+    iJ = np.linalg.inv(np.array([[dx / 2, shear * dx / 2], [0.0, dy / 2]]))
+    s = FREuler2D(ps, 1.4, "rusanov", "periodic")
+    s.set_geometry(iJ)
+    if stagger is not None:
+        s.set_stagger(stagger)
+    st = torch.cuda.Stream()
+    s.use_torch_stream(st)
+    s.set_state(u0)
+    dt = 0.02 * dx
+    with torch.cuda.stream(st):
+        s.step_graph(dt, steps, 1)
+        s.set_state(u0)
+        s.step(dt, 300)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        s.step_graph(dt, steps, 1)
+        e1.record(st)
+        torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / (3 * steps)
+    dof = n * n * 16 * 4
+    s.check_state()
+    stag = s.stagger()
+    s.close()
+    return {"grid": f"parallelogram {n}x{n} (shear {shear})", "dof": dof, "stage_us": us,
+            "dof_stage_per_s": dof / (us * 1e-6), "alg_GBs": (64.0 / 3.0) * dof / (us * 1e-6) / 1e9, "stagger": stag}
+
+
 def shock_vortex_loop(mode="modal", steps=600):
     """`steps` iterations of the shock-vortex host loop (example/shock-vortex.jl:296-326) on the
     512 x 256, p = 3 case: positivity, SSP-RK3, sensor + filter, ghost refresh."""
@@ -203,11 +244,13 @@ def vortex_grid(n=1024, steps=60, flux="rusanov"):
 
 def main():
     ap = argparse.ArgumentParser(description="run one measured workload and print one JSON line")
-    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "bgk1d", "bgk1d64", "polar", "shock_vortex",
-                                         "vortex_grid"])
-    ap.add_argument("--n", type=int, default=None, help="vortex_grid (1024) / polar (256): cells per side")
+    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "bgk1d", "bgk1d64", "polar", "parallelogram",
+                                         "shock_vortex", "vortex_grid"])
+    ap.add_argument("--n", type=int, default=None, help="vortex_grid (1024) / polar, parallelogram (256): cells per side")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--mode", default="modal", choices=["modal", "learned", "off"])
+    ap.add_argument("--stagger", type=int, default=None,
+                    help="parallelogram: fixed staggered-start delay (e.g. under a profiler)")
     a = ap.parse_args()
     if a.workload == "ensemble":
         r = riemann_ensemble(a.steps or 10)
@@ -221,6 +264,8 @@ def main():
         r = bgk1d(a.steps or 4, members=64)
     elif a.workload == "polar":
         r = polar_stage(a.n or 256, a.steps or 20)
+    elif a.workload == "parallelogram":
+        r = parallelogram_stage(a.n or 256, a.steps or 20, stagger=a.stagger)
     elif a.workload == "vortex_grid":
         r = vortex_grid(a.n or 1024, a.steps or 60)
     else:
