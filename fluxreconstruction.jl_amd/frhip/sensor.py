@@ -198,23 +198,24 @@ def shock_vortex_solver(nx, ny, deg=3, gamma=1.4, flux="hll", **ic):
 
 
 def run_shock_vortex(s: FREuler2D, dt, nsteps, mode="modal", lam=SHOCK_VORTEX["lam"], kappa=SHOCK_VORTEX["kappa"], callback=None,
-                     every=0, fused=True, native=False):
+                     every=0, fused=True, native=False, scheme="ssprk3"):
     """Host loop of example/shock-vortex.jl:296-326 on the device state: positivity limiter on
     every cell (:298-303), SSP-RK3 step, sensor + filter (mode "modal" / "learned" / "off",
     :306-321), ghost refresh of the bottom, top and right sides with the inflow column kept
     (:322-325).  fused: the filter, the ghost refresh and the next iteration's limiter run as one
     pass over the state (fr2d_filter_refresh_limit, the same arithmetic); the last iteration ends
     with the filter and the ghost refresh.  native: the whole fused loop is enqueued by the
-    library (fr2d_run_host_loop, no callback) with one synchronisation at the end."""
+    library (fr2d_run_host_loop, no callback) with one synchronisation at the end.  scheme: the
+    stepper of every iteration (SSP-RK3 by default; the script itself steps with Midpoint)."""
     S0 = -3.0 * np.log10(s.ps.deg)
     if native:
         if callback is not None or not fused:
             raise ValueError("run_shock_vortex: the native loop is the fused loop without callbacks")
-        s.run_host_loop(dt, nsteps, mode, lam, S0, kappa, "inflow_left")
+        s.run_host_loop(dt, nsteps, mode, lam, S0, kappa, "inflow_left", scheme=scheme)
         return
     s.positivity()
     for n in range(nsteps):
-        s.step(dt, 1, "ssprk3")
+        s.step(dt, 1, scheme)
         if fused and n + 1 < nsteps:
             s.filter_refresh_limit(mode, lam, S0, kappa, "inflow_left")
         else:

@@ -106,9 +106,11 @@ def test_solver_features_and_learned_filter():
     assert np.max(np.abs(ug - uc)) <= 1e-13 * np.max(np.abs(uc))
 
 
-@pytest.mark.parametrize("bc,iters", [("ghost", 12), ("ghost", 7), ("periodic", 5)])
+@pytest.mark.parametrize("bc,iters,scheme", [("ghost", 12, "ssprk3"), ("ghost", 7, "ssprk3"),
+                                             ("periodic", 5, "ssprk3"), ("ghost", 5, "tsit5"),
+                                             ("periodic", 4, "midpoint")])
 @pytest.mark.parametrize("mode", ["off", "modal", "learned"])
-def test_fused_filter_refresh_limit_equals_host_loop(mode, bc, iters):
+def test_fused_filter_refresh_limit_equals_host_loop(mode, bc, iters, scheme):
     """fr2d_filter_refresh_limit (filter, inflow-keeping ghost refresh and the next limiter in one
     pass) and the natively enqueued loop fr2d_run_host_loop reproduce the unfused host loop of
     example/shock-vortex.jl:296-326 bitwise, for odd and even iteration counts and on periodic
@@ -129,7 +131,7 @@ def test_fused_filter_refresh_limit_equals_host_loop(mode, bc, iters):
         s.set_state(u0)
         if mlp is not None:
             S.attach(s, mlp)
-        S.run_shock_vortex(s, dt, iters, mode, fused=fused, native=native)
+        S.run_shock_vortex(s, dt, iters, mode, fused=fused, native=native, scheme=scheme)
         out.append(s.get_state())
         s.close()
     assert np.all(np.isfinite(out[0]))
