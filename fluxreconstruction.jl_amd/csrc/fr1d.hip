@@ -533,9 +533,9 @@ struct Batch1D {
     return FR_OK;
   }
 
-  // opts (SSP-RK3 only): FR_ENS_FILTER runs the sensor + filter on the state before every step
-  // (example/euler_highlevel.jl:37-54), FR_ENS_LIMIT the positivity limiter on the input of every
-  // stage (the state before the step and the two intermediate stage results)
+  // opts (SSP-RK3 and Midpoint): FR_ENS_FILTER runs the sensor + filter on the state before every
+  // step (example/euler_highlevel.jl:37-54), FR_ENS_LIMIT the positivity limiter on the input of
+  // every stage (the state before the step and the intermediate stage results)
   int enqueue_steps(int scheme, int mode, double dt, int nsteps, hipStream_t st, int opts = 0) {
     double *U = d_u, *S1 = d_s1, *S2 = d_s2, *A = d_acc;
     auto stg = [&](const double *u0, const double *us, double *out, double a0, double a1, double b,
@@ -571,7 +571,10 @@ struct Batch1D {
           // consecutive steps alternate the state between U and S2; the call's last step writes U
           double *const X = (n & 1) ? S2 : U;
           double *const out = n == nsteps - 1 ? U : ((n & 1) ? U : S2);
+          if (filt && (rc = diss_launch(X, 1, dt, mode, st))) return rc;
+          if (lim && (rc = diss_launch(X, 2, 0.0, mode, st))) return rc;
           if ((rc = stg(X, X, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0))) return rc;
+          if (lim && (rc = diss_launch(S1, 2, 0.0, mode, st))) return rc;
           if ((rc = stg(X, S1, out, 1.0, 0.0, dt, nullptr, nullptr, 0.0))) return rc;
           break;
         }
@@ -811,19 +814,22 @@ static int ens_read_counts(fr_ens_ctx *c, const char *who, int *n_flagged, int *
 int fr_ens_run(fr_ens_ctx *c, int mode, int opts, double dt, int nsteps, int *n_flagged, int *n_limited) {
   FR_REQUIRE(c && (mode == 0 || mode == 1) && nsteps >= 0 && std::isfinite(dt) && dt >= 0.0, FR_E_INVALID,
              "fr_ens_run: bad arguments");
-  FR_REQUIRE((opts & ~(FR_ENS_FILTER | FR_ENS_LIMIT)) == 0, FR_E_INVALID, "fr_ens_run: unknown option bits");
+  FR_REQUIRE((opts & ~(FR_ENS_FILTER | FR_ENS_LIMIT | FR_ENS_MIDPOINT)) == 0, FR_E_INVALID,
+             "fr_ens_run: unknown option bits");
   int rc = c->ensure_stage_buffers(false);
   if (rc) return rc;
   if (!c->d_count) FR_HIP(hipMalloc(&c->d_count, 2 * sizeof(int)));
   FR_HIP(hipMemsetAsync(c->d_count, 0, 2 * sizeof(int), c->stream));
-  if ((rc = c->enqueue_steps(FR_RK_SSPRK3, mode, dt, nsteps, c->stream, opts))) return rc;
+  const int scheme = (opts & FR_ENS_MIDPOINT) ? FR_RK_MIDPOINT : FR_RK_SSPRK3;
+  if ((rc = c->enqueue_steps(scheme, mode, dt, nsteps, c->stream, opts))) return rc;
   return ens_read_counts(c, "fr_ens_run", n_flagged, n_limited);
 }
 
 int fr_ens_run_graph(fr_ens_ctx *c, int opts, double dt, int nsteps, int nrep, int *n_flagged, int *n_limited) {
   FR_REQUIRE(c && nsteps >= 1 && nrep >= 0 && std::isfinite(dt) && dt >= 0.0, FR_E_INVALID,
              "fr_ens_run_graph: bad arguments");
-  FR_REQUIRE((opts & ~(FR_ENS_FILTER | FR_ENS_LIMIT)) == 0, FR_E_INVALID, "fr_ens_run_graph: unknown option bits");
+  FR_REQUIRE((opts & ~(FR_ENS_FILTER | FR_ENS_LIMIT | FR_ENS_MIDPOINT)) == 0, FR_E_INVALID,
+             "fr_ens_run_graph: unknown option bits");
   int rc = c->ensure_stage_buffers(false);
   if (rc) return rc;
   if (!c->d_count) FR_HIP(hipMalloc(&c->d_count, 2 * sizeof(int)));
@@ -838,7 +844,7 @@ int fr_ens_run_graph(fr_ens_ctx *c, int opts, double dt, int nsteps, int nrep, i
     }
     hipGraph_t graph = nullptr;
     FR_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    rc = c->enqueue_steps(FR_RK_SSPRK3, 0, dt, nsteps, c->stream, opts);
+    rc = c->enqueue_steps((opts & FR_ENS_MIDPOINT) ? FR_RK_MIDPOINT : FR_RK_SSPRK3, 0, dt, nsteps, c->stream, opts);
     hipError_t e = hipStreamEndCapture(c->stream, &graph);
     if (rc) {
       if (graph) (void)hipGraphDestroy(graph);

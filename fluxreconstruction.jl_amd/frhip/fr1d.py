@@ -42,7 +42,13 @@ _lib.SIGNATURES.update({
     "fr_ens_run_graph": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int, _ip, _ip]),
 })
 
-ENS_FILTER, ENS_LIMIT = 1, 2
+ENS_FILTER, ENS_LIMIT, ENS_MIDPOINT = 1, 2, 4
+
+
+def _opts(filter, limit, scheme):
+    if scheme not in ("ssprk3", "midpoint"):
+        raise ValueError("ensemble loops step with ssprk3 or midpoint")
+    return (ENS_FILTER if filter else 0) | (ENS_LIMIT if limit else 0) | (ENS_MIDPOINT if scheme == "midpoint" else 0)
 
 
 def _bind():
@@ -205,21 +211,24 @@ class Ensemble1D:
     def step_graph(self, dt: float, nsteps: int, nrep: int = 1):
         check(_lib.lib().fr_ens_step_graph(self.h, float(dt), int(nsteps), int(nrep)))
 
-    def run(self, dt: float, nsteps: int, filter: bool = True, limit: bool = False, looped: bool = False):
+    def run(self, dt: float, nsteps: int, filter: bool = True, limit: bool = False, looped: bool = False,
+            scheme: str = "ssprk3"):
         """The reference's per-step 1D loop (example/euler_highlevel.jl:37-54) over every member:
-        sensor + filter before every SSP-RK3 step (filter), positivity limiter on the input of every
-        stage (limit).  Returns (filtered cells, limited cells) summed over the loop."""
+        sensor + filter before every step (filter), positivity limiter on the input of every stage
+        (limit); steps of SSP-RK3 or of the script's own Midpoint (scheme "midpoint").  Returns
+        (filtered cells, limited cells) summed over the loop."""
         nf, nl = C.c_int(0), C.c_int(0)
-        opts = (ENS_FILTER if filter else 0) | (ENS_LIMIT if limit else 0)
+        opts = _opts(filter, limit, scheme)
         check(_lib.lib().fr_ens_run(self.h, 1 if looped else 0, opts, float(dt), int(nsteps), C.byref(nf),
                                     C.byref(nl)))
         return nf.value, nl.value
 
-    def run_graph(self, dt: float, nsteps: int, nrep: int = 1, filter: bool = True, limit: bool = False):
+    def run_graph(self, dt: float, nsteps: int, nrep: int = 1, filter: bool = True, limit: bool = False,
+                  scheme: str = "ssprk3"):
         """run() of nsteps steps captured once in a hipGraph (kept by the context) and replayed nrep
         times; nrep = 0 only captures."""
         nf, nl = C.c_int(0), C.c_int(0)
-        opts = (ENS_FILTER if filter else 0) | (ENS_LIMIT if limit else 0)
+        opts = _opts(filter, limit, scheme)
         check(_lib.lib().fr_ens_run_graph(self.h, opts, float(dt), int(nsteps), int(nrep), C.byref(nf), C.byref(nl)))
         return nf.value, nl.value
 

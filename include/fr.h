@@ -366,14 +366,14 @@ int fr_ens_get_state(fr_ens_ctx *ctx, double *u);
 int fr_ens_step(fr_ens_ctx *ctx, int mode, double dt, int nsteps);
 /* capture nsteps batched steps in a hipGraph and replay nrep times (fr_ens_run_graph, no options) */
 int fr_ens_step_graph(fr_ens_ctx *ctx, double dt, int nsteps, int nrep);
-/* The per-step host loop of example/euler_highlevel.jl:37-54 over every member, with SSP-RK3 in
- * place of Midpoint: FR_ENS_FILTER runs fr1d_sensor_filter's sensor + filter (filter time step dt)
- * on the state before every step; FR_ENS_LIMIT runs positive_limiter (src/dissipation.jl:61-123)
- * on the input of every stage.  mode 0: one batched launch per pass; 1: one launch per member per
- * pass (looped baseline).  n_flagged / n_limited (optional) sum the filtered / limited cells over
- * the loop.  A cell whose mean density is not positive is left unchanged and reported as
- * FR_E_LIMITER after the loop. */
-enum { FR_ENS_FILTER = 1, FR_ENS_LIMIT = 2 };
+/* The per-step host loop of example/euler_highlevel.jl:37-54 over every member, with SSP-RK3 steps
+ * or, with FR_ENS_MIDPOINT, the script's own Midpoint steps: FR_ENS_FILTER runs
+ * fr1d_sensor_filter's sensor + filter (filter time step dt) on the state before every step;
+ * FR_ENS_LIMIT runs positive_limiter (src/dissipation.jl:61-123) on the input of every stage.
+ * mode 0: one batched launch per pass; 1: one launch per member per pass (looped baseline).
+ * n_flagged / n_limited (optional) sum the filtered / limited cells over the loop.  A cell whose
+ * mean density is not positive is left unchanged and reported as FR_E_LIMITER after the loop. */
+enum { FR_ENS_FILTER = 1, FR_ENS_LIMIT = 2, FR_ENS_MIDPOINT = 4 };
 int fr_ens_run(fr_ens_ctx *ctx, int mode, int opts, double dt, int nsteps, int *n_flagged, int *n_limited);
 /* the batched loop of nsteps steps captured in a hipGraph (kept while opts, dt, nsteps and the
  * work mapping are unchanged, so later calls only replay it) and replayed nrep times */

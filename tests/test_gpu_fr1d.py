@@ -109,6 +109,49 @@ def test_euler_highlevel_sod_degree7():
         assert np.array_equal(single.get_state(), e.member(out, b)), b
 
 
+def test_ensemble_midpoint_loop_is_the_script_loop():
+    """FR_ENS_MIDPOINT: the batched loop with the script's own Midpoint steps (sensor + filter
+    before every step, example/euler_highlevel.jl:37-54) over euler_highlevel.jl's Sod member
+    (100 cells, degree 7, gamma 5/3) and a degree-2 member equals, member by member and bitwise,
+    the single-problem loop of sensor_filter + one Midpoint step (which the Sod test above pins
+    against the oracle); the captured loop equals the eager one; and limiting every stage input
+    leaves these admissible states unchanged."""
+    ncell, deg, g = 100, 7, 5.0 / 3.0
+    ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
+    xc = (np.arange(ncell) + 0.5) / ncell
+    left = np.repeat((xc <= 0.5)[:, None], deg + 1, axis=1)
+    prim = np.stack([np.where(left, 1.0, 0.125), np.zeros(left.shape), np.where(left, 0.5, 0.625)], -1)
+    u0 = np.asfortranarray(ics.prim_conserve1(prim, g))
+    ps2 = frhip.FRPSpace1D(0.0, 1.0, 64, 2)
+    u2 = np.asfortranarray(euler_state(64, 2, seed=3, gamma=g))
+    dt, n = 0.01 / ncell, 150
+    out = {}
+    for graph in (False, True):
+        e = frhip.Ensemble1D([ncell, 64], [deg, 2], [1.0 / ncell, 1.0 / 64], g, "hll", "dirichlet")
+        e.set_state(e.pack([u0, u2]))
+        nf = (e.run_graph(dt, n, 1, filter=True, scheme="midpoint") if graph
+              else e.run(dt, n, filter=True, scheme="midpoint"))[0]
+        out[graph] = (e.get_state(), nf)
+        e.close()
+    assert np.array_equal(out[False][0], out[True][0]) and out[False][1] == out[True][1] > 0
+    nf_single = 0
+    e = frhip.Ensemble1D([ncell, 64], [deg, 2], [1.0 / ncell, 1.0 / 64], g, "hll", "dirichlet")
+    for b, (pp, uu) in enumerate(((ps, u0), (ps2, u2))):
+        single = frhip.FREulerProblem(pp, g, "dirichlet", "hll")
+        single.set_state(uu)
+        for _ in range(n):
+            nf_single += single.sensor_filter(dt)
+            single.step(dt, 1, "midpoint")
+        assert np.array_equal(single.get_state(), e.member(out[False][0], b)), b
+    assert nf_single == out[False][1]
+    e.set_state(e.pack([u0, u2]))
+    e.run(dt, n, filter=True, limit=True, scheme="midpoint")
+    assert np.array_equal(e.get_state(), out[False][0])
+    with pytest.raises(ValueError):
+        e.run(dt, 1, scheme="rk4")
+    e.close()
+
+
 def test_c1_advection_rk4_to_t1():
     """1D linear advection, 100 elements, p = 2, Radau, upwind, u0 = sin(2 pi x), RK4 at
     dt = 0.1 dx (1000 steps) to t = 1; GPU vs oracle steps and vs the exact solution."""
