@@ -76,5 +76,6 @@ def test_noalloc_refuses_oversize_and_rk4():
     ps2, u2, s2 = _vortex(16, 8, "periodic", seed=3)
     ud = torch.from_numpy(u2.ravel(order="F").copy()).cuda()
     assert L.fr2d_step_dev_noalloc(s2.h, C.c_void_p(ud.data_ptr()), _lib.RK["rk4"], 1e-3, 1) == 5
+    assert L.fr2d_step_dev_noalloc(s2.h, C.c_void_p(ud.data_ptr()), _lib.RK["tsit5"], 1e-3, 1) == 5
     assert np.array_equal(ud.cpu().numpy(), u2.ravel(order="F"))
     s2.close()
