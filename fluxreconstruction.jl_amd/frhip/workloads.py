@@ -188,6 +188,43 @@ def parallelogram_stage(n=256, steps=20, shear=0.4, stagger=None):
             "dof_stage_per_s": dof / (us * 1e-6), "alg_GBs": (64.0 / 3.0) * dof / (us * 1e-6) / 1e9, "stagger": stag}
 
 
+def ops_rows(nrow=1 << 20, n=4, nbatch=16, reps=20):
+    """interp_face! and poly_derivative! (src/interpolate.jl:1-10, src/derivative.jl:1-10) on device
+    batches f(row, q, b) of nrow x n x nbatch doubles (the layout of a Julia Array{Float64,3}; 4 x
+    16 = the 64 planes of a p = 3 2D state contracted along one point index), `reps` calls each,
+    HIP events on the launch stream.  Algorithmic bytes per call: 8 (n + nout) per (row, batch)."""
+    import torch
+
+    from . import ops
+
+    r, _ = np.polynomial.legendre.leggauss(n)
+    ll = np.array([np.prod([(-1.0 - r[m]) / (r[q] - r[m]) for m in range(n) if m != q]) for q in range(n)])
+    lr = np.array([np.prod([(1.0 - r[m]) / (r[q] - r[m]) for m in range(n) if m != q]) for q in range(n)])
+    pdm = np.asfortranarray(np.random.default_rng(0).standard_normal((n, n)))
+    f = torch.rand((nbatch, n, nrow), dtype=torch.float64, device="cuda")
+    ff = torch.empty((nbatch, 2, nrow), dtype=torch.float64, device="cuda")
+    df = torch.empty_like(f)
+    out = {}
+    for name, call, nout, dst in (("interp_face", lambda: ops.interp_face(ff, f, ll, lr), 2, ff),
+                                  ("poly_derivative", lambda: ops.poly_derivative(df, f, pdm), n, df)):
+        for _ in range(3):
+            call()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(20_000_000)  # hold the stream while the calls are queued
+        e0.record()
+        for _ in range(reps):
+            call()
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        nbytes = 8.0 * (n + nout) * nrow * nbatch
+        out[name] = {"us_per_call": us, "alg_GBs": nbytes / (us * 1e-6) / 1e9, "bytes": nbytes}
+    out["shape"] = [nrow, n, nbatch]
+    return out
+
+
 def shock_vortex_loop(mode="modal", steps=600):
     """`steps` iterations of the shock-vortex host loop (example/shock-vortex.jl:296-326) on the
     512 x 256, p = 3 case: positivity, SSP-RK3, sensor + filter, ghost refresh."""
@@ -245,7 +282,7 @@ def vortex_grid(n=1024, steps=60, flux="rusanov"):
 def main():
     ap = argparse.ArgumentParser(description="run one measured workload and print one JSON line")
     ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "bgk1d", "bgk1d64", "polar", "parallelogram",
-                                         "shock_vortex", "vortex_grid"])
+                                         "ops", "shock_vortex", "vortex_grid"])
     ap.add_argument("--n", type=int, default=None, help="vortex_grid (1024) / polar, parallelogram (256): cells per side")
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--mode", default="modal", choices=["modal", "learned", "off"])
@@ -264,6 +301,8 @@ def main():
         r = bgk1d(a.steps or 4, members=64)
     elif a.workload == "polar":
         r = polar_stage(a.n or 256, a.steps or 20)
+    elif a.workload == "ops":
+        r = ops_rows()
     elif a.workload == "parallelogram":
         r = parallelogram_stage(a.n or 256, a.steps or 20, stagger=a.stagger)
     elif a.workload == "vortex_grid":
