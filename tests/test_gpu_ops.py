@@ -51,6 +51,37 @@ def test_interp_face_batches_and_polynomials(n):
     assert np.max(np.abs(gg - ends)) <= 1e-9 * np.max(np.abs(ends))
 
 
+@pytest.mark.parametrize("n", [3, 4, 9])
+def test_row_pairs_and_single_rows_agree(n):
+    """An even row count on 16-byte aligned device arrays takes the two-rows-per-thread kernel
+    (16-byte loads and stores); the same data one double off alignment takes the one-row kernel.
+    Both give the same bits, and the numpy contraction to round-off."""
+    import torch
+
+    rng = np.random.default_rng(40 + n)
+    nrow, nb = 20012, 5
+    f = np.asfortranarray(rng.standard_normal((nrow, n, nb)))
+    pdm = np.asfortranarray(rng.standard_normal((n, n)))
+    flat = torch.from_numpy(f.ravel(order="F").copy()).cuda()
+    buf = torch.zeros(flat.numel() + 1, dtype=torch.float64, device="cuda")
+    buf[1:] = flat
+    shifted = buf[1:]  # 8 bytes past a 16-byte boundary
+    assert flat.data_ptr() % 16 == 0 and shifted.data_ptr() % 16 == 8
+    outs = []
+    for src in (flat, shifted):
+        d = torch.empty(nrow * n * nb, dtype=torch.float64, device="cuda")
+        frhip.poly_derivative(d.view(nb, n, nrow), src.view(nb, n, nrow), pdm)
+        ff = torch.empty(nrow * 2 * nb, dtype=torch.float64, device="cuda")
+        frhip.interp_face(ff.view(nb, 2, nrow), src.view(nb, n, nrow), pdm[0], pdm[1])
+        torch.cuda.synchronize()
+        outs.append((d.cpu().numpy(), ff.cpu().numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    ref = np.einsum("iqb,pq->ipb", f, pdm).ravel(order="F")
+    assert rel_inf(outs[0][0], ref) <= 1e-13
+    reff = np.stack([np.einsum("iqb,q->ib", f, pdm[0]), np.einsum("iqb,q->ib", f, pdm[1])], axis=1).ravel(order="F")
+    assert rel_inf(outs[0][1], reff) <= 1e-13
+
+
 @pytest.mark.parametrize("deg", [1, 2, 3, 4, 6])
 def test_poly_derivative_state_and_exactness(deg):
     ops = O.Ops(deg)
