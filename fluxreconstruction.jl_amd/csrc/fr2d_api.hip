@@ -338,12 +338,22 @@ int fr2d_euler_rhs(fr2d_ctx *c, const double *u, double *du, double t) {
   return FR_OK;
 }
 
+// Host states cross the bus as one contiguous copy of the reference layout through the staging
+// array d_in; the pitch changes on the device.  (A pitched copy between pageable host memory and
+// the padded array moves one row of nx+2 doubles at a time: 117 ms for a 256 x 256, p = 3 state.)
+static int ensure_staging(Fr2dCtx &c) {
+  if (!c.d_in) FR_HIP(hipMalloc(&c.d_in, c.len * sizeof(double)));
+  return FR_OK;
+}
+
 int fr2d_set_state(fr2d_ctx *c, const double *u) {
   FR_REQUIRE(c && u, FR_E_INVALID, "fr2d_set_state: null argument");
   int rc = ensure_state(*c);
+  if (!rc) rc = ensure_staging(*c);
   if (rc) return rc;
   hipStream_t st = stream_of(*c);
-  if ((rc = copy_to_padded(*c, c->d_u, u, hipMemcpyHostToDevice, st))) return rc;
+  FR_HIP(hipMemcpyAsync(c->d_in, u, c->len * sizeof(double), hipMemcpyHostToDevice, st));
+  if ((rc = copy_to_padded(*c, c->d_u, c->d_in, hipMemcpyDeviceToDevice, st))) return rc;
   FR_HIP(hipStreamSynchronize(st));
   return FR_OK;
 }
@@ -351,9 +361,11 @@ int fr2d_set_state(fr2d_ctx *c, const double *u) {
 int fr2d_get_state(fr2d_ctx *c, double *u) {
   FR_REQUIRE(c && u, FR_E_INVALID, "fr2d_get_state: null argument");
   FR_REQUIRE(c->d_u, FR_E_INVALID, "fr2d_get_state: no state");
-  hipStream_t st = stream_of(*c);
-  int rc = copy_from_padded(*c, u, c->d_u, hipMemcpyDeviceToHost, st);
+  int rc = ensure_staging(*c);
   if (rc) return rc;
+  hipStream_t st = stream_of(*c);
+  if ((rc = copy_from_padded(*c, c->d_in, c->d_u, hipMemcpyDeviceToDevice, st))) return rc;
+  FR_HIP(hipMemcpyAsync(u, c->d_in, c->len * sizeof(double), hipMemcpyDeviceToHost, st));
   FR_HIP(hipStreamSynchronize(st));
   return FR_OK;
 }
@@ -655,7 +667,7 @@ int fr2d_set_metrics(fr2d_ctx *c, const double *iJ) {
     c->gexec = nullptr;
   }
   FR_HIP(hipMemcpyAsync(c->d_mj_ref, iJ, c->len * sizeof(double), hipMemcpyHostToDevice, st));
-  int rc = copy_to_padded(*c, c->d_mj_pad, iJ, hipMemcpyHostToDevice, st);
+  int rc = copy_to_padded(*c, c->d_mj_pad, c->d_mj_ref, hipMemcpyDeviceToDevice, st);
   if (rc) return rc;
   FR_HIP(hipStreamSynchronize(st));
   c->geom = 2;
