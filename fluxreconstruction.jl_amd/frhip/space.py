@@ -394,9 +394,12 @@ class _Forward:
 
 
 class FRPSpace1D(_Forward):
-    """1D FR space (src/struct.jl:13-88): J, xpl, xpg, wp, dl, ll, lr, dll, dlr, dhl, dhr, V, iV."""
+    """1D FR space (src/struct.jl:13-88): J, xpl, xpg, wp, dl, ll, lr, dll, dlr, dhl, dhr, V, iV.
+    Positional or keyword arguments; other keywords are ignored, as the reference's keyword
+    constructor FRPSpace1D(; x0, x1, nx, deg, ng, correction, kwargs...) (struct.jl:87-88) ignores
+    them (example/advection_lowlevel.jl:51 passes its whole configuration)."""
 
-    def __init__(self, x0, x1, nx, deg, ng=0, correction="radau"):
+    def __init__(self, x0, x1, nx, deg, ng=0, correction="radau", **kwargs):
         self.base = PSpace1D(x0, x1, nx, ng)
         self.deg = int(deg)
         self.np = deg + 1

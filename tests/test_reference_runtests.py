@@ -115,3 +115,12 @@ def test_dlegendre_closed_forms():
         assert np.allclose(got, ref, rtol=1e-13, atol=1e-13), p
         assert frhip.dlegendre(p, 1.0) == p * (p + 1) / 2
         assert frhip.dlegendre(p, -1.0) == (-1.0) ** (p + 1) * p * (p + 1) / 2
+
+
+def test_frpspace1d_keyword_constructor():
+    """FRPSpace1D(; cfg...) of example/advection_lowlevel.jl:49-51 (extra keys ignored, as the
+    reference's keyword method, struct.jl:87-88, does)."""
+    cfg = dict(x0=-1, x1=1, nx=100, deg=2, cfl=0.05, t=0.0, a=1.0)
+    a = frhip.FRPSpace1D(**cfg)
+    b = frhip.FRPSpace1D(-1, 1, 100, 2)
+    assert np.array_equal(a.xpg, b.xpg) and np.array_equal(a.dl, b.dl) and a.nx == 100
