@@ -430,9 +430,11 @@ class FRPSpace2D(_Forward):
     as (nx+2g, ny+2g, 2, 2) views; J_elem / iJ_elem).  Over any other mesh they are per solution
     point as in the reference, J[i, j, k, l] = rs_jacobi(r[k], r[l], vertices[i, j]) with iJ its
     inverse, arrays of shape (nx+2g, ny+2g, n, n, 2, 2), and Ji the Jacobians the reference forms
-    at its interface points (struct.jl:144-158)."""
+    at its interface points (struct.jl:144-158).  Keyword construction ignores other keywords, as
+    the reference's FRPSpace2D(; x0, x1, nx, y0, y1, ny, deg, ngx, ngy, kwargs...) does
+    (struct.jl:244-245)."""
 
-    def __init__(self, x0, x1=None, nx=None, y0=None, y1=None, ny=None, deg=None, ngx=0, ngy=0):
+    def __init__(self, x0, x1=None, nx=None, y0=None, y1=None, ny=None, deg=None, ngx=0, ngy=0, **kwargs):
         if hasattr(x0, "vertices"):  # FRPSpace2D(base, deg)
             base, deg = x0, x1
         else:

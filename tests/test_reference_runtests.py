@@ -124,3 +124,10 @@ def test_frpspace1d_keyword_constructor():
     a = frhip.FRPSpace1D(**cfg)
     b = frhip.FRPSpace1D(-1, 1, 100, 2)
     assert np.array_equal(a.xpg, b.xpg) and np.array_equal(a.dl, b.dl) and a.nx == 100
+
+
+def test_frpspace2d_keyword_constructor():
+    cfg = dict(x0=0.0, x1=1.0, nx=20, y0=0.0, y1=1.0, ny=10, deg=3, ngx=1, ngy=1, cfl=0.1, gamma=1.4)
+    a = frhip.FRPSpace2D(**cfg)
+    b = frhip.FRPSpace2D(0.0, 1.0, 20, 0.0, 1.0, 10, 3, 1, 1)
+    assert np.array_equal(a.xpg, b.xpg) and np.array_equal(a.iJ, b.iJ)
