@@ -147,14 +147,37 @@ class FR1D:
         return n.value
 
 
-def FREulerProblem(ps: FRPSpace1D, gamma: float, bc: str = "dirichlet", flux: str = "hll") -> FR1D:
-    """src/Equation/eq_euler.jl:1-27 (bc :dirichlet or :period)."""
-    return FR1D(ps, "euler", gamma=gamma, flux=flux, bc=bc)
+def _problem(args, kwargs, eq, param, default_bc):
+    """(ps, param[, bc]) or the reference's (u, tspan, ps, param, bc): the latter sets u as the
+    device state and keeps tspan (the reference returns ODEProblem(frode!, u, tspan, p))."""
+    u = tspan = None
+    if args and not isinstance(args[0], FRPSpace1D):
+        u, tspan, *args = args
+    ps, value, *rest = args
+    bc = str(rest[0] if rest else kwargs.pop("bc", default_bc)).lstrip(":")
+    opts = {param: value, "bc": bc}
+    if eq == "euler":
+        opts["flux"] = rest[1] if len(rest) > 1 else kwargs.pop("flux", "hll")
+    if kwargs:
+        raise TypeError(f"unexpected arguments {sorted(kwargs)}")
+    pr = FR1D(ps, eq, **opts)
+    if u is not None:
+        pr.set_state(u)
+        pr.tspan = tuple(tspan)
+    return pr
 
 
-def FRAdvectionProblem(ps: FRPSpace1D, a: float, bc: str = "period") -> FR1D:
-    """src/Equation/eq_advection.jl:1-26."""
-    return FR1D(ps, "advection", a=a, bc=bc)
+def FREulerProblem(*args, **kwargs) -> FR1D:
+    """src/Equation/eq_euler.jl:1-27: FREulerProblem(ps, gamma[, bc[, flux]]) or the reference's
+    FREulerProblem(u, tspan, ps, gamma, bc) (bc "dirichlet" / "period", a leading ":" allowed),
+    which also makes u the device state."""
+    return _problem(args, kwargs, "euler", "gamma", "dirichlet")
+
+
+def FRAdvectionProblem(*args, **kwargs) -> FR1D:
+    """src/Equation/eq_advection.jl:1-26: FRAdvectionProblem(ps, a[, bc]) or the reference's
+    FRAdvectionProblem(u, tspan, ps, a, bc), which also makes u the device state."""
+    return _problem(args, kwargs, "advection", "a", "period")
 
 
 class Ensemble1D:

@@ -616,8 +616,9 @@ def test_advection_highlevel_as_written():
     dt = 0.1 * dx
     nt = int(2.0 / dt)
     assert nt == 160
-    pr = frhip.FRAdvectionProblem(ps, 1.0, "period")
-    pr.set_state(u0)
+    # the script's own call, FR.FRAdvectionProblem(u, tspan, ps, a, bc), which sets the state
+    pr = frhip.FRAdvectionProblem(u0, (0.0, 2.0), ps, 1.0, ":period")
+    assert pr.tspan == (0.0, 2.0) and np.array_equal(pr.get_state(), u0)
     pr.step(dt, nt, "tsit5")
     u = pr.get_state()
     ops = oracle_ops(deg)

@@ -131,3 +131,16 @@ def test_frpspace2d_keyword_constructor():
     a = frhip.FRPSpace2D(**cfg)
     b = frhip.FRPSpace2D(0.0, 1.0, 20, 0.0, 1.0, 10, 3, 1, 1)
     assert np.array_equal(a.xpg, b.xpg) and np.array_equal(a.iJ, b.iJ)
+
+
+def test_problem_constructors_take_both_forms():
+    """FREulerProblem / FRAdvectionProblem accept frhip's (ps, param, bc[, flux]) and the reference's
+    (u, tspan, ps, param, bc) (eq_euler.jl:1, eq_advection.jl:1); argument errors are raised before
+    any device call."""
+    import pytest
+
+    ps = frhip.FRPSpace1D(0.0, 1.0, 10, 2)
+    with pytest.raises(TypeError):
+        frhip.FREulerProblem(ps, 1.4, "dirichlet", "hll", speed=1)
+    with pytest.raises(ValueError):
+        frhip.FREulerProblem(np.zeros((10, 3, 3)), (0.0, 1.0))  # no space
