@@ -452,6 +452,17 @@ int fr_shock_detector(double Se, int deg, double S0, double kappa) {
   return sigma < 0.99 ? 1 : 0;
 }
 
+int fr_modal_filter_l2(double *u, int n, double lambda) {
+  FR_REQUIRE(u && n >= 0 && lambda >= 0.0 && std::isfinite(lambda), FR_E_INVALID,
+             "fr_modal_filter_l2: bad arguments");
+  // mode k (0-based) of the modal vector divided by 1 + lambda (k+1)^2 k^2; mode 0 is kept
+  for (int k = 1; k < n; ++k) {
+    const double kk = (double)k * (double)(k + 1);
+    u[k] /= 1.0 + lambda * kk * kk;
+  }
+  return FR_OK;
+}
+
 int fr_positive_limiter_scalar(double *u, int n, const double *weights, double wsum, const double *ll,
                                const double *lr) {
   FR_REQUIRE(u && weights && ll && lr && n >= 1 && wsum != 0.0, FR_E_INVALID, "fr_positive_limiter_scalar: bad arguments");

@@ -31,6 +31,7 @@ from .space import (  # noqa: F401
     r_x,
     global_sp,
     positive_limiter,
+    modal_filter,
     basis_norm,
     shock_detector,
     L1_error,

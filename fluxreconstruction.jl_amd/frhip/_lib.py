@@ -96,6 +96,7 @@ SIGNATURES = {
     "fr_rs_jacobi_mesh": (C.c_int, [C.c_int, C.c_int, _dp, C.c_int, _dp, _dp, _dp]),
     "fr_shock_detector": (C.c_int, [C.c_double, C.c_int, C.c_double, C.c_double]),
     "fr_positive_limiter_scalar": (C.c_int, [_dp, C.c_int, _dp, C.c_double, _dp, _dp]),
+    "fr_modal_filter_l2": (C.c_int, [_dp, C.c_int, C.c_double]),
     "fr_positive_limiter_euler1d": (C.c_int, [_dp, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double]),
     "fr_positive_limiter_euler2d": (C.c_int, [_dp, C.c_int, C.c_double, _dp, _dp, _dp, C.c_double]),
     "fr2d_create": (C.c_int, [C.POINTER(_vp), C.POINTER(Fr2dDesc)]),

@@ -206,6 +206,21 @@ def shock_detector(Se: float, deg: int, S0: float | None = None, kappa: float = 
     return bool(_lib.lib().fr_shock_detector(float(Se), int(deg), float(S0), float(kappa)))
 
 
+def modal_filter(u, lam: float, filter: str = "l2"):
+    """KitBase's modal_filter!(u, lambda; filter = :l2) on a modal vector, in place (the call of
+    example/shock-vortex.jl:316 and example/euler_highlevel.jl:45 on iV * u of an element): mode
+    k (0-based) is divided by 1 + lambda (k+1)^2 k^2.  Only the :l2 filter is built."""
+    if str(filter).lstrip(":") != "l2":
+        raise ValueError("modal_filter: only filter = l2 is built")
+    if not isinstance(u, np.ndarray) or u.dtype != np.float64 or u.ndim != 1:
+        raise ValueError("u must be a float64 numpy vector (filtered in place)")
+    a = np.ascontiguousarray(u)
+    check(_lib.lib().fr_modal_filter_l2(dptr(a), a.size, float(lam)))
+    if a is not u:
+        u[...] = a
+    return None
+
+
 def positive_limiter(u, *args):
     """positive_limiter of src/dissipation.jl on one element, in place (the reference's three
     methods, chosen by u's dimension like Julia's dispatch), computed by libfrhip's native host

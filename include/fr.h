@@ -106,6 +106,12 @@ int fr_shock_detector(double Se, int deg, double S0, double kappa);
  * MethodError) and the closed-form root of tj_equation (:208-215) in place of its Newton solve. */
 int fr_positive_limiter_scalar(double *u, int n, const double *weights, double wsum, const double *ll,
                                const double *lr);
+/* KitBase's modal_filter!(u, lambda; filter = :l2) on one modal vector u(n), in place, on the host
+ * (called by example/shock-vortex.jl:316 and example/euler_highlevel.jl:45 on the modes iV * u of
+ * an element): mode k (0-based) is divided by 1 + lambda (k+1)^2 k^2.  The same damping as the
+ * device filters of fr2d_sensor_filter / fr1d_sensor_filter.  KitBase is not part of the
+ * reference tree, so the coefficients follow its published form, not its code. */
+int fr_modal_filter_l2(double *u, int n, double lambda);
 int fr_positive_limiter_euler1d(double *u, int nsp, double gamma, const double *weights, const double *ll,
                                 const double *lr, double t0);
 int fr_positive_limiter_euler2d(double *u, int nsp, double gamma, const double *weights, const double *ll,

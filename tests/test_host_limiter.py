@@ -110,3 +110,26 @@ def test_non_positive_mean_raises_and_keeps_the_element():
     with pytest.raises(frhip.FRError):
         frhip.positive_limiter(u, ops.w / 2.0, ops.ll, ops.lr)
     assert np.array_equal(u, -np.ones(4))
+
+
+def test_modal_filter_l2():
+    """modal_filter (KitBase modal_filter! :l2 as the scripts call it) against the oracle's
+    restatement, and what the form fixes: mode 0 kept, lambda = 0 the identity, every higher mode
+    damped by 1 / (1 + lambda (k+1)^2 k^2), the factor falling with k."""
+    rng = np.random.default_rng(5)
+    for n in (3, 9, 16, 36):
+        u = rng.standard_normal(n)
+        for lam in (0.0, 5e-4, 0.3):
+            got = u.copy()
+            frhip.modal_filter(got, lam)
+            assert np.allclose(got, O.modal_filter_l2(u, lam), rtol=1e-15, atol=0)
+            assert got[0] == u[0]
+            if lam == 0.0:
+                assert np.array_equal(got, u)
+        k = np.arange(n)
+        ratio = np.ones(n)
+        frhip.modal_filter(ratio, 0.3)
+        assert np.allclose(ratio, 1.0 / (1.0 + 0.3 * ((k + 1) * k) ** 2), rtol=1e-15, atol=0)
+        assert np.all(np.diff(ratio) < 0)
+    with pytest.raises(ValueError):
+        frhip.modal_filter(np.ones(4), 1e-3, filter="exp")
