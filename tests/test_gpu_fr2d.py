@@ -53,6 +53,28 @@ def test_residual_parity_shapes(nx, ny):
     assert rel_inf(s.rhs(u), orc.rhs(u)) <= RES_TOL
 
 
+def test_residual_and_stage_parity_random_shapes():
+    """Seeded sweep over grid shapes (1-53 x 1-21 cells: full, partial and single tiles in both
+    directions), degrees 1-5, the three fluxes and both boundary modes: the residual (drop-in,
+    reference layout) and one SSP-RK3 step (padded stage arrays) against the oracle, 24 draws."""
+    rng = np.random.default_rng(2027)
+    for draw in range(24):
+        nx, ny = int(rng.integers(1, 54)), int(rng.integers(1, 22))
+        deg = int(rng.integers(1, 6))
+        flux = ["hll", "rusanov", "roe"][draw % 3]
+        bc = ["ghost", "periodic"][(draw // 3) % 2]
+        ps, u, s, orc = setup(nx, ny, deg, flux, bc, seed=draw)
+        case = (nx, ny, deg, flux, bc)
+        assert rel_inf(s.rhs(u), orc.rhs(u)) <= RES_TOL, case
+        dt = 0.02 * min(ps.dx0, 10.0 / ny) / 4.0
+        s.set_state(u)
+        s.step(dt, 1, "ssprk3")
+        uc = u.copy(order="F")
+        orc.ssprk3(uc, dt, 1)
+        assert rel_l2(s.get_state(), uc) <= STATE_TOL, case
+        s.close()
+
+
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_wide_grid_strip_order(bc):
     """1100 x 6: 69 tiles across, so the workgroups take column strips of 16 tiles with a last
