@@ -75,6 +75,32 @@ def test_residual_and_stage_parity_random_shapes():
         s.close()
 
 
+def test_general_geometry_parity_random_shapes():
+    """The same kind of seeded sweep for the two general-geometry kernels: sheared parallelogram
+    cells (one 2x2 metric, face frames) and polar meshes (a metric per solution point), shapes
+    1-45 x 1-17, degrees 1-5, fluxes and boundary modes; residual against the oracle."""
+    from test_oracle_metrics import G, oracle_for, polar_space, smooth_state
+
+    rng = np.random.default_rng(2028)
+    for draw in range(16):
+        nx, ny = int(rng.integers(1, 46)), int(rng.integers(1, 18))
+        deg = int(rng.integers(1, 6))
+        flux = ["hll", "rusanov", "roe"][draw % 3]
+        bc = ["ghost", "periodic"][(draw // 3) % 2]
+        case = (nx, ny, deg, flux, bc)
+        ps, u, s, _ = setup(nx, ny, deg, flux, bc, seed=100 + draw)
+        dx, dy, iJ, n1, n2 = _sheared_geometry(nx, ny, shear=0.2 + 0.05 * draw)
+        s.set_geometry(iJ, n1, n2)
+        orc = O.Fr2d(nx, ny, deg, dx, dy, 1.4, flux, bc, iJ=iJ, n1=n1, n2=n2)
+        assert rel_inf(s.rhs(u), orc.rhs(u)) <= RES_TOL, ("parallelogram",) + case
+        s.close()
+        pp = polar_space(nx, ny, deg)
+        up = smooth_state(pp, seed=draw)
+        sp = frhip.FREuler2D(pp, G, flux, bc)
+        assert rel_inf(sp.rhs(up), oracle_for(pp, flux, bc).rhs(up)) <= RES_TOL, ("polar",) + case
+        sp.close()
+
+
 @pytest.mark.parametrize("bc", ["ghost", "periodic"])
 def test_wide_grid_strip_order(bc):
     """1100 x 6: 69 tiles across, so the workgroups take column strips of 16 tiles with a last
