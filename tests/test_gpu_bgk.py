@@ -58,3 +58,26 @@ def test_bgk_ensemble_stepping_parity():
         mass = float((rho * wpt[None, :]).sum())
         assert abs(mass - 4.0 * nx * ny) <= 1e-10 * 4.0 * nx * ny
     assert w.shape == (len(taus), 4, npts)
+
+
+def test_bgk_residual_parity_random_shapes():
+    """Seeded sweep: grids 1-40 x 1-40 (single and partial 16 x 16 tiles), degrees 1-4, 2-10
+    velocities per direction (unequal counts), one or three members; residual against the oracle."""
+    rng = np.random.default_rng(2029)
+    for draw in range(14):
+        nx, ny = int(rng.integers(1, 41)), int(rng.integers(1, 41))
+        deg = int(rng.integers(1, 5))
+        nu, nv = int(rng.integers(2, 11)), int(rng.integers(2, 11))
+        nmem = 1 if draw % 2 == 0 else 3
+        u, wu = bgk.velocity_grid(nu)
+        v, wv = bgk.velocity_grid(nv)
+        taus = [1e-2] if nmem == 1 else [1e-3, 1e-2, 1e-1]
+        f = bgk.maxwellian_state(nx, ny, deg, u, v, amp=0.1, nmem=nmem)
+        f = f * (1.0 + 0.02 * rng.uniform(-1, 1, size=f.size))
+        g = bgk.BGK2D(nx, ny, deg, u, wu, v, wv, taus if nmem > 1 else taus[0])
+        df = g.rhs(f).reshape(nmem, -1)
+        m = f.size // nmem
+        for i, tau in enumerate(taus):
+            ref = O.bgk2d_rhs(f[i * m:(i + 1) * m], nx, ny, deg, u, wu, v, wv, tau)
+            case = (nx, ny, deg, nu, nv, nmem, i)
+            assert np.max(np.abs(df[i] - ref.ravel())) <= 1e-12 * np.max(np.abs(ref)), case

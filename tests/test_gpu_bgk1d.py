@@ -115,6 +115,31 @@ def test_moments_launch_shapes_agree_bitwise():
     one.close()
 
 
+def test_residual_parity_random_shapes():
+    """Seeded sweep: 1-300 cells, degrees 1-8, 2-70 velocities (fewer than the moments kernel's
+    eight slices too), one to five members with their own tau; residual against the oracle."""
+    rng = np.random.default_rng(2030)
+    for draw in range(16):
+        nx, deg, nu = int(rng.integers(1, 301)), int(rng.integers(1, 9)), int(rng.integers(2, 71))
+        nm = int(rng.integers(1, 6))
+        ps = frhip.FRPSpace1D(0.0, 1.0, nx, deg)
+        vs = bgk.VSpace1D(-5.0, 5.0, nu)
+        f0 = frhip.ics.perturbed(bgk.bgk_wave_initial(ps, vs), seed=draw, amplitude=0.05)
+        taus = list(10.0 ** rng.uniform(-3, -1, size=nm))
+        g = bgk.BGK1D(ps, vs, tau=taus if nm > 1 else taus[0])
+        flat = np.concatenate([frhip.ics.perturbed(f0, seed=100 * draw + m, amplitude=0.02).ravel(order="F")
+                               for m in range(nm)])
+        du = g.rhs(flat)
+        du = du.reshape(f0.shape + ((nm,) if nm > 1 else ()), order="F")
+        n1 = f0.size
+        for m in range(nm):
+            fm = flat[m * n1:(m + 1) * n1].reshape(f0.shape, order="F")
+            ref = O.Bgk1d(nx, deg, vs.u, vs.weights, 1.0 / nx, tau=taus[m]).rhs(fm)
+            got = du[..., m] if nm > 1 else du
+            assert rel_inf(got, ref) <= RES_TOL, (nx, deg, nu, nm, m)
+        g.close()
+
+
 def test_mol_on_device_arrays_side_stream():
     import torch
 
