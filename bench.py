@@ -106,11 +106,13 @@ def bench_bgk(steps=3, ens_steps=1):
 
 def bench_bgk1d():
     """The 1D BGK model of example/bgk_wave.jl as written, 8192 cells x 256 velocities p = 2, one
-    member and 64 members (frhip.workloads)."""
+    member and 64 members, and the script's own 20-cell run to t = 1 eager and graph-replayed
+    (frhip.workloads)."""
     from frhip import workloads
 
     return {"workload": "1D BGK (bgk_wave.jl) 8192 cells x 256 velocities, p=2, SSP-RK3", "unit": "values·RK-stage/s",
-            "single": workloads.bgk1d(20), "ensemble64": workloads.bgk1d(4, members=64)}
+            "single": workloads.bgk1d(20), "ensemble64": workloads.bgk1d(4, members=64),
+            "script_case_20x100_midpoint_1000_steps": workloads.bgk1d_script()}
 
 
 def bench_polar():

@@ -48,6 +48,9 @@ _lib.SIGNATURES.update({
     "fr_bgk1d_set_state": (C.c_int, [_vp, _dp]),
     "fr_bgk1d_get_state": (C.c_int, [_vp, _dp]),
     "fr_bgk1d_step": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int]),
+    "fr_bgk1d_step_graph": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int, C.c_int]),
+    "fr_bgk1d_set_path": (C.c_int, [_vp, C.c_int]),
+    "fr_bgk1d_get_path": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "fr_bgk1d_moments": (C.c_int, [_vp, _dp]),
 })
 
@@ -58,7 +61,8 @@ def _bind():
                  "fr_bgk_set_state_dev",
                  "fr_bgk_get_state", "fr_bgk_step", "fr_bgk_moments", "fr_gauss_hermite",
                  "fr_bgk1d_create", "fr_bgk1d_destroy", "fr_bgk1d_state_len", "fr_bgk1d_rhs", "fr_bgk1d_rhs_dev",
-                 "fr_bgk1d_set_state", "fr_bgk1d_get_state", "fr_bgk1d_step", "fr_bgk1d_moments"):
+                 "fr_bgk1d_set_state", "fr_bgk1d_get_state", "fr_bgk1d_step", "fr_bgk1d_step_graph", "fr_bgk1d_set_path",
+                 "fr_bgk1d_get_path", "fr_bgk1d_moments"):
         res, args = _lib.SIGNATURES[name]
         fn = getattr(L, name)
         fn.restype = res
@@ -264,6 +268,30 @@ class BGK1D:
 
     def step(self, dt, nsteps=1, scheme="midpoint"):
         check(_lib.lib().fr_bgk1d_step(self.h, _lib.RK[scheme], float(dt), int(nsteps)))
+
+    PATHS = {"auto": 0, "stream": 1, "resident": 2}
+
+    def set_path(self, path):
+        """'stream' (two launches per stage over HBM), 'resident' (a workgroup per member holding
+        it in LDS for up to 256 steps per launch; FRError at step time if a member does not fit)
+        or 'auto' (resident whenever it fits).  Every path gives the same bits."""
+        check(_lib.lib().fr_bgk1d_set_path(self.h, self.PATHS[path]))
+
+    def path(self, scheme="midpoint"):
+        """The path step() takes for `scheme` under the current setting."""
+        v = C.c_int()
+        check(_lib.lib().fr_bgk1d_get_path(self.h, _lib.RK[scheme], C.byref(v)))
+        return {1: "stream", 2: "resident"}[v.value]
+
+    def step_graph(self, dt, nsteps, scheme="midpoint", chunk=100):
+        """nsteps steps as replays of a captured graph of `chunk` steps (kept by the context while
+        scheme, dt and chunk are unchanged) and the remainder as eager steps: the bits of step()."""
+        chunk = max(1, min(int(chunk), int(nsteps))) if nsteps > 0 else 1
+        nrep, rest = divmod(int(nsteps), chunk)
+        if nrep:
+            check(_lib.lib().fr_bgk1d_step_graph(self.h, _lib.RK[scheme], float(dt), chunk, nrep))
+        if rest:
+            self.step(dt, rest, scheme)
 
     def moments(self):
         """(rho, rho U, E) of the device state at every point: shape (nmem, 3, nx, nsp)."""

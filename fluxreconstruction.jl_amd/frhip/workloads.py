@@ -111,6 +111,58 @@ def bgk1d(steps=20, members=1, nx=8192, nu=256):
             "alg_GBs": vals * (8 + 64 / 3) / stage / 1e9, "steps": steps}
 
 
+def bgk1d_script(steps=1000):
+    """example/bgk_wave.jl as the script runs it: 20 cells, p = 2, 100 velocities on [-5, 5],
+    tau = 1e-2, Midpoint with dt = cfl dx / u1 = 1e-3 for 1000 steps (t = 1).  Wall time of the
+    whole run from the same initial state on the streaming path (two launches per stage) eagerly and
+    as replays of a captured 100-step graph, and on the LDS-resident path (one launch per 256
+    steps, the default for this size); all three end in the same bits."""
+    ps = FRPSpace1D(0.0, 1.0, 20, 2)
+    vs = VSpace1D(-5.0, 5.0, 100)
+    f0 = bgk_wave_initial(ps, vs)
+    g = BGK1D(ps, vs)
+    dt = 0.1 * (1.0 / 20) / 5.0
+    out, states = {}, []
+    for mode in ("stream", "stream_graph", "resident"):
+        g.set_path("resident" if mode == "resident" else "stream")
+        g.set_state(f0)
+        g.step(dt, 100, "midpoint")  # warm-up (and the graph's capture)
+        g.step_graph(dt, 100, "midpoint", chunk=100)
+        g.set_state(f0)
+        t0 = time.perf_counter()
+        if mode == "stream_graph":
+            g.step_graph(dt, steps, "midpoint", chunk=100)
+        else:
+            g.step(dt, steps, "midpoint")
+        el = time.perf_counter() - t0
+        out[mode] = {"run_ms": el * 1e3, "stage_us": el / (2 * steps) * 1e6}
+        states.append(g.get_state())
+    g.close()
+    # a tau ensemble of 256 such members (tau log-uniform in [3e-3, 1e-1]; Midpoint at this dt
+    # breaks down below tau = 1.5e-3): one resident workgroup per member against the streaming
+    # launches over all members
+    taus = list(np.exp(np.random.default_rng(5).uniform(np.log(3e-3), np.log(1e-1), 256)))
+    e = BGK1D(ps, vs, tau=taus)
+    ens, est = {}, []
+    for path in ("stream", "resident"):
+        e.set_path(path)
+        e.set_state(np.tile(f0.ravel(order="F"), len(taus)))
+        e.step(dt, 20, "midpoint")
+        t0 = time.perf_counter()
+        e.step(dt, 200, "midpoint")
+        el = time.perf_counter() - t0
+        ens[path] = {"stage_us": el / 400 * 1e6, "values_stage_per_s": e.len * 400 / el}
+        est.append(e.get_state())
+    e.close()
+    return {"steps": steps, "values": int(f0.size), **out,
+            "resident_speedup": out["stream"]["run_ms"] / out["resident"]["run_ms"],
+            "bitwise_equal": bool(all(np.array_equal(states[0], x) for x in states[1:])),
+            "ensemble256_200_steps": {**ens, "resident_speedup": ens["resident"]["values_stage_per_s"] /
+                                      ens["stream"]["values_stage_per_s"],
+                                      "bitwise_equal": bool(np.array_equal(est[0], est[1])),
+                                      "finite": bool(np.all(np.isfinite(est[0])))}}
+
+
 def polar_stage(n=256, steps=20):
     """Cells with point-varying metrics: the p = 3 SSP-RK3 stage on the polar mesh
     CSpace2D(1, 6, n, 0, pi, n, 1, 1) (fr2d_set_metrics), Rusanov, ghost layer as data, a smooth
@@ -281,7 +333,7 @@ def vortex_grid(n=1024, steps=60, flux="rusanov"):
 
 def main():
     ap = argparse.ArgumentParser(description="run one measured workload and print one JSON line")
-    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "bgk1d", "bgk1d64", "polar", "parallelogram",
+    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "bgk1d", "bgk1d64", "bgk1d_script", "polar", "parallelogram",
                                          "ops", "shock_vortex", "vortex_grid"])
     ap.add_argument("--n", type=int, default=None, help="vortex_grid (1024) / polar, parallelogram (256): cells per side")
     ap.add_argument("--steps", type=int, default=None)
@@ -297,6 +349,8 @@ def main():
         r = bgk(a.steps or 1, members=64)
     elif a.workload == "bgk1d":
         r = bgk1d(a.steps or 20)
+    elif a.workload == "bgk1d_script":
+        r = bgk1d_script(a.steps or 1000)
     elif a.workload == "bgk1d64":
         r = bgk1d(a.steps or 4, members=64)
     elif a.workload == "polar":

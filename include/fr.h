@@ -421,6 +421,23 @@ int fr_bgk1d_get_state(fr_bgk1d_ctx *ctx, double *f);
 /* nsteps steps of the device state: FR_RK_SSPRK3, FR_RK_MIDPOINT (the script's Midpoint()) or
  * FR_RK_EULER */
 int fr_bgk1d_step(fr_bgk1d_ctx *ctx, int scheme, double dt, int nsteps);
+/* nrep replays of a hipGraph of nsteps such steps, captured on the first call and kept while
+ * scheme, dt and nsteps are unchanged: the bits of fr_bgk1d_step over nrep * nsteps steps, for the
+ * launch-bound small cases (the script's 20 cells x 100 velocities) */
+int fr_bgk1d_step_graph(fr_bgk1d_ctx *ctx, int scheme, double dt, int nsteps, int nrep);
+/* Execution path of fr_bgk1d_step / fr_bgk1d_step_graph (same bits on every path):
+ * FR_BGK1D_PATH_STREAM: two launches per stage (velocity sums, transport/relaxation) over the
+ * whole state in HBM; FR_BGK1D_PATH_RESIDENT: one launch per up to 256 steps, a workgroup per
+ * member holding the member's state, stage arrays and point data in LDS (members of up to about
+ * 6,800 values under SSP-RK3, 10,000 under midpoint / Euler, at 160 KiB of LDS; a step on a larger
+ * member returns FR_E_UNSUPPORTED); FR_BGK1D_PATH_AUTO (the default): resident whenever it fits.
+ * No counterpart in the reference: the path is this library's execution choice. */
+#define FR_BGK1D_PATH_AUTO 0
+#define FR_BGK1D_PATH_STREAM 1
+#define FR_BGK1D_PATH_RESIDENT 2
+int fr_bgk1d_set_path(fr_bgk1d_ctx *ctx, int path);
+/* the path fr_bgk1d_step takes for this scheme under the current setting */
+int fr_bgk1d_get_path(const fr_bgk1d_ctx *ctx, int scheme, int *path);
 /* (rho, rho U, E) of the device state at every point: w[member][c][i + nx k] */
 int fr_bgk1d_moments(fr_bgk1d_ctx *ctx, double *w);
 

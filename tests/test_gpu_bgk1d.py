@@ -140,6 +140,120 @@ def test_residual_parity_random_shapes():
         g.close()
 
 
+@pytest.mark.parametrize("scheme,chunk", [("midpoint", 4), ("ssprk3", 5), ("euler", 3), ("euler", 4)])
+def test_step_graph_equals_eager_steps(scheme, chunk):
+    """Graph replays give the bits of eager steps: 13 steps as replays of a captured graph of
+    `chunk` steps plus eager remainder steps (Euler with an odd chunk ends a replay in the second
+    buffer and copies back inside the graph); the kept graph is replayed again after eager work
+    and checked against the eager run, and a new dt is captured afresh."""
+    ps, vs, f0 = script_case(nx=12, deg=2, nu=40)
+    f = frhip.ics.perturbed(f0, seed=5, amplitude=0.05)
+    dt = 1e-3
+    a, b = bgk.BGK1D(ps, vs), bgk.BGK1D(ps, vs)
+    a.set_state(f)
+    b.set_state(f)
+    a.step(dt, 13, scheme)
+    b.step_graph(dt, 13, scheme, chunk=chunk)
+    assert np.array_equal(a.get_state(), b.get_state())
+    for rep in range(3):  # the kept graph, with eager steps of both contexts in between
+        a.step(dt, 1, scheme)
+        b.step(dt, 1, scheme)
+        a.step(dt, 2 * chunk, scheme)
+        b.step_graph(dt, 2 * chunk, scheme, chunk=chunk)
+        assert np.array_equal(a.get_state(), b.get_state()), rep
+    a.step(0.5 * dt, 7, scheme)
+    b.step_graph(0.5 * dt, 7, scheme, chunk=chunk)
+    assert np.array_equal(a.get_state(), b.get_state())
+    ref = O.Bgk1d(12, 2, vs.u, vs.weights, 1.0 / 12).step(f, dt, 13, scheme)
+    b.set_state(f)
+    b.step_graph(dt, 13, scheme, chunk=chunk)
+    assert rel_l2(b.get_state(), ref) <= STATE_TOL
+    a.close()
+    b.close()
+
+
+def test_bgk_wave_to_t_end_graph_replayed():
+    """The script's run (20 cells, 100 velocities, Midpoint, 1000 steps) as ten replays of a
+    100-step graph: the bits of the eager run."""
+    ps, vs, f0 = script_case()
+    dt = 0.1 * (1.0 / 20) / 5.0
+    a, b = bgk.BGK1D(ps, vs), bgk.BGK1D(ps, vs)
+    a.set_state(f0)
+    b.set_state(f0)
+    a.step(dt, 1000, "midpoint")
+    b.step_graph(dt, 1000, "midpoint", chunk=100)
+    assert np.array_equal(a.get_state(), b.get_state())
+    a.close()
+    b.close()
+
+
+RESIDENT_CASES = [(20, 2, 100, 1), (7, 5, 33, 1), (2, 1, 5, 1), (5, 8, 20, 1), (13, 3, 17, 3)]
+
+
+@pytest.mark.parametrize("scheme", ["ssprk3", "midpoint", "euler"])
+@pytest.mark.parametrize("nx,deg,nu,nm", RESIDENT_CASES)
+def test_resident_path_equals_streaming_path(nx, deg, nu, nm, scheme):
+    """The LDS-resident path (a workgroup per member, up to 256 steps per launch) gives the bits of
+    the streaming path over 300 steps (two resident launches), for one member and a 3-member tau
+    ensemble, degrees 1-8, fewer velocities than the eight velocity slices too."""
+    ps = frhip.FRPSpace1D(0.0, 1.0, nx, deg)
+    vs = bgk.VSpace1D(-5.0, 5.0, nu)
+    f0 = frhip.ics.perturbed(bgk.bgk_wave_initial(ps, vs), seed=nx + deg, amplitude=0.05)
+    taus = [1e-2, 3e-2, 5e-3][:nm]
+    flat = np.concatenate([frhip.ics.perturbed(f0, seed=m, amplitude=0.02).ravel(order="F") for m in range(nm)])
+    res, st = bgk.BGK1D(ps, vs, tau=taus if nm > 1 else taus[0]), bgk.BGK1D(ps, vs, tau=taus if nm > 1 else taus[0])
+    res.set_path("resident")
+    st.set_path("stream")
+    assert res.path(scheme) == "resident" and st.path(scheme) == "stream"
+    dt = 0.1 * (1.0 / nx) / (5.0 * (deg + 1) ** 2)
+    for g in (res, st):
+        g.set_state(flat)
+        g.step(dt, 300, scheme)
+    a, b = res.get_state(), st.get_state()
+    assert np.all(np.isfinite(b))
+    assert np.array_equal(a, b), float(np.max(np.abs(a - b)))
+    res.close()
+    st.close()
+
+
+def test_resident_path_ensemble_of_script_members():
+    """64 members of the script's case with their own tau (one resident workgroup each) over 40
+    Midpoint steps: the bits of the streaming path, every member finite."""
+    ps, vs, f0 = script_case()
+    taus = list(np.exp(np.random.default_rng(8).uniform(np.log(3e-3), np.log(1e-1), 64)))
+    flat = np.concatenate([frhip.ics.perturbed(f0, seed=m, amplitude=0.02).ravel(order="F") for m in range(64)])
+    got = []
+    for path in ("stream", "resident"):
+        g = bgk.BGK1D(ps, vs, tau=taus)
+        g.set_path(path)
+        g.set_state(flat)
+        g.step(1e-3, 40, "midpoint")
+        got.append(g.get_state())
+        g.close()
+    assert np.all(np.isfinite(got[0]))
+    assert np.array_equal(got[0], got[1])
+
+
+def test_resident_path_selection():
+    """auto takes the resident path when a member's arrays fit the workgroup's LDS (the script's
+    case under every scheme) and the streaming path otherwise; forcing the resident path on a member
+    that does not fit is refused at step time."""
+    ps, vs, f0 = script_case()
+    g = bgk.BGK1D(ps, vs)
+    assert [g.path(s) for s in ("ssprk3", "midpoint", "euler")] == ["resident"] * 3
+    g.close()
+    big_ps = frhip.FRPSpace1D(0.0, 1.0, 2048, 2)
+    big = bgk.BGK1D(big_ps, bgk.VSpace1D(-5.0, 5.0, 40))
+    assert big.path("midpoint") == "stream"
+    big.set_path("resident")
+    big.set_state(bgk.bgk_wave_initial(big_ps, bgk.VSpace1D(-5.0, 5.0, 40)))
+    with pytest.raises(frhip.FRError):
+        big.step(1e-5, 1, "midpoint")
+    big.set_path("auto")
+    big.step(1e-5, 1, "midpoint")
+    big.close()
+
+
 def test_mol_on_device_arrays_side_stream():
     import torch
 
@@ -171,6 +285,8 @@ def test_create_rejects_bad_input():
     g = bgk.BGK1D(ps, vs)
     with pytest.raises(frhip.FRError):
         g.step(1e-3, 1, "rk4")
+    with pytest.raises(frhip.FRError):
+        g.step_graph(1e-3, 4, "rk4")
     with pytest.raises(ValueError):
         g.set_state(np.zeros(7))
     g.close()
