@@ -169,30 +169,35 @@ __device__ __forceinline__ void chunk1d(const Prob1D &pb, int first, const doubl
 // instance, whose register budget is that of those degrees (119 VGPRs for Euler); sets with
 // higher degrees (up to 8) the NMAX = NSP_MAX one (179 VGPRs, fewer waves per SIMD).
 template <int EQ, int FLUX, int NMAX>
+__device__ __forceinline__ void stage_chunk(const Prob1D &pb, int first, const double *__restrict__ ops,
+                                            const Phys1D &ph, const Stage1D &sa, double *sh) {
+  const double *o = ops + OPS_STRIDE * (pb.nsp - 2);
+  switch (pb.nsp) {
+    case 2: chunk1d<2, EQ, FLUX>(pb, first, o, ph, sa, sh); break;
+    case 3: chunk1d<3, EQ, FLUX>(pb, first, o, ph, sa, sh); break;
+    case 4: chunk1d<4, EQ, FLUX>(pb, first, o, ph, sa, sh); break;
+    case 5: chunk1d<5, EQ, FLUX>(pb, first, o, ph, sa, sh); break;
+    default:
+      if constexpr (NMAX > 5) {
+        switch (pb.nsp) {
+          case 6: chunk1d<6, EQ, FLUX>(pb, first, o, ph, sa, sh); break;
+          case 7: chunk1d<7, EQ, FLUX>(pb, first, o, ph, sa, sh); break;
+          case 8: chunk1d<8, EQ, FLUX>(pb, first, o, ph, sa, sh); break;
+          default: chunk1d<9, EQ, FLUX>(pb, first, o, ph, sa, sh); break;
+        }
+      }
+      break;
+  }
+}
+
+template <int EQ, int FLUX, int NMAX>
 __global__ __launch_bounds__(256) void fr1d_kernel(const Prob1D *__restrict__ probs, const Work1D *__restrict__ work,
                                                    int nwork, const double *__restrict__ ops, Phys1D ph, Stage1D sa) {
   constexpr int NV = (EQ == FR_EQ_EULER) ? 3 : 1;
   __shared__ double sh[3 * NV * 256];
   for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
     const Work1D wk = work[wi];
-    const Prob1D pb = probs[wk.prob];
-    const double *o = ops + OPS_STRIDE * (pb.nsp - 2);
-    switch (pb.nsp) {
-      case 2: chunk1d<2, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-      case 3: chunk1d<3, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-      case 4: chunk1d<4, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-      case 5: chunk1d<5, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-      default:
-        if constexpr (NMAX > 5) {
-          switch (pb.nsp) {
-            case 6: chunk1d<6, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-            case 7: chunk1d<7, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-            case 8: chunk1d<8, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-            default: chunk1d<9, EQ, FLUX>(pb, wk.first, o, ph, sa, sh); break;
-          }
-        }
-        break;
-    }
+    stage_chunk<EQ, FLUX, NMAX>(probs[wk.prob], wk.first, ops, ph, sa, sh);
   }
 }
 
@@ -321,13 +326,11 @@ __device__ __forceinline__ void diss1d_cell(double *__restrict__ u, long long n,
     }
 }
 
+// the dissipation pass over the interior elements of one work item
 template <int NMAX>
-__global__ __launch_bounds__(256) void fr1d_diss_kernel(const Prob1D *__restrict__ probs, const Work1D *__restrict__ work,
-                                                        int nwork, const double *__restrict__ ops, double *u, int mode,
-                                                        double g, double dt, int *count) {
-  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
-    const Work1D wk = work[wi];
-    const Prob1D pb = probs[wk.prob];
+__device__ __forceinline__ void diss_chunk(const Prob1D &pb, const Work1D &wk, const double *__restrict__ ops,
+                                           double *u, int mode, double g, double dt, int *count) {
+  {
     const int tid = threadIdx.x, T = blockDim.x;
     const int e = wk.first - 1 + tid;
     const bool interior = tid >= 1 && tid <= T - 2 && e < pb.ncell;
@@ -360,6 +363,71 @@ __global__ __launch_bounds__(256) void fr1d_diss_kernel(const Prob1D *__restrict
   }
 }
 
+template <int NMAX>
+__global__ __launch_bounds__(256) void fr1d_diss_kernel(const Prob1D *__restrict__ probs, const Work1D *__restrict__ work,
+                                                        int nwork, const double *__restrict__ ops, double *u, int mode,
+                                                        double g, double dt, int *count) {
+  for (int wi = blockIdx.x; wi < nwork; wi += gridDim.x) {
+    const Work1D wk = work[wi];
+    diss_chunk<NMAX>(probs[wk.prob], wk, ops, u, mode, g, dt, count);
+  }
+}
+
+// ------------------------------------------------------------------ per-member stepping
+// Small problem sets (the reference's example runs: one problem of 100 cells is one work item)
+// are bound by the latency of a launch per stage and per dissipation pass.  Here a workgroup takes
+// one member and runs a whole schedule of operations on it: each stage over the member's work
+// items in turn (stage_chunk, the batched kernel's code), each dissipation pass (diss_chunk) and
+// each copy, with a workgroup barrier after every operation, so the next one sees the whole
+// member's result.  The schedule is staged through LDS 32 operations at a time (one coalesced
+// read instead of a dependent scalar load per operation).  The bits are the batched launches'.
+struct Op1D {
+  int kind;     // kOpStage, kOpDiss, kOpCopy
+  int dmode;    // kOpDiss: 1 sensor + filter, 2 positivity limiter
+  double ddt;   // kOpDiss: the filter time step
+  Stage1D s;    // kOpStage
+  double *dst;  // kOpDiss: the state; kOpCopy: the destination
+  const double *src;  // kOpCopy
+};
+constexpr int kOpStage = 0, kOpDiss = 1, kOpCopy = 2;
+constexpr int kMemberOpsPerLaunch = 3072;
+
+template <int EQ, int FLUX, int NMAX>
+__global__ __launch_bounds__(256) void fr1d_member_kernel(const Prob1D *__restrict__ probs,
+                                                          const int *__restrict__ mcount,
+                                                          const double *__restrict__ ops, Phys1D ph,
+                                                          const Op1D *__restrict__ sched, int nops, int *count) {
+  constexpr int NV = (EQ == FR_EQ_EULER) ? 3 : 1;
+  constexpr int kWin = 32;                               // operations staged in LDS at a time
+  constexpr int kOpWords = (int)(sizeof(Op1D) / sizeof(unsigned long long));
+  static_assert(sizeof(Op1D) % sizeof(unsigned long long) == 0, "Op1D is copied as 8-byte words");
+  __shared__ double sh[3 * NV * 256];
+  __shared__ unsigned long long win[kWin * kOpWords];
+  const Prob1D pb = probs[blockIdx.x];
+  const int nitem = mcount[blockIdx.x];
+  // a member's work items start at cells 0, T - 2, 2 (T - 2), .. (the lists of Batch1D::build)
+  const int per = (int)blockDim.x - 2;
+  const long long mlen = (long long)pb.ncell * pb.nsp * NV;
+  const unsigned long long *sw = reinterpret_cast<const unsigned long long *>(sched);
+  for (int q0 = 0; q0 < nops; q0 += kWin) {
+    const int nq = min(kWin, nops - q0);
+    for (int i = threadIdx.x; i < nq * kOpWords; i += blockDim.x) win[i] = sw[(long long)q0 * kOpWords + i];
+    __syncthreads();
+    for (int q = 0; q < nq; ++q) {
+      const Op1D op = reinterpret_cast<const Op1D *>(win)[q];
+      if (op.kind == kOpStage) {
+        for (int k = 0; k < nitem; ++k) stage_chunk<EQ, FLUX, NMAX>(pb, k * per, ops, ph, op.s, sh);
+      } else if (op.kind == kOpDiss) {
+        for (int k = 0; k < nitem; ++k)
+          diss_chunk<NMAX>(pb, Work1D{(int)blockIdx.x, k * per}, ops, op.dst, op.dmode, ph.g, op.ddt, count);
+      } else {
+        for (long long v = threadIdx.x; v < mlen; v += blockDim.x) op.dst[pb.off + v] = op.src[pb.off + v];
+      }
+      __syncthreads();
+    }
+  }
+}
+
 // ------------------------------------------------------------------ batched problem set
 struct Batch1D {
   int eq = FR_EQ_EULER, flux = FR_FLUX_HLL, bc = FR_BC1_DIRICHLET, correction = FR_CORR_RADAU;
@@ -380,6 +448,15 @@ struct Batch1D {
   Work1D *d_work[3] = {nullptr, nullptr, nullptr};
   int nwork[3] = {0, 0, 0};
   std::vector<int> member_first[3], member_count[3];
+  int *d_mcount[3] = {nullptr, nullptr, nullptr};  // member_count on the device
+  // per-member stepping (fr1d_member_kernel): the path setting, the schedule being recorded by
+  // enqueue_steps (null: launch instead), its host copy (kept until the next call has synchronised)
+  // and its device copy
+  int path = FR1D_PATH_AUTO;
+  std::vector<Op1D> *rec = nullptr;
+  std::vector<Op1D> sched_host;
+  Op1D *d_sched = nullptr;
+  size_t sched_cap = 0;
   // the last step loop captured by fr_ens_run_graph, replayed while its parameters match
   hipGraphExec_t gexec = nullptr;
   int g_opts = -1, g_nsteps = 0, g_threads = 0, g_grid = -1;
@@ -430,6 +507,8 @@ struct Batch1D {
       nwork[v] = (int)wl.size();
       FR_HIP(hipMalloc(&d_work[v], wl.size() * sizeof(Work1D)));
       FR_HIP(hipMemcpy(d_work[v], wl.data(), wl.size() * sizeof(Work1D), hipMemcpyHostToDevice));
+      FR_HIP(hipMalloc(&d_mcount[v], probs.size() * sizeof(int)));
+      FR_HIP(hipMemcpy(d_mcount[v], member_count[v].data(), probs.size() * sizeof(int), hipMemcpyHostToDevice));
     }
     FR_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
     stream = own_stream;
@@ -447,6 +526,9 @@ struct Batch1D {
     if (d_count) (void)hipFree(d_count);
     for (auto *w : d_work)
       if (w) (void)hipFree(w);
+    for (int v = 0; v < 3; ++v)
+      if (d_mcount[v]) (void)hipFree(d_mcount[v]);
+    if (d_sched) (void)hipFree(d_sched);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     if (own_stream) (void)hipStreamDestroy(own_stream);
   }
@@ -541,7 +623,18 @@ struct Batch1D {
     auto stg = [&](const double *u0, const double *us, double *out, double a0, double a1, double b,
                    const double *ai, double *ao, double b2) {
       Stage1D s{us, out, u0, ai, ao, a0, a1, b, b2, 0};
+      if (rec) {
+        rec->push_back(Op1D{kOpStage, 0, 0.0, s, nullptr, nullptr});
+        return (int)FR_OK;
+      }
       return stage(s, mode, st);
+    };
+    auto dis = [&](double *u, int dmode, double ddt) {
+      if (rec) {
+        rec->push_back(Op1D{kOpDiss, dmode, ddt, Stage1D{}, u, nullptr});
+        return (int)FR_OK;
+      }
+      return diss_launch(u, dmode, ddt, mode, st);
     };
     const bool filt = (opts & FR_ENS_FILTER) != 0, lim = (opts & FR_ENS_LIMIT) != 0;
     for (int n = 0; n < nsteps; ++n) {
@@ -552,12 +645,12 @@ struct Batch1D {
           // cell), so the last stage never writes the u^n it reads; the call's last step writes U
           double *const X = (n & 1) ? S1 : U, *const Y = (n & 1) ? U : S1;
           double *const out = n == nsteps - 1 ? U : Y;
-          if (filt && (rc = diss_launch(X, 1, dt, mode, st))) return rc;
-          if (lim && (rc = diss_launch(X, 2, 0.0, mode, st))) return rc;
+          if (filt && (rc = dis(X, 1, dt))) return rc;
+          if (lim && (rc = dis(X, 2, 0.0))) return rc;
           if ((rc = stg(X, X, Y, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
-          if (lim && (rc = diss_launch(Y, 2, 0.0, mode, st))) return rc;
+          if (lim && (rc = dis(Y, 2, 0.0))) return rc;
           if ((rc = stg(X, Y, S2, 0.75, 0.25, 0.25 * dt, nullptr, nullptr, 0.0))) return rc;
-          if (lim && (rc = diss_launch(S2, 2, 0.0, mode, st))) return rc;
+          if (lim && (rc = dis(S2, 2, 0.0))) return rc;
           if ((rc = stg(X, S2, out, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0 * dt, nullptr, nullptr, 0.0))) return rc;
           break;
         }
@@ -571,10 +664,10 @@ struct Batch1D {
           // consecutive steps alternate the state between U and S2; the call's last step writes U
           double *const X = (n & 1) ? S2 : U;
           double *const out = n == nsteps - 1 ? U : ((n & 1) ? U : S2);
-          if (filt && (rc = diss_launch(X, 1, dt, mode, st))) return rc;
-          if (lim && (rc = diss_launch(X, 2, 0.0, mode, st))) return rc;
+          if (filt && (rc = dis(X, 1, dt))) return rc;
+          if (lim && (rc = dis(X, 2, 0.0))) return rc;
           if ((rc = stg(X, X, S1, 0.0, 1.0, 0.5 * dt, nullptr, nullptr, 0.0))) return rc;
-          if (lim && (rc = diss_launch(S1, 2, 0.0, mode, st))) return rc;
+          if (lim && (rc = dis(S1, 2, 0.0))) return rc;
           if ((rc = stg(X, S1, out, 1.0, 0.0, dt, nullptr, nullptr, 0.0))) return rc;
           break;
         }
@@ -583,10 +676,16 @@ struct Batch1D {
           // copies it back once
           double *const X = (n & 1) ? S1 : U, *const Y = (n & 1) ? U : S1;
           if ((rc = stg(X, X, Y, 0.0, 1.0, dt, nullptr, nullptr, 0.0))) return rc;
-          if (n == nsteps - 1 && Y != U) FR_HIP(hipMemcpyAsync(U, Y, len * sizeof(double), hipMemcpyDeviceToDevice, st));
+          if (n == nsteps - 1 && Y != U) {
+            if (rec)
+              rec->push_back(Op1D{kOpCopy, 0, 0.0, Stage1D{}, U, Y});
+            else
+              FR_HIP(hipMemcpyAsync(U, Y, len * sizeof(double), hipMemcpyDeviceToDevice, st));
+          }
           break;
         }
         case FR_RK_TSIT5: {
+          if (rec) return fail(FR_E_UNSUPPORTED, "fr1d: Tsit5 is stepped by the batched launches only");
           // k_s = L(Y_s) (residual launch), Y_1 = U, Y_{s+1} = U + dt sum_j a_{s+1,j} k_j, and
           // U <- U + dt sum_j b_j k_j (the last combination, in place)
           for (int sg = 0; sg < 6; ++sg) {
@@ -604,6 +703,74 @@ struct Batch1D {
       }
     }
     return FR_OK;
+  }
+
+  // the per-member path: FR1D_PATH_MEMBER, or FR1D_PATH_AUTO when no member has more than
+  // kMemberAutoItems work items (a workgroup runs a member's items one after the other, so the path
+  // pays off where a stage of the batched launch is a launch's latency over a few items).  Tsit5
+  // and the per-member launch mode always take the batched launches.
+  static constexpr int kMemberAutoItems = 2;
+  bool use_member(int scheme, int mode) const {
+    if (scheme == FR_RK_TSIT5 || mode != 0 || path == FR1D_PATH_BATCHED) return false;
+    if (path == FR1D_PATH_MEMBER) return true;
+    const auto &mc = member_count[tindex(threads)];
+    return !mc.empty() && *std::max_element(mc.begin(), mc.end()) <= kMemberAutoItems;
+  }
+
+  template <int NMAX>
+  void launch_member_eq(const Phys1D &ph, const Op1D *ops_, int n, hipStream_t st) const {
+    const int v = tindex(threads);
+    const dim3 g((unsigned)probs.size()), b((unsigned)threads);
+    if (eq == FR_EQ_ADVECTION)
+      hipLaunchKernelGGL((fr1d_member_kernel<FR_EQ_ADVECTION, FR_FLUX_HLL, NMAX>), g, b, 0, st, d_probs, d_mcount[v], d_ops, ph,
+                         ops_, n, d_count);
+    else if (flux == FR_FLUX_HLL)
+      hipLaunchKernelGGL((fr1d_member_kernel<FR_EQ_EULER, FR_FLUX_HLL, NMAX>), g, b, 0, st, d_probs, d_mcount[v], d_ops, ph,
+                         ops_, n, d_count);
+    else if (flux == FR_FLUX_RUSANOV)
+      hipLaunchKernelGGL((fr1d_member_kernel<FR_EQ_EULER, FR_FLUX_RUSANOV, NMAX>), g, b, 0, st, d_probs, d_mcount[v], d_ops, ph,
+                         ops_, n, d_count);
+    else
+      hipLaunchKernelGGL((fr1d_member_kernel<FR_EQ_EULER, FR_FLUX_ROE, NMAX>), g, b, 0, st, d_probs, d_mcount[v], d_ops, ph,
+                         ops_, n, d_count);
+  }
+
+  // nsteps steps as one schedule run by fr1d_member_kernel (a workgroup per member), in launches of
+  // at most kMemberOpsPerLaunch operations
+  int member_steps(int scheme, double dt, int nsteps, hipStream_t st, int opts) {
+    // the previous call's upload may still read sched_host
+    FR_HIP(hipStreamSynchronize(st));
+    sched_host.clear();
+    rec = &sched_host;
+    int rc = enqueue_steps(scheme, 0, dt, nsteps, st, opts);
+    rec = nullptr;
+    if (rc) return rc;
+    if (sched_host.empty()) return FR_OK;
+    if (sched_host.size() > sched_cap) {
+      if (d_sched) FR_HIP(hipFree(d_sched));
+      d_sched = nullptr;
+      sched_cap = 0;
+      FR_HIP(hipMalloc(&d_sched, sched_host.size() * sizeof(Op1D)));
+      sched_cap = sched_host.size();
+    }
+    FR_HIP(hipMemcpyAsync(d_sched, sched_host.data(), sched_host.size() * sizeof(Op1D), hipMemcpyHostToDevice, st));
+    Phys1D ph{eq, bc, gamma, gamma - 1.0, a};
+    const int nops = (int)sched_host.size();
+    for (int q0 = 0; q0 < nops; q0 += kMemberOpsPerLaunch) {
+      const int n = std::min(kMemberOpsPerLaunch, nops - q0);
+      if (max_nsp <= 5)
+        launch_member_eq<5>(ph, d_sched + q0, n, st);
+      else
+        launch_member_eq<NSP_MAX>(ph, d_sched + q0, n, st);
+      FR_LAUNCHED("fr1d_member_kernel");
+    }
+    return FR_OK;
+  }
+
+  // the steps on the path use_member selects
+  int run_steps(int scheme, int mode, double dt, int nsteps, hipStream_t st, int opts = 0) {
+    if (use_member(scheme, mode)) return member_steps(scheme, dt, nsteps, st, opts);
+    return enqueue_steps(scheme, mode, dt, nsteps, st, opts);
   }
 
   // mode 1 sensor + filter, 2 positivity; n_out (optional) receives the two counters
@@ -726,7 +893,21 @@ int fr1d_step(fr1d_ctx *c, int scheme, double dt, int nsteps) {
   FR_REQUIRE(scheme >= FR_RK_SSPRK3 && scheme <= FR_RK_TSIT5, FR_E_INVALID, "fr1d_step: unknown scheme");
   int rc = c->ensure_stage_buffers(scheme == FR_RK_RK4, scheme == FR_RK_TSIT5);
   if (rc) return rc;
-  return c->enqueue_steps(scheme, 0, dt, nsteps, c->stream);
+  return c->run_steps(scheme, 0, dt, nsteps, c->stream);
+}
+
+int fr1d_set_path(fr1d_ctx *c, int path) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr1d_set_path: null context");
+  FR_REQUIRE(path >= FR1D_PATH_AUTO && path <= FR1D_PATH_MEMBER, FR_E_INVALID,
+             "fr1d_set_path: path must be FR1D_PATH_AUTO, _BATCHED or _MEMBER");
+  c->path = path;
+  return FR_OK;
+}
+
+int fr1d_get_path(const fr1d_ctx *c, int scheme, int *path) {
+  FR_REQUIRE(c && path, FR_E_INVALID, "fr1d_get_path: null argument");
+  *path = c->use_member(scheme, 0) ? FR1D_PATH_MEMBER : FR1D_PATH_BATCHED;
+  return FR_OK;
 }
 
 // ------------------------------------------------------------------ ensembles
@@ -792,9 +973,17 @@ int fr_ens_step(fr_ens_ctx *c, int mode, double dt, int nsteps) {
              "fr_ens_step: bad arguments");
   int rc = c->ensure_stage_buffers(false);
   if (rc) return rc;
-  rc = c->enqueue_steps(FR_RK_SSPRK3, mode, dt, nsteps, c->stream);
+  rc = c->run_steps(FR_RK_SSPRK3, mode, dt, nsteps, c->stream);
   if (rc) return rc;
   FR_HIP(hipStreamSynchronize(c->stream));
+  return FR_OK;
+}
+
+int fr_ens_set_path(fr_ens_ctx *c, int path) {
+  FR_REQUIRE(c, FR_E_INVALID, "fr_ens_set_path: null context");
+  FR_REQUIRE(path >= FR1D_PATH_AUTO && path <= FR1D_PATH_MEMBER, FR_E_INVALID,
+             "fr_ens_set_path: path must be FR1D_PATH_AUTO, _BATCHED or _MEMBER");
+  c->path = path;
   return FR_OK;
 }
 
@@ -821,7 +1010,7 @@ int fr_ens_run(fr_ens_ctx *c, int mode, int opts, double dt, int nsteps, int *n_
   if (!c->d_count) FR_HIP(hipMalloc(&c->d_count, 2 * sizeof(int)));
   FR_HIP(hipMemsetAsync(c->d_count, 0, 2 * sizeof(int), c->stream));
   const int scheme = (opts & FR_ENS_MIDPOINT) ? FR_RK_MIDPOINT : FR_RK_SSPRK3;
-  if ((rc = c->enqueue_steps(scheme, mode, dt, nsteps, c->stream, opts))) return rc;
+  if ((rc = c->run_steps(scheme, mode, dt, nsteps, c->stream, opts))) return rc;
   return ens_read_counts(c, "fr_ens_run", n_flagged, n_limited);
 }
 

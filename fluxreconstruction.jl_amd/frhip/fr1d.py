@@ -25,6 +25,9 @@ _lib.SIGNATURES.update({
     "fr1d_set_state": (C.c_int, [_vp, _dp]),
     "fr1d_get_state": (C.c_int, [_vp, _dp]),
     "fr1d_step": (C.c_int, [_vp, C.c_int, C.c_double, C.c_int]),
+    "fr1d_set_path": (C.c_int, [_vp, C.c_int]),
+    "fr1d_get_path": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
+    "fr_ens_set_path": (C.c_int, [_vp, C.c_int]),
     "fr_ens_create": (C.c_int, [C.POINTER(_vp), C.c_int, _ip, _ip, _dp, C.c_double, C.c_int, C.c_int]),
     "fr_ens_destroy": (C.c_int, [_vp]),
     "fr_ens_state_len": (C.c_size_t, [_vp]),
@@ -58,7 +61,7 @@ def _bind():
                  "fr1d_get_state", "fr1d_step", "fr_ens_create", "fr_ens_destroy", "fr_ens_state_len", "fr_ens_offset",
                  "fr_ens_set_state", "fr_ens_get_state", "fr_ens_step", "fr_ens_step_graph", "fr_ens_autotune",
                  "fr1d_sensor_filter", "fr1d_positivity", "fr_ens_sensor_filter", "fr_ens_positivity", "fr_ens_run",
-                 "fr_ens_run_graph"):
+                 "fr_ens_run_graph", "fr1d_set_path", "fr1d_get_path", "fr_ens_set_path"):
         res, args = _lib.SIGNATURES[name]
         fn = getattr(L, name)
         fn.restype = res
@@ -133,6 +136,20 @@ class FR1D:
 
     def step(self, dt: float, nsteps: int = 1, scheme: str = "ssprk3"):
         check(_lib.lib().fr1d_step(self.h, _lib.RK[scheme], float(dt), int(nsteps)))
+
+    PATHS = {"auto": 0, "batched": 1, "member": 2}
+
+    def set_path(self, path: str):
+        """'batched' (a launch per stage), 'member' (one workgroup runs the whole call's stages in
+        a few launches) or 'auto' (member when the problem has at most two work items, i.e. up to
+        508 cells at 256 threads).  Every path gives the same bits."""
+        check(_lib.lib().fr1d_set_path(self.h, self.PATHS[path]))
+
+    def path(self, scheme: str = "ssprk3") -> str:
+        """The path step() takes for `scheme` under the current setting."""
+        v = C.c_int()
+        check(_lib.lib().fr1d_get_path(self.h, _lib.RK[scheme], C.byref(v)))
+        return {1: "batched", 2: "member"}[v.value]
 
     def sensor_filter(self, dt: float) -> int:
         """example/euler_highlevel.jl:37-52 on the device state; returns the flagged-cell count."""
@@ -230,6 +247,11 @@ class Ensemble1D:
 
     def step(self, dt: float, nsteps: int = 1, looped: bool = False):
         check(_lib.lib().fr_ens_step(self.h, 1 if looped else 0, float(dt), int(nsteps)))
+
+    def set_path(self, path: str):
+        """Execution path of step() and run(): 'batched', 'member' (a workgroup per member) or
+        'auto' (member when no member has more than two work items); same bits on every path."""
+        check(_lib.lib().fr_ens_set_path(self.h, FR1D.PATHS[path]))
 
     def step_graph(self, dt: float, nsteps: int, nrep: int = 1):
         check(_lib.lib().fr_ens_step_graph(self.h, float(dt), int(nsteps), int(nrep)))

@@ -163,6 +163,45 @@ def bgk1d_script(steps=1000):
                                       "finite": bool(np.all(np.isfinite(est[0])))}}
 
 
+def fr1d_small(steps=1000):
+    """The reference's small 1D runs, wall time of the whole run on the batched path (a launch per
+    stage) and on the member path (one workgroup runs every stage, a launch per 3072 stages), from
+    the same state; both end in the same bits:
+    - advection, 100 cells, p = 2, RK4, dt = 0.1 dx (example/advection_lowlevel.jl's setup, 1000
+      steps to t = 1);
+    - Euler (HLL, Dirichlet), Sod's problem on 100 cells, p = 2, Midpoint, dt = 1e-4 (1000 steps)."""
+    from . import fr1d as _fr1d
+
+    out = {}
+    ps = FRPSpace1D(0.0, 1.0, 100, 2)
+    cases = {
+        "advection_100_p2_rk4": (lambda: _fr1d.FRAdvectionProblem(ps, 1.0, "period"),
+                                 np.asfortranarray(np.sin(2 * np.pi * ps.xpg)), 0.1 / 100, "rk4", 4),
+    }
+    x = ps.xpg
+    prim = np.stack([np.where(x < 0.5, 1.0, 0.125), np.zeros_like(x), np.where(x < 0.5, 0.5, 0.625)], -1)
+    cases["sod_100_p2_midpoint"] = (lambda: _fr1d.FREulerProblem(ps, 1.4, "dirichlet", "hll"),
+                                    np.asfortranarray(ics.prim_conserve1(prim, 1.4)), 1e-4, "midpoint", 2)
+    for name, (mk, u0, dt, scheme, nst) in cases.items():
+        res, states = {}, []
+        for path in ("batched", "member"):
+            g = mk()
+            g.set_path(path)
+            g.set_state(u0)
+            g.step(dt, 50, scheme)
+            g.set_state(u0)
+            g.get_state()
+            t0 = time.perf_counter()
+            g.step(dt, steps, scheme)
+            states.append(g.get_state())
+            el = time.perf_counter() - t0
+            res[path] = {"run_ms": el * 1e3, "stage_us": el / (nst * steps) * 1e6}
+            g.close()
+        out[name] = {**res, "member_speedup": res["batched"]["run_ms"] / res["member"]["run_ms"],
+                     "bitwise_equal": bool(np.array_equal(states[0], states[1]))}
+    return out
+
+
 def polar_stage(n=256, steps=20):
     """Cells with point-varying metrics: the p = 3 SSP-RK3 stage on the polar mesh
     CSpace2D(1, 6, n, 0, pi, n, 1, 1) (fr2d_set_metrics), Rusanov, ghost layer as data, a smooth
@@ -333,7 +372,7 @@ def vortex_grid(n=1024, steps=60, flux="rusanov"):
 
 def main():
     ap = argparse.ArgumentParser(description="run one measured workload and print one JSON line")
-    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "bgk1d", "bgk1d64", "bgk1d_script", "polar", "parallelogram",
+    ap.add_argument("workload", choices=["ensemble", "bgk", "bgk64", "bgk1d", "bgk1d64", "bgk1d_script", "fr1d_small", "polar", "parallelogram",
                                          "ops", "shock_vortex", "vortex_grid"])
     ap.add_argument("--n", type=int, default=None, help="vortex_grid (1024) / polar, parallelogram (256): cells per side")
     ap.add_argument("--steps", type=int, default=None)
@@ -349,6 +388,8 @@ def main():
         r = bgk(a.steps or 1, members=64)
     elif a.workload == "bgk1d":
         r = bgk1d(a.steps or 20)
+    elif a.workload == "fr1d_small":
+        r = fr1d_small(a.steps or 1000)
     elif a.workload == "bgk1d_script":
         r = bgk1d_script(a.steps or 1000)
     elif a.workload == "bgk1d64":

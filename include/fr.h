@@ -346,6 +346,20 @@ int fr1d_set_stream(fr1d_ctx *ctx, void *hip_stream);
 int fr1d_set_state(fr1d_ctx *ctx, const double *u);
 int fr1d_get_state(fr1d_ctx *ctx, double *u);
 int fr1d_step(fr1d_ctx *ctx, int scheme, double dt, int nsteps);
+/* Execution path of fr1d_step, fr_ens_step and fr_ens_run (same bits on every path; no
+ * counterpart in the reference, the path is this library's execution choice):
+ * FR1D_PATH_BATCHED: a launch per stage and per dissipation pass over all members' work items;
+ * FR1D_PATH_MEMBER: a workgroup per member runs the whole call's schedule (every stage,
+ * dissipation pass and copy, a workgroup barrier after each) in launches of up to 3072 operations;
+ * FR1D_PATH_AUTO (the default): the member path when no member has more than two work items of
+ * the context's workgroup size (up to 508 cells at 256 threads), else batched.  Tsit5,
+ * fr_ens_step's mode 1 and the graph entry points always take the batched launches. */
+#define FR1D_PATH_AUTO 0
+#define FR1D_PATH_BATCHED 1
+#define FR1D_PATH_MEMBER 2
+int fr1d_set_path(fr1d_ctx *ctx, int path);
+/* the path fr1d_step takes for this scheme under the current setting */
+int fr1d_get_path(const fr1d_ctx *ctx, int scheme, int *path);
 /* The per-step host loop of example/euler_highlevel.jl:37-52 on the device state (Euler only):
  * modal sensor on rho (su = u_hat[end]^2 / (sum u_hat^2 + 1e-6), shock_detector of
  * src/dissipation.jl:13 with S0 = -3 log10(deg), kappa = 4) and, on a shock, modal_filter!
@@ -370,6 +384,8 @@ int fr_ens_set_state(fr_ens_ctx *ctx, const double *u);  /* concatenated member 
 int fr_ens_get_state(fr_ens_ctx *ctx, double *u);
 /* mode 0: one batched launch per stage; 1: one launch per member per stage (looped baseline) */
 int fr_ens_step(fr_ens_ctx *ctx, int mode, double dt, int nsteps);
+/* the execution path of fr_ens_step / fr_ens_run (FR1D_PATH_*; see fr1d_set_path) */
+int fr_ens_set_path(fr_ens_ctx *ctx, int path);
 /* capture nsteps batched steps in a hipGraph and replay nrep times (fr_ens_run_graph, no options) */
 int fr_ens_step_graph(fr_ens_ctx *ctx, double dt, int nsteps, int nrep);
 /* The per-step host loop of example/euler_highlevel.jl:37-54 over every member, with SSP-RK3 steps

@@ -679,3 +679,66 @@ def test_tsit5_parity_dirichlet(deg):
     ref = O.tsit5(lambda v: O.fr1d_euler_rhs(v, ps.J, ops, 1.4, "rusanov", "dirichlet"), u0, dt, 9)
     assert np.linalg.norm(one - ref) / np.linalg.norm(ref) <= 1e-10
     assert np.array_equal(one[0], u0[0]) and np.array_equal(one[-1], u0[-1])
+
+
+@pytest.mark.parametrize("scheme", ["ssprk3", "rk4", "midpoint", "euler"])
+@pytest.mark.parametrize("eq,flux,bc,ncell,deg", [("euler", "hll", "dirichlet", 100, 2), ("euler", "rusanov", "period", 400, 3),
+                                                  ("euler", "roe", "period", 1000, 8), ("advection", None, "period", 100, 2),
+                                                  ("advection", None, "dirichlet", 509, 4)])
+def test_member_path_equals_batched_path(eq, flux, bc, ncell, deg, scheme):
+    """One workgroup running every stage of the call (a workgroup barrier between operations)
+    gives the bits of a launch per stage: 37 steps (Euler's odd count ends in the copy back), for
+    one to four 254-cell work items per problem (1000 cells forced onto the member path)."""
+    ps = frhip.FRPSpace1D(0.0, 1.0, ncell, deg)
+    if eq == "euler":
+        u0 = euler_state(ncell, deg, seed=ncell + deg)
+        mk = lambda: frhip.FREulerProblem(ps, 1.4, bc, flux)  # noqa: E731
+        dt = 0.02 / ncell / (deg + 1)
+    else:
+        u0 = np.asfortranarray(np.sin(2 * np.pi * ps.xpg) + 0.1 * np.cos(6 * np.pi * ps.xpg))
+        mk = lambda: frhip.FRAdvectionProblem(ps, 1.0, bc)  # noqa: E731
+        dt = 0.05 / ncell / (deg + 1)
+    out = []
+    for path in ("batched", "member"):
+        g = mk()
+        g.set_path(path)
+        assert g.path(scheme) == path
+        g.set_state(u0)
+        g.step(dt, 37, scheme)
+        out.append(g.get_state())
+        g.close()
+    assert np.all(np.isfinite(out[0]))
+    assert np.array_equal(out[0], out[1]), float(np.max(np.abs(out[0] - out[1])))
+
+
+def test_member_path_selection():
+    ps = frhip.FRPSpace1D(0.0, 1.0, 100, 2)
+    g = frhip.FREulerProblem(ps, 1.4, "dirichlet", "hll")
+    assert g.path("ssprk3") == "member" and g.path("midpoint") == "member"
+    assert g.path("tsit5") == "batched"
+    g.set_path("batched")
+    assert g.path("ssprk3") == "batched"
+    g.close()
+    big = frhip.FREulerProblem(frhip.FRPSpace1D(0.0, 1.0, 509, 2), 1.4, "dirichlet", "hll")
+    assert big.path("ssprk3") == "batched"  # three work items
+    big.close()
+
+
+@pytest.mark.parametrize("scheme", ["ssprk3", "midpoint"])
+def test_member_path_ensemble_loop_equals_batched(scheme):
+    """The reference's per-step loop (filter before every step, limiter on every stage input) over
+    20 members of 64-600 cells on the member path and batched: the same bits and the same counts."""
+    ncell, deg, dx, members = _ensemble(20, seed=21)
+    dt = 0.05 * float(np.min(dx)) / _max_speed(members)
+    res = []
+    for path in ("batched", "member"):
+        e = frhip.Ensemble1D(ncell, deg, dx, 1.4, "rusanov", "dirichlet")
+        e.set_path(path)
+        e.set_state(e.pack(members))
+        counts = e.run(dt, 15, filter=True, limit=True, scheme=scheme)
+        e.step(dt, 3)
+        res.append((e.get_state(), counts))
+        e.close()
+    assert res[0][1] == res[1][1] and res[0][1][0] > 0
+    assert np.all(np.isfinite(res[0][0]))
+    assert np.array_equal(res[0][0], res[1][0])
