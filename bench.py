@@ -115,6 +115,14 @@ def bench_bgk1d():
             "script_case_20x100_midpoint_1000_steps": workloads.bgk1d_script()}
 
 
+def bench_fr1d_small():
+    """The reference's small 1D runs (100 cells, p = 2, 1000 steps: advection RK4, Sod Midpoint)
+    with a launch per stage and on the per-member path (frhip.workloads)."""
+    from frhip import workloads
+
+    return {"workload": "1D FR 100 cells p=2, 1000 steps", "unit": "ms per run", **workloads.fr1d_small()}
+
+
 def bench_polar():
     """The p = 3 stage on a 256 x 256 polar mesh with point-varying metrics (frhip.workloads)."""
     from frhip import workloads
@@ -321,6 +329,7 @@ def main():
     if not args.no_extras:
         for key, fn in (("vortex_larger_grids", bench_grids), ("ensemble_riemann512", bench_ensemble),
                         ("bgk_128sq_p2_v24", bench_bgk), ("bgk1d_8192_v256", bench_bgk1d),
+                        ("fr1d_small_100cells", bench_fr1d_small),
                         ("polar_256sq_p3", bench_polar), ("parallelogram_256sq_p3", bench_parallelogram),
                         ("sensor_shock_vortex", bench_sensor)):
             try:
